@@ -1,0 +1,50 @@
+"""Named inputs of the golden force vectors (shared by make_golden.py and the tests)."""
+from __future__ import annotations
+
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(HERE)), "photons-2.0_amd"))
+
+import ic  # noqa: E402
+
+
+def _clustered(nside, nside_pm, rc_factor=6.0, seed=1, theta=0.4, maxleaf=32):
+    n = nside ** 3
+    box = 500000.0 * (n / 256.0 ** 3) ** (1.0 / 3.0)
+    pos, _ = ic.clustered(n, box, seed=seed)
+    return pos, ic.force_params(n, box, nside_pm, rc_factor=rc_factor, theta=theta, maxleaf=maxleaf)
+
+
+def _lattice(rc_factor, maxleaf=32, theta=0.4):
+    pos, prm = ic.workload_lattice32(rc_factor=rc_factor)
+    prm.maxleaf = maxleaf
+    prm.theta = theta
+    return pos, prm
+
+
+def _lcdm():
+    pos = np.load(os.path.join(HERE, "lcdm_pos.npy")).astype(np.float64)
+    return pos, ic.force_params(len(pos), 100000.0, 64, rc_factor=6.0)
+
+
+FORCE_CASES = {
+    # 32^3 lattice + 0.2-cell Gaussian, 100 Mpc/h, PM 64^3, theta 0.4, leaf 32
+    "lattice32_rc4.5": lambda: _lattice(4.5),
+    "lattice32_rc6": lambda: _lattice(6.0),
+    "lattice32_leaf8_th0.5": lambda: _lattice(6.0, maxleaf=8, theta=0.5),
+    # clustered (half in Plummer spheres), density of 256^3 in 500 Mpc/h
+    "clus16_pm32": lambda: _clustered(16, 32),
+    "clus24_pm48": lambda: _clustered(24, 48),
+    "clus32_pm32": lambda: _clustered(32, 32),
+    "clus24_pm48_th0.3": lambda: _clustered(24, 48, theta=0.3, seed=7),
+    # the reference's own demo initial conditions (demo/ic_lcdm.gdt2)
+    "lcdm_demo": _lcdm,
+}
+
+
+def case_inputs(name):
+    return FORCE_CASES[name]()
