@@ -1,0 +1,195 @@
+/*
+ * photons_fmm.h -- C ABI of the MI355X short-range FMM force library
+ * (libphotons_fmm.so).  Plain pointers and sizes only.
+ *
+ * Two layers:
+ *  1. the native handle API (photons_*): a context owns device-resident particles,
+ *     tree, expansions and interaction lists; one call runs the whole short-range
+ *     force on the GPU;
+ *  2. the drop-in layer: the reference's own entry points and globals
+ *     (inc/fmm.h, inc/operator.h, inc/kernels.h, inc/photoNs.h), implemented on
+ *     top of layer 1, so the reference driver's force call links unchanged.
+ *
+ * All functions of layer 1 return 0 on success and a negative code on failure;
+ * photons_last_error() describes the last failure of the calling thread.
+ * Nothing here falls back to the CPU: without a usable GPU every call fails.
+ */
+#ifndef PHOTONS_FMM_H
+#define PHOTONS_FMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================= */
+/* 1. native handle API                                                     */
+/* ======================================================================= */
+
+enum { PHOTONS_FP64 = 0, PHOTONS_FP32_MIXED = 1 };
+
+/* Force parameters; the names are the reference globals they stand for
+ * (inc/photoNs.h:20-45, 125-126; src/initial.c:316-345).                   */
+typedef struct {
+    double boxsize;   /* BOXSIZE: periodic box [0, boxsize)^3                 */
+    double mass;      /* MASSPART                                             */
+    double soft;      /* SoftenScale                                          */
+    double rs;        /* splitRadius                                          */
+    double rc;        /* cutoffRadius                                         */
+    double theta;     /* open_angle                                           */
+    int32_t maxleaf;  /* MAXLEAF (>= 2)                                       */
+    int32_t order;    /* expansion order p: 3 (reference), 4 or 5             */
+    int32_t periodic; /* 1: the 26 image walks of fmm_ext (PERIODIC_CONDITION)*/
+    int32_t precision;/* PHOTONS_FP64 or PHOTONS_FP32_MIXED                   */
+} photons_params;
+
+typedef struct {
+    int64_t n;                 /* particles                                         */
+    int32_t nleaf, nnode;      /* cells                                             */
+    int32_t depth;             /* levels below the root                             */
+    int32_t pad;
+    int64_t p2p_pairs;         /* (target leaf, source leaf, shift) entries         */
+    int64_t p2p_local_pairs;   /* idxP2P: those with shift 0                        */
+    int64_t m2l_pairs;         /* (target cell, source cell, shift) entries         */
+    int64_t m2l_local_pairs;   /* idxM2L                                            */
+    int64_t p2p_interactions;  /* particle pairs i != j evaluated                   */
+    int64_t p2p_image_interactions; /* p2p_count_remote                             */
+} photons_stats;
+
+/* Phases of one force evaluation, milliseconds of GPU time of the last call
+ * (filled when timing is enabled with photons_set_timing).                */
+enum {
+    PHOTONS_PH_UPLOAD = 0, PHOTONS_PH_TREE, PHOTONS_PH_UPWARD, PHOTONS_PH_WALK,
+    PHOTONS_PH_M2L, PHOTONS_PH_L2L, PHOTONS_PH_P2P, PHOTONS_PH_DOWNLOAD, PHOTONS_NPHASE
+};
+
+typedef struct photons_ctx photons_ctx;
+
+const char *photons_last_error(void);
+int photons_version(void);
+
+/* stream: a hipStream_t to run on, or NULL for a private stream. */
+int photons_create(photons_ctx **ctx, const photons_params *prm, int device, void *stream);
+int photons_destroy(photons_ctx *ctx);
+int photons_set_params(photons_ctx *ctx, const photons_params *prm);
+int photons_get_params(const photons_ctx *ctx, photons_params *prm);
+int photons_set_timing(photons_ctx *ctx, int enabled);
+int photons_get_phase_ms(const photons_ctx *ctx, double *ms /* [PHOTONS_NPHASE] */);
+int photons_get_stats(const photons_ctx *ctx, photons_stats *st);
+
+/* One short-range force evaluation (tree build through L2P) of n particles.
+ * pos and acc are [n][3] doubles in the caller's order; acc is overwritten with
+ * the short-range acceleration without G (the reference's part[].acc, which the
+ * caller zeroes before the force call: src/photoNs.c:222-226).
+ *   _host:   host arrays (copies in and out over PCIe),
+ *   _device: device arrays on the context's device, work queued on its stream;
+ *            returns after the work is queued, photons_sync() waits.        */
+int photons_force_host(photons_ctx *ctx, int64_t n, const double *pos, double *acc);
+int photons_force_device(photons_ctx *ctx, int64_t n, const double *d_pos, double *d_acc);
+int photons_sync(photons_ctx *ctx);
+
+/* The same force call in two steps, as the reference splits it:
+ *   photons_prepare_host   upload, tree, P2M and M2M   (fmm_prepare)
+ *   photons_evaluate_host  walk, M2L, L2L, P2P and L2P (fmm_task + fmm_ext);
+ *                          acc is written in tree order (photons_tree_perm). */
+int photons_prepare_host(photons_ctx *ctx, int64_t n, const double *pos);
+int photons_evaluate_host(photons_ctx *ctx, double *acc_tree_order);
+
+/* Inspection of the last force call (tree order; see photons_tree_perm).
+ * Leaves are numbered in depth-first order, nodes in pre-order; a cell id is
+ * the leaf id for a leaf and nleaf + node id for a node; list entries are
+ * (target cell, source cell, shift id), shift id 0 = home box and 1..26 the
+ * images in the order of src/fmm.c:1029-1044.                               */
+int photons_tree_perm(const photons_ctx *ctx, int32_t *perm /* [n]: caller index of slot i */);
+int photons_tree_cells(const photons_ctx *ctx, int32_t *npart, int32_t *ipart, int32_t *son /* [ncell][2] */,
+                       double *center /* [ncell][3] */, double *width /* [ncell][3] */);
+int photons_p2p_list(const photons_ctx *ctx, int64_t cap, int32_t *tgt, int32_t *src, int8_t *shift, int64_t *count);
+int photons_m2l_list(const photons_ctx *ctx, int64_t cap, int32_t *tgt, int32_t *src, int8_t *shift, int64_t *count);
+/* expansions of every cell, [ncell][nterms(p)] doubles (FP32-mixed values are
+ * returned converted to kpc/h units)                                         */
+int photons_expansions(const photons_ctx *ctx, double *M, double *L);
+
+/* Batched operators on device arrays (order p, FP64), one item per thread.
+ * Layouts: positions [n][3]; expansions [count][nterms(p)]; offsets [count][3].
+ *   p2m:  M[k] = P2M of particles first[k] .. first[k]+npart[k]-1 about center[k]
+ *   m2m:  tM[k] += shift(M[k]) by d[k] = parent - child
+ *   m2l:  L[k] += M2L(M[k]) at R[k] = c_target - c_source
+ *   l2l:  tL[k] += shift(L[k]) by d[k] = child - parent
+ *   l2p:  acc[i] += grad of L[owner[i]] at pos[i] - center[owner[i]]        */
+int photons_p2m_batch(photons_ctx *ctx, int order, int count, const double *d_pos, const int32_t *d_first,
+                      const int32_t *d_npart, const double *d_center, double mass, double *d_M);
+int photons_m2m_batch(photons_ctx *ctx, int order, int count, const double *d_d, const double *d_M, double *d_tM);
+int photons_m2l_batch(photons_ctx *ctx, int order, int count, double rs, const double *d_R, const double *d_M,
+                      double *d_L);
+int photons_l2l_batch(photons_ctx *ctx, int order, int count, const double *d_d, const double *d_L, double *d_tL);
+int photons_l2p_batch(photons_ctx *ctx, int order, int n, const double *d_pos, const int32_t *d_owner,
+                      const double *d_center, const double *d_L, double *d_acc);
+
+/* ======================================================================= */
+/* 2. drop-in layer: the reference's entry points                            */
+/* ======================================================================= */
+/* Globals read and written by the force call have the reference's names and
+ * types (inc/photoNs.h, inc/typesdef.h); a caller built against the reference
+ * headers shares them.  Only p = 3 (NMULTI 20) layouts are exported here.
+ * The layer keeps one GPU context (device 0, FP64) behind the globals;
+ * photons_compat_set_precision() switches it to FP32-mixed.               */
+
+typedef struct {
+    double pos[3];
+    double acc[3];
+    double vel[3];
+    double acc_pm[3];
+} photons_body; /* Body, inc/typesdef.h:48-57 */
+
+int photons_compat_set_precision(int precision);
+photons_ctx *photons_compat_context(void);
+
+/* the force call, inc/fmm.h:11-22 and src/fmm.c:180-264, 329-360, 719-1087:
+ *   fmm_construct   domain box [0, BOXSIZE)^3 (one rank)
+ *   fmm_prepare     upload part[].pos, build the tree, P2M and M2M on the GPU;
+ *                   part[] is permuted into tree order like the reference
+ *   fmm_task        dual-tree walk, P2P and M2L on the GPU (home box)
+ *   fmm_ext         the 26 periodic images, L2L and L2P; part[].acc += result
+ *   fmm_deconstruct release the host mirrors (device buffers stay for reuse) */
+void fmm_construct(void);
+void fmm_prepare(void);
+void fmm_task(void);
+void fmm_ext(void);
+void fmm_deconstruct(void);
+void build_localtree(void);
+
+/* operators, inc/operator.h:6-19 and inc/photoNs.h:325-330 (p = 3) */
+void p2m(int iPart, int npart, double center[3], double M[]);
+void m2m(double dx, double dy, double dz, double M[], double tM[]);
+void m2l(double dx, double dy, double dz, double M[], double toL[]);
+void l2l(double dx, double dy, double dz, double L[], double toL[]);
+void l2p(int ileaf);
+void walk_m2m(int iNode);
+void walk_l2l(int inode);
+int acceptance(double wi[3], double wj[3], double dist[3]);
+
+/* pair kernels, inc/kernels.h:4-8 (declared, never defined by the reference):
+ * part[i].acc += P2P from the particles of leaf jnode onto those of leaf inode
+ *   p2p_kernel           FP64
+ *   p2p_kernel_inleaf    FP64, skipping i == j (inode == jnode)
+ *   p2p_kernel_rinvsqrt  FP32-mixed (rsqrt pair kernel)                       */
+void p2p_kernel(int inode, int jnode);
+void p2p_kernel_inleaf(int inode, int jnode);
+void p2p_kernel_rinvsqrt(int inode, int jnode);
+
+/* dual-tree walks, src/fmm.c:406,569 and src/remotes.c:213,405: from the pair
+ * (im, jm) (reference index spaces), evaluate the P2P (resp. M2L) work the
+ * reference walk would emit: P2P into part[].acc, M2L into the cell L.  The
+ * _ext forms use the image set by the last fmm_remote() call.              */
+void walk_task_p2p(int im, int jm);
+void walk_task_m2l(int im, int jm);
+void walk_task_p2p_ext(int im, int jm);
+void walk_task_m2l_ext(int im, int jm);
+/* image exchange, src/remotes.c:684-751 (one rank: idx = 0 only) */
+void fmm_remote(int idx, double shift[3]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PHOTONS_FMM_H */

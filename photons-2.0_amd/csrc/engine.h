@@ -1,0 +1,129 @@
+// engine.h -- device-resident state of one short-range force context (internal).
+//
+// HBM layout (all arrays indexed in tree order unless noted):
+//   particles  px, py, pz (double SoA), pidx (int32: caller index of each slot),
+//              ox, oy, oz (float SoA: offset from the leaf centre, FP32-mixed path)
+//   cells      leaves first (depth-first order, = increasing particle order), then
+//              nodes in pre-order: CellGeom (centre, width; 48 B) and CellInfo (32 B)
+//   expansions M, L: ncell x nterms(p) doubles (FP64 path) or floats in units of
+//              rs (FP32-mixed path), one row per cell
+//   lists      P2P: CSR over target leaves; M2L: CSR over target cells; an entry is
+//              source cell | shift id << 27 (shift id 0 = home box, 1..26 images)
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/photons_fmm.h"
+
+namespace photons {
+
+constexpr int kShiftBits = 27;
+constexpr int kCellMask = (1 << kShiftBits) - 1;
+// list entries: source cell in the low 27 bits, shift id (0..26) in the top 5 bits
+__host__ __device__ inline int32_t pack_entry(int cell, int sid)
+{
+    return (int32_t)((uint32_t)cell | ((uint32_t)sid << kShiftBits));
+}
+__host__ __device__ inline int entry_cell(int32_t e) { return (int)((uint32_t)e & (uint32_t)kCellMask); }
+__host__ __device__ inline int entry_shift(int32_t e) { return (int)((uint32_t)e >> kShiftBits); }
+constexpr int kWalkStack = 160;  // entries per thread; needs 2*depth + 2
+constexpr int kMaxDepth = (kWalkStack - 2) / 2;
+
+struct CellGeom {
+    double c[3];
+    double w[3];
+};
+
+struct CellInfo {
+    int32_t npart, ipart;  // particle range (tree order)
+    int32_t son0, son1;    // child cell ids, -1 for leaves
+    int32_t dfs, dsize;    // pre-order rank over all cells and subtree cell count
+    int32_t parent;        // parent cell id, -1 at the root
+    int32_t depth;
+};
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+struct Timing {
+    bool enabled = false;
+    hipEvent_t ev[16];
+    int nev = 0;
+};
+
+struct Ctx {
+    photons_params prm;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    int64_t n = 0;
+    int nleaf = 0, nnode = 0, ncell = 0, nlevel = 0, maxdepth = 0;
+    std::vector<int> level_off;  // nodes of depth d: level_nodes[level_off[d] .. level_off[d+1])
+
+    // persistent device buffers, grown on demand
+    DevBuf px, py, pz, pidx;       // tree-ordered particles
+    DevBuf qx, qy, qz, qidx;       // build double buffer
+    DevBuf ox, oy, oz;             // FP32 offsets
+    DevBuf acc;                    // tree-order acceleration [3][n] doubles
+    DevBuf geom, info;             // CellGeom / CellInfo [ncell]
+    DevBuf level_nodes;            // node cell ids by depth
+    DevBuf M, L;                   // expansions
+    DevBuf p2p_off, p2p_src, m2l_off, m2l_src, cnt_tmp;
+    DevBuf scan_tmp;
+    // build temporaries (level order)
+    int tcap = 0, lcap = 0;        // capacities of the level-order temporaries
+    DevBuf t_start, t_count, t_split, t_lo, t_hi, t_son, t_depth, t_parent;
+    DevBuf l_start, l_count, l_lo, l_hi, l_parent;
+    DevBuf seg_chunk, chunk_part, chunk_rcnt, seg_nleft, seg_flag, seg_nn, seg_nl, seg_non, seg_lon;
+    DevBuf t_size, t_lsize, t_dfs, t_lfirst, t_final, l_final;
+    DevBuf err;                    // device error flag (int)
+    DevBuf stats;                  // device counters (int64)
+    DevBuf pinned;                 // small host-pinned staging
+
+    int64_t n_p2p = 0, n_m2l = 0;  // list lengths
+    int64_t n_p2p_local = 0, n_m2l_local = 0;
+
+    Timing timing;
+    double phase_ms[16] = {0};
+    int phase_count = 0;
+
+    std::string err_msg;
+};
+
+// ---- helpers shared by the .hip files
+#define PH_CHECK(x)                                                                 \
+    do {                                                                            \
+        hipError_t e_ = (x);                                                        \
+        if (e_ != hipSuccess) {                                                     \
+            throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+        }                                                                           \
+    } while (0)
+
+void *grow(DevBuf &b, size_t bytes);
+void upload_positions(Ctx &c, const double *d_pos_aos);  // AoS [n][3] -> px, py, pz, pidx
+template <typename T>
+T *dptr(DevBuf &b)
+{
+    return reinterpret_cast<T *>(b.p);
+}
+
+// phases (tree.hip, walk.hip, kernels.hip)
+void build_tree(Ctx &c);                 // px.. sorted, cells and levels built
+void compute_offsets_f32(Ctx &c);        // ox, oy, oz
+void upward_pass(Ctx &c);                // P2M + M2M (+ FP32 conversion)
+void run_walk(Ctx &c, int shift_lo, int shift_hi);  // lists for shifts [lo, hi)
+void m2l_pass(Ctx &c, bool accumulate);  // L (+)= M2L over the M2L list
+void downward_pass(Ctx &c);              // L2L
+void p2p_l2p_pass(Ctx &c, bool with_l2p, bool accumulate);  // acc (+)= P2P (+ L2P)
+
+int nterms_of(int order);
+void phase_mark(Ctx &c);                 // records a timing event (when enabled)
+
+}  // namespace photons

@@ -1,0 +1,728 @@
+// engine.hip -- context lifetime, force pipeline and the native C API.
+//
+// One force evaluation (photons_force_device), all on the context's stream:
+//   upload    AoS positions -> SoA tree buffers
+//   tree      kd build (tree.hip)
+//   upward    P2M + M2M (kernels.hip)
+//   walk      per-target dual-tree walk -> P2P and M2L lists (walk.hip)
+//   m2l       M2L over the list, one thread per target cell
+//   l2l       L2L level by level
+//   p2p       P2P + L2P, one wavefront per target leaf, result in caller order
+// The tree build and the walk read a few counts back to size the next step.
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+
+#include "engine.h"
+#include "fmm_math.h"
+
+namespace photons {
+void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, double *out);
+void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int start_source);
+void op_p2m(hipStream_t st, int order, int count, const double *pos, const int32_t *first, const int32_t *npart,
+            const double *center, double mass, double *M);
+void op_shift(hipStream_t st, int order, int kind, int count, const double *d, const double *X, double *Y);
+void op_m2l(hipStream_t st, int order, int count, double rs, const double *R, const double *M, double *L);
+void op_l2p(hipStream_t st, int order, int n, const double *pos, const int32_t *owner, const double *center,
+            const double *L, double *acc);
+
+void *grow(DevBuf &b, size_t bytes)
+{
+    if (bytes == 0) bytes = 1;
+    if (bytes <= b.cap) return b.p;
+    if (b.p) PH_CHECK(hipFree(b.p));
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = bytes + bytes / 8;
+    PH_CHECK(hipMalloc(&b.p, want));
+    b.cap = want;
+    return b.p;
+}
+
+static void release(DevBuf &b)
+{
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.cap = 0;
+}
+
+void phase_mark(Ctx &c)
+{
+    if (!c.timing.enabled || c.timing.nev >= 16) return;
+    PH_CHECK(hipEventRecord(c.timing.ev[c.timing.nev++], c.stream));
+}
+
+}  // namespace photons
+
+using namespace photons;
+
+struct photons_ctx {
+    Ctx c;
+    DevBuf io_pos, io_acc, op_a, op_b, op_c, op_d, op_e;
+};
+
+static thread_local std::string g_last_error;
+
+static int fail(const char *where, const std::exception &e)
+{
+    g_last_error = std::string(where) + ": " + e.what();
+    return -1;
+}
+
+static int fail_msg(const std::string &m)
+{
+    g_last_error = m;
+    return -2;
+}
+
+static std::string check_params(const photons_params *p)
+{
+    if (!p) return "null parameters";
+    if (!(p->boxsize > 0.0)) return "boxsize must be positive";
+    if (!(p->rs > 0.0) || !(p->rc > 0.0)) return "rs and rc must be positive";
+    if (!(p->soft >= 0.0)) return "soft must be non-negative";
+    if (!(p->theta > 0.0)) return "theta must be positive";
+    if (p->maxleaf < 2 || p->maxleaf > 64) return "maxleaf must be in [2, 64]";
+    if (p->order < 3 || p->order > 5) return "order must be 3, 4 or 5";
+    if (p->precision != PHOTONS_FP64 && p->precision != PHOTONS_FP32_MIXED) return "unknown precision";
+    return "";
+}
+
+extern "C" {
+
+const char *photons_last_error(void) { return g_last_error.c_str(); }
+int photons_version(void) { return 100; }
+
+int photons_create(photons_ctx **out, const photons_params *prm, int device, void *stream)
+{
+    try {
+        std::string m = check_params(prm);
+        if (!m.empty()) return fail_msg(m);
+        int ndev = 0;
+        if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+            return fail_msg("no usable GPU for this context");
+        PH_CHECK(hipSetDevice(device));
+        photons_ctx *x = new photons_ctx();
+        x->c.prm = *prm;
+        x->c.device = device;
+        if (stream) {
+            x->c.stream = reinterpret_cast<hipStream_t>(stream);
+        } else {
+            PH_CHECK(hipStreamCreateWithFlags(&x->c.stream, hipStreamNonBlocking));
+            x->c.own_stream = true;
+        }
+        PH_CHECK(hipHostMalloc(&x->c.pinned.p, 4096, hipHostMallocDefault));
+        x->c.pinned.cap = 4096;
+        grow(x->c.err, sizeof(int));
+        for (int i = 0; i < 16; i++) PH_CHECK(hipEventCreate(&x->c.timing.ev[i]));
+        *out = x;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_create", e);
+    }
+}
+
+int photons_destroy(photons_ctx *x)
+{
+    if (!x) return 0;
+    try {
+        Ctx &c = x->c;
+        PH_CHECK(hipSetDevice(c.device));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        DevBuf *bufs[] = {&c.px, &c.py, &c.pz, &c.pidx, &c.qx, &c.qy, &c.qz, &c.qidx, &c.ox, &c.oy, &c.oz,
+                          &c.acc, &c.geom, &c.info, &c.level_nodes, &c.M, &c.L, &c.p2p_off, &c.p2p_src,
+                          &c.m2l_off, &c.m2l_src, &c.cnt_tmp, &c.scan_tmp, &c.t_start, &c.t_count, &c.t_split,
+                          &c.t_lo, &c.t_hi, &c.t_son, &c.t_depth, &c.t_parent, &c.l_start, &c.l_count, &c.l_lo,
+                          &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
+                          &c.seg_flag, &c.seg_nn, &c.seg_nl, &c.seg_non, &c.seg_lon, &c.t_size, &c.t_lsize,
+                          &c.t_dfs, &c.t_lfirst, &c.t_final, &c.l_final, &c.err, &c.stats,
+                          &x->io_pos, &x->io_acc, &x->op_a, &x->op_b, &x->op_c, &x->op_d, &x->op_e};
+        for (DevBuf *b : bufs) release(*b);
+        if (c.pinned.p) (void)hipHostFree(c.pinned.p);
+        for (int i = 0; i < 16; i++) (void)hipEventDestroy(c.timing.ev[i]);
+        if (c.own_stream) (void)hipStreamDestroy(c.stream);
+        delete x;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_destroy", e);
+    }
+}
+
+int photons_set_params(photons_ctx *x, const photons_params *prm)
+{
+    std::string m = check_params(prm);
+    if (!m.empty()) return fail_msg(m);
+    x->c.prm = *prm;
+    return 0;
+}
+
+int photons_get_params(const photons_ctx *x, photons_params *prm)
+{
+    *prm = x->c.prm;
+    return 0;
+}
+
+int photons_set_timing(photons_ctx *x, int enabled)
+{
+    x->c.timing.enabled = enabled != 0;
+    return 0;
+}
+
+int photons_get_phase_ms(const photons_ctx *x, double *ms)
+{
+    for (int i = 0; i < PHOTONS_NPHASE; i++) ms[i] = x->c.phase_ms[i];
+    return 0;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------- pipeline
+
+namespace photons {
+
+// phases up to and including the multipoles (fmm_prepare)
+static void prepare(Ctx &c, const double *d_pos)
+{
+    c.timing.nev = 0;
+    phase_mark(c);
+    upload_positions(c, d_pos);
+    phase_mark(c);
+    build_tree(c);
+    if (c.prm.precision == PHOTONS_FP32_MIXED) compute_offsets_f32(c);
+    phase_mark(c);
+    upward_pass(c);
+    phase_mark(c);
+}
+
+// walk, M2L, L2L, P2P + L2P (fmm_task + fmm_ext)
+static void evaluate(Ctx &c, double *d_out, bool caller_order)
+{
+    run_walk(c, 0, c.prm.periodic ? 27 : 1);
+    phase_mark(c);
+    m2l_pass(c, false);
+    phase_mark(c);
+    downward_pass(c);
+    phase_mark(c);
+    p2p_l2p_launch(c, true, false, caller_order, d_out);
+    phase_mark(c);
+}
+
+static void collect_timing(Ctx &c, bool with_io)
+{
+    for (int i = 0; i < PHOTONS_NPHASE; i++) c.phase_ms[i] = 0.0;
+    if (!c.timing.enabled) return;
+    PH_CHECK(hipEventSynchronize(c.timing.ev[c.timing.nev - 1]));
+    // marks: [upload] start, after upload, after tree, after upward, after walk, after m2l, after l2l, after p2p
+    const int ph[] = {PHOTONS_PH_UPLOAD, PHOTONS_PH_TREE, PHOTONS_PH_UPWARD, PHOTONS_PH_WALK,
+                      PHOTONS_PH_M2L, PHOTONS_PH_L2L, PHOTONS_PH_P2P};
+    for (int i = 0; i + 1 < c.timing.nev && i < 7; i++) {
+        float ms = 0.f;
+        PH_CHECK(hipEventElapsedTime(&ms, c.timing.ev[i], c.timing.ev[i + 1]));
+        c.phase_ms[ph[i]] = ms;
+    }
+    (void)with_io;
+}
+
+static std::string check_n(const Ctx &c, int64_t n)
+{
+    if (n < 0) return "negative particle count";
+    if (n > (int64_t)0x7fffffff / 2) return "too many particles for one context";
+    if (n > 0 && c.prm.periodic && n <= c.prm.maxleaf)
+        return "periodic force needs more than maxleaf particles (the image tree root would read as a leaf)";
+    return "";
+}
+
+}  // namespace photons
+
+extern "C" {
+
+int photons_force_device(photons_ctx *x, int64_t n, const double *d_pos, double *d_acc)
+{
+    try {
+        Ctx &c = x->c;
+        std::string m = check_n(c, n);
+        if (!m.empty()) return fail_msg(m);
+        PH_CHECK(hipSetDevice(c.device));
+        c.n = n;
+        if (n == 0) {
+            c.nleaf = c.nnode = c.ncell = 0;
+            return 0;
+        }
+        prepare(c, d_pos);
+        evaluate(c, d_acc, true);
+        collect_timing(c, false);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_force_device", e);
+    }
+}
+
+int photons_force_host(photons_ctx *x, int64_t n, const double *pos, double *acc)
+{
+    try {
+        Ctx &c = x->c;
+        std::string m = check_n(c, n);
+        if (!m.empty()) return fail_msg(m);
+        PH_CHECK(hipSetDevice(c.device));
+        c.n = n;
+        if (n == 0) return 0;
+        double *dp = reinterpret_cast<double *>(grow(x->io_pos, sizeof(double) * 3 * (size_t)n));
+        double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)n));
+        PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
+        prepare(c, dp);
+        evaluate(c, da, true);
+        PH_CHECK(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        collect_timing(c, true);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_force_host", e);
+    }
+}
+
+// two-phase form used by the drop-in layer: prepare (tree + multipoles), then
+// evaluate into a host array in tree order
+int photons_prepare_host(photons_ctx *x, int64_t n, const double *pos)
+{
+    try {
+        Ctx &c = x->c;
+        std::string m = check_n(c, n);
+        if (!m.empty()) return fail_msg(m);
+        PH_CHECK(hipSetDevice(c.device));
+        c.n = n;
+        if (n == 0) return 0;
+        double *dp = reinterpret_cast<double *>(grow(x->io_pos, sizeof(double) * 3 * (size_t)n));
+        PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
+        prepare(c, dp);
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_prepare_host", e);
+    }
+}
+
+int photons_evaluate_host(photons_ctx *x, double *acc_tree_order)
+{
+    try {
+        Ctx &c = x->c;
+        PH_CHECK(hipSetDevice(c.device));
+        if (c.n == 0) return 0;
+        double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)c.n));
+        evaluate(c, da, false);
+        PH_CHECK(hipMemcpyAsync(acc_tree_order, da, sizeof(double) * 3 * (size_t)c.n, hipMemcpyDeviceToHost,
+                                c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_evaluate_host", e);
+    }
+}
+
+int photons_sync(photons_ctx *x)
+{
+    try {
+        PH_CHECK(hipStreamSynchronize(x->c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_sync", e);
+    }
+}
+
+// ---------------------------------------------------------------------------- inspection
+
+__global__ void k_list_stats(int nleaf, const CellInfo *info, const int64_t *off, const int32_t *src,
+                             unsigned long long *out)
+{
+    int T = blockIdx.x * blockDim.x + threadIdx.x;
+    if (T >= nleaf) return;
+    long long nT = info[T].npart;
+    unsigned long long local = 0, inter = 0, image = 0;
+    for (int64_t e = off[T]; e < off[T + 1]; e++) {
+        int s = src[e], S = entry_cell(s), sid = entry_shift(s);
+        long long nS = info[S].npart;
+        long long k = nT * nS;
+        if (sid == 0) {
+            local++;
+            if (S == T) k -= nT;
+        } else {
+            image += k;
+        }
+        inter += k;
+    }
+    atomicAdd(&out[0], local);
+    atomicAdd(&out[1], inter);
+    atomicAdd(&out[2], image);
+}
+
+__global__ void k_m2l_local(int ncell, const int64_t *off, const int32_t *src, unsigned long long *out)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    if (X >= ncell) return;
+    unsigned long long local = 0;
+    for (int64_t e = off[X]; e < off[X + 1]; e++) local += entry_shift(src[e]) == 0;
+    atomicAdd(&out[3], local);
+}
+
+int photons_get_stats(const photons_ctx *xc, photons_stats *st)
+{
+    try {
+        photons_ctx *x = const_cast<photons_ctx *>(xc);
+        Ctx &c = x->c;
+        std::memset(st, 0, sizeof *st);
+        st->n = c.n;
+        st->nleaf = c.nleaf;
+        st->nnode = c.nnode;
+        st->depth = c.maxdepth;
+        st->p2p_pairs = c.n_p2p;
+        st->m2l_pairs = c.n_m2l;
+        if (c.n == 0 || c.ncell == 0) return 0;
+        unsigned long long *d = reinterpret_cast<unsigned long long *>(grow(c.stats, 4 * sizeof(unsigned long long)));
+        PH_CHECK(hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), c.stream));
+        k_list_stats<<<(c.nleaf + 127) / 128, 128, 0, c.stream>>>(c.nleaf, dptr<CellInfo>(c.info),
+                                                                  dptr<int64_t>(c.p2p_off), dptr<int32_t>(c.p2p_src), d);
+        k_m2l_local<<<(c.ncell + 127) / 128, 128, 0, c.stream>>>(c.ncell, dptr<int64_t>(c.m2l_off),
+                                                                 dptr<int32_t>(c.m2l_src), d);
+        unsigned long long h[4];
+        PH_CHECK(hipMemcpyAsync(h, d, sizeof h, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        st->p2p_local_pairs = (int64_t)h[0];
+        st->p2p_interactions = (int64_t)h[1];
+        st->p2p_image_interactions = (int64_t)h[2];
+        st->m2l_local_pairs = (int64_t)h[3];
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_get_stats", e);
+    }
+}
+
+int photons_tree_perm(const photons_ctx *xc, int32_t *perm)
+{
+    try {
+        photons_ctx *x = const_cast<photons_ctx *>(xc);
+        Ctx &c = x->c;
+        if (c.n == 0) return 0;
+        PH_CHECK(hipMemcpyAsync(perm, c.pidx.p, sizeof(int32_t) * c.n, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_tree_perm", e);
+    }
+}
+
+int photons_tree_cells(const photons_ctx *xc, int32_t *npart, int32_t *ipart, int32_t *son, double *center,
+                       double *width)
+{
+    try {
+        photons_ctx *x = const_cast<photons_ctx *>(xc);
+        Ctx &c = x->c;
+        if (c.ncell == 0) return 0;
+        std::vector<CellInfo> I(c.ncell);
+        std::vector<CellGeom> G(c.ncell);
+        PH_CHECK(hipMemcpyAsync(I.data(), c.info.p, sizeof(CellInfo) * c.ncell, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipMemcpyAsync(G.data(), c.geom.p, sizeof(CellGeom) * c.ncell, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        for (int k = 0; k < c.ncell; k++) {
+            if (npart) npart[k] = I[k].npart;
+            if (ipart) ipart[k] = I[k].ipart;
+            if (son) {
+                son[2 * k] = I[k].son0;
+                son[2 * k + 1] = I[k].son1;
+            }
+            for (int d = 0; d < 3; d++) {
+                if (center) center[3 * k + d] = G[k].c[d];
+                if (width) width[3 * k + d] = G[k].w[d];
+            }
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_tree_cells", e);
+    }
+}
+
+static int export_list(Ctx &c, int ntarget, DevBuf &offb, DevBuf &srcb, int64_t total, int64_t cap, int32_t *tgt,
+                       int32_t *src, int8_t *shift, int64_t *count)
+{
+    *count = total;
+    if (!tgt || total == 0) return 0;
+    std::vector<int64_t> off(ntarget + 1);
+    std::vector<int32_t> s(total);
+    PH_CHECK(hipMemcpyAsync(off.data(), offb.p, sizeof(int64_t) * (ntarget + 1), hipMemcpyDeviceToHost, c.stream));
+    PH_CHECK(hipMemcpyAsync(s.data(), srcb.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost, c.stream));
+    PH_CHECK(hipStreamSynchronize(c.stream));
+    for (int t = 0; t < ntarget; t++)
+        for (int64_t e = off[t]; e < off[t + 1] && e < cap; e++) {
+            tgt[e] = t;
+            src[e] = entry_cell(s[e]);
+            shift[e] = (int8_t)entry_shift(s[e]);
+        }
+    return 0;
+}
+
+int photons_p2p_list(const photons_ctx *xc, int64_t cap, int32_t *tgt, int32_t *src, int8_t *shift, int64_t *count)
+{
+    try {
+        photons_ctx *x = const_cast<photons_ctx *>(xc);
+        return export_list(x->c, x->c.nleaf, x->c.p2p_off, x->c.p2p_src, x->c.n_p2p, cap, tgt, src, shift, count);
+    } catch (const std::exception &e) {
+        return fail("photons_p2p_list", e);
+    }
+}
+
+int photons_m2l_list(const photons_ctx *xc, int64_t cap, int32_t *tgt, int32_t *src, int8_t *shift, int64_t *count)
+{
+    try {
+        photons_ctx *x = const_cast<photons_ctx *>(xc);
+        return export_list(x->c, x->c.ncell, x->c.m2l_off, x->c.m2l_src, x->c.n_m2l, cap, tgt, src, shift, count);
+    } catch (const std::exception &e) {
+        return fail("photons_m2l_list", e);
+    }
+}
+
+int photons_expansions(const photons_ctx *xc, double *M, double *L)
+{
+    try {
+        photons_ctx *x = const_cast<photons_ctx *>(xc);
+        Ctx &c = x->c;
+        const int NT = nterms_of(c.prm.order);
+        size_t cnt = (size_t)c.ncell * NT;
+        if (cnt == 0) return 0;
+        const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+        if (M) PH_CHECK(hipMemcpyAsync(M, c.M.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
+        std::vector<float> lf;
+        if (L) {
+            if (f32) {
+                lf.resize(cnt);
+                PH_CHECK(hipMemcpyAsync(lf.data(), c.L.p, sizeof(float) * cnt, hipMemcpyDeviceToHost, c.stream));
+            } else {
+                PH_CHECK(hipMemcpyAsync(L, c.L.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
+            }
+        }
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        if (L && f32) {
+            // L_a = L'_a / rs^(|a|+1)
+            std::vector<double> sc(NT);
+            int k = 0;
+            for (int n = 0; n <= c.prm.order; n++)
+                for (int a = n; a >= 0; a--)
+                    for (int b = n - a; b >= 0; b--) sc[k++] = 1.0 / std::pow(c.prm.rs, n + 1);
+            for (size_t i = 0; i < cnt; i++) L[i] = lf[i] * sc[i % NT];
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_expansions", e);
+    }
+}
+
+// ---------------------------------------------------------------------------- batched operators
+
+int photons_p2m_batch(photons_ctx *x, int order, int count, const double *d_pos, const int32_t *d_first,
+                      const int32_t *d_npart, const double *d_center, double mass, double *d_M)
+{
+    try {
+        if (count <= 0) return 0;
+        op_p2m(x->c.stream, order, count, d_pos, d_first, d_npart, d_center, mass, d_M);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_p2m_batch", e);
+    }
+}
+
+int photons_m2m_batch(photons_ctx *x, int order, int count, const double *d_d, const double *d_M, double *d_tM)
+{
+    try {
+        if (count <= 0) return 0;
+        op_shift(x->c.stream, order, 0, count, d_d, d_M, d_tM);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_m2m_batch", e);
+    }
+}
+
+int photons_m2l_batch(photons_ctx *x, int order, int count, double rs, const double *d_R, const double *d_M,
+                      double *d_L)
+{
+    try {
+        if (count <= 0) return 0;
+        op_m2l(x->c.stream, order, count, rs, d_R, d_M, d_L);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_m2l_batch", e);
+    }
+}
+
+int photons_l2l_batch(photons_ctx *x, int order, int count, const double *d_d, const double *d_L, double *d_tL)
+{
+    try {
+        if (count <= 0) return 0;
+        op_shift(x->c.stream, order, 1, count, d_d, d_L, d_tL);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_l2l_batch", e);
+    }
+}
+
+int photons_l2p_batch(photons_ctx *x, int order, int n, const double *d_pos, const int32_t *d_owner,
+                      const double *d_center, const double *d_L, double *d_acc)
+{
+    try {
+        if (n <= 0) return 0;
+        op_l2p(x->c.stream, order, n, d_pos, d_owner, d_center, d_L, d_acc);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_l2p_batch", e);
+    }
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------- drop-in support
+// Entry points used by the reference-named functions (compat.c).  Cell ids are
+// those of photons_tree_cells; accumulation targets are host arrays in tree order.
+
+namespace photons {
+__global__ void k_zero_f64(double *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+}  // namespace photons
+
+extern "C" {
+
+int photons_acceptance(const double *wi, const double *wj, const double *dist, double theta, double rc)
+{
+    return mac(wi, wj, dist[0], dist[1], dist[2], theta * theta, rc * rc);
+}
+
+// Walk from the pair (start_target, start_source) over shifts [shift_lo, shift_hi) and
+// evaluate what it emits: P2P added into acc_tree (host, [n][3], tree order) and M2L
+// added into the context's L.  Returns the numbers of P2P and M2L entries.
+int photons_walk_eval(photons_ctx *x, int start_target, int start_source, int shift_lo, int shift_hi, int do_p2p,
+                      int do_m2l, double *acc_tree, int64_t *n_p2p, int64_t *n_m2l)
+{
+    try {
+        Ctx &c = x->c;
+        PH_CHECK(hipSetDevice(c.device));
+        if (c.ncell == 0) return 0;
+        if (start_target < 0 || start_target >= c.ncell || start_source < 0 || start_source >= c.ncell)
+            return fail_msg("walk start pair outside the tree");
+        run_walk_from(c, shift_lo, shift_hi, start_target, start_source);
+        if (n_p2p) *n_p2p = c.n_p2p;
+        if (n_m2l) *n_m2l = c.n_m2l;
+        if (do_m2l) {
+            if (!c.L.p) {
+                const int NT = nterms_of(c.prm.order);
+                size_t bytes = (c.prm.precision == PHOTONS_FP32_MIXED ? 4 : 8) * (size_t)c.ncell * NT;
+                grow(c.L, bytes);
+                PH_CHECK(hipMemsetAsync(c.L.p, 0, bytes, c.stream));
+            }
+            m2l_pass(c, true);
+        }
+        if (do_p2p && acc_tree) {
+            double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)c.n));
+            PH_CHECK(hipMemcpyAsync(da, acc_tree, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
+            if (c.prm.precision == PHOTONS_FP32_MIXED && !c.ox.p) compute_offsets_f32(c);
+            p2p_l2p_launch(c, false, true, false, da);
+            PH_CHECK(hipMemcpyAsync(acc_tree, da, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
+        }
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_walk_eval", e);
+    }
+}
+
+// P2P of one (target leaf, source leaf, shift) pair list, added into acc_tree
+// (host, tree order); precision selects the FP64 or the FP32-mixed pair kernel.
+int photons_p2p_pairs(photons_ctx *x, int count, const int32_t *tgt, const int32_t *src, const int8_t *shift,
+                      int precision, double *acc_tree)
+{
+    try {
+        Ctx &c = x->c;
+        PH_CHECK(hipSetDevice(c.device));
+        if (count <= 0 || c.nleaf == 0) return 0;
+        // CSR over all leaves from the (unsorted) pairs, keeping their order per target
+        std::vector<int64_t> off(c.nleaf + 1, 0);
+        for (int k = 0; k < count; k++) {
+            if (tgt[k] < 0 || tgt[k] >= c.nleaf || src[k] < 0 || src[k] >= c.nleaf)
+                return fail_msg("p2p pair outside the leaves");
+            off[tgt[k] + 1]++;
+        }
+        for (int t = 0; t < c.nleaf; t++) off[t + 1] += off[t];
+        std::vector<int32_t> ent(count);
+        std::vector<int64_t> fill(off.begin(), off.end() - 1);
+        for (int k = 0; k < count; k++) ent[fill[tgt[k]]++] = pack_entry(src[k], shift[k]);
+        int64_t *doff = reinterpret_cast<int64_t *>(grow(c.p2p_off, sizeof(int64_t) * (c.nleaf + 1)));
+        int32_t *dsrc = reinterpret_cast<int32_t *>(grow(c.p2p_src, sizeof(int32_t) * count));
+        PH_CHECK(hipMemcpyAsync(doff, off.data(), sizeof(int64_t) * (c.nleaf + 1), hipMemcpyHostToDevice, c.stream));
+        PH_CHECK(hipMemcpyAsync(dsrc, ent.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, c.stream));
+        c.n_p2p = count;
+        double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)c.n));
+        PH_CHECK(hipMemcpyAsync(da, acc_tree, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
+        int saved = c.prm.precision;
+        c.prm.precision = precision;
+        if (precision == PHOTONS_FP32_MIXED) compute_offsets_f32(c);
+        try {
+            p2p_l2p_launch(c, false, true, false, da);
+        } catch (...) {
+            c.prm.precision = saved;
+            throw;
+        }
+        c.prm.precision = saved;
+        PH_CHECK(hipMemcpyAsync(acc_tree, da, sizeof(double) * 3 * c.n, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_p2p_pairs", e);
+    }
+}
+
+// the context's local expansions, read or overwritten (double, [ncell][nterms])
+int photons_local_expansions(photons_ctx *x, double *L, int write)
+{
+    try {
+        Ctx &c = x->c;
+        const int NT = nterms_of(c.prm.order);
+        size_t cnt = (size_t)c.ncell * NT;
+        if (cnt == 0) return 0;
+        if (c.prm.precision == PHOTONS_FP32_MIXED) return fail_msg("local expansions are float in this context");
+        if (write) {
+            grow(c.L, sizeof(double) * cnt);
+            PH_CHECK(hipMemcpyAsync(c.L.p, L, sizeof(double) * cnt, hipMemcpyHostToDevice, c.stream));
+        } else {
+            PH_CHECK(hipMemcpyAsync(L, c.L.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
+        }
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_local_expansions", e);
+    }
+}
+
+// device scratch of the context for the single-operator entry points
+void *photons_scratch(photons_ctx *x, int slot, size_t bytes)
+{
+    try {
+        DevBuf *b[] = {&x->op_a, &x->op_b, &x->op_c, &x->op_d, &x->op_e};
+        if (slot < 0 || slot > 4) return nullptr;
+        return grow(*b[slot], bytes);
+    } catch (const std::exception &e) {
+        fail("photons_scratch", e);
+        return nullptr;
+    }
+}
+
+int photons_copy(photons_ctx *x, void *dst, const void *src, size_t bytes, int to_device)
+{
+    try {
+        PH_CHECK(hipMemcpyAsync(dst, src, bytes, to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost,
+                                x->c.stream));
+        PH_CHECK(hipStreamSynchronize(x->c.stream));
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_copy", e);
+    }
+}
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+// fmm_math.h -- Cartesian multipole algebra, pair kernels and acceptance rules
+// for the short-range (erfc-split) FMM, as device functions for gfx950.
+//
+// Multi-index layout: terms ordered by total degree n, then a (x power)
+// descending, then b descending -- the layout of inc/operator.h:24-164, extended
+// to any order.  index(a,b,c) = n(n+1)(n+2)/6 + m(m+1)/2 + c with m = b + c.
+//
+// Conventions (src/operator.c):
+//   P2M  M_a  = sum_i m (-(x_i - c))^a / a!
+//   M2M  M'_a = sum_{b<=a} M_b d^(a-b)/(a-b)!            d = c_parent - c_child
+//   M2L  L_a += sum_{|a|+|b|<=p} M_b D_{a+b}(R)          R = c_target - c_source
+//   L2L  L'_a += sum_g L_{a+g} d^g/g!                     d = c_child - c_parent
+//   L2P  acc_i += sum_{|a|<=p-1} L_{a+e_i} d^a/a!         d = x - c_leaf
+//   D_a(R) = d^a g(|R|),  g(r) = erfc(r/2rs)/r,
+//   D_a = sum_k f_{|a|-k} sum_{kx+ky+kz=k} prod_i a_i!/(2^k_i k_i! (a_i-2k_i)!) R_i^(a_i-2k_i)
+//   with f_n = (r^-1 d/dr)^n g.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace photons {
+
+__host__ __device__ constexpr int nterms(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
+__host__ __device__ constexpr int mindex(int a, int b, int c)
+{
+    return (a + b + c) * (a + b + c + 1) * (a + b + c + 2) / 6 + (b + c) * (b + c + 1) / 2 + c;
+}
+
+constexpr int kMaxOrder = 5;
+constexpr int kMaxTerms = nterms(kMaxOrder);
+
+struct MultiIndexTable {
+    int a[nterms(2 * kMaxOrder)], b[nterms(2 * kMaxOrder)], c[nterms(2 * kMaxOrder)];
+    double inv_fact[nterms(2 * kMaxOrder)];  // 1/(a! b! c!)
+    constexpr MultiIndexTable() : a(), b(), c(), inv_fact()
+    {
+        int k = 0;
+        for (int n = 0; n <= 2 * kMaxOrder; n++)
+            for (int x = n; x >= 0; x--)
+                for (int y = n - x; y >= 0; y--) {
+                    a[k] = x;
+                    b[k] = y;
+                    c[k] = n - x - y;
+                    double f = 1.0;
+                    for (int i = 2; i <= x; i++) f *= i;
+                    for (int i = 2; i <= y; i++) f *= i;
+                    for (int i = 2; i <= n - x - y; i++) f *= i;
+                    inv_fact[k] = 1.0 / f;
+                    k++;
+                }
+    }
+};
+constexpr MultiIndexTable kMI{};
+
+__host__ __device__ constexpr int ma(int k) { return kMI.a[k]; }
+__host__ __device__ constexpr int mb(int k) { return kMI.b[k]; }
+__host__ __device__ constexpr int mc(int k) { return kMI.c[k]; }
+__host__ __device__ constexpr int mdeg(int k) { return kMI.a[k] + kMI.b[k] + kMI.c[k]; }
+
+// a!/(2^k k! (a-2k)!): coefficient of the k-th contraction of d^a applied to a radial function
+__host__ __device__ constexpr double hcoef(int a, int k)
+{
+    double num = 1.0, den = 1.0;
+    for (int i = 2; i <= a; i++) num *= i;
+    for (int i = 2; i <= k; i++) den *= i;
+    for (int i = 2; i <= a - 2 * k; i++) den *= i;
+    for (int i = 0; i < k; i++) den *= 2.0;
+    return num / den;
+}
+
+// Powers x^e / e! for e = 0..P, i.e. the monomial weights of a Taylor shift.
+template <int P, typename T>
+__device__ __forceinline__ void scaled_powers(T x, T *px)
+{
+    px[0] = T(1);
+#pragma unroll
+    for (int e = 1; e <= P; e++) px[e] = px[e - 1] * x * T(1.0 / e);
+}
+
+template <int P, typename T>
+__device__ __forceinline__ void plain_powers(T x, T *px)
+{
+    px[0] = T(1);
+#pragma unroll
+    for (int e = 1; e <= P; e++) px[e] = px[e - 1] * x;
+}
+
+// ----------------------------------------------------------------------------- P2M / M2M
+
+// M += m * (-d)^a / a!  for one particle at offset d = x - c
+template <int P, typename T>
+__device__ __forceinline__ void p2m_add(T m, T dx, T dy, T dz, T *M)
+{
+    T px[P + 1], py[P + 1], pz[P + 1];
+    scaled_powers<P>(-dx, px);
+    scaled_powers<P>(-dy, py);
+    scaled_powers<P>(-dz, pz);
+#pragma unroll
+    for (int k = 0; k < nterms(P); k++) M[k] += m * px[ma(k)] * py[mb(k)] * pz[mc(k)];
+}
+
+// tM += shift(M), d = c_parent - c_child
+template <int P, typename T>
+__device__ __forceinline__ void m2m_add(T dx, T dy, T dz, const T *M, T *tM)
+{
+    T px[P + 1], py[P + 1], pz[P + 1];
+    scaled_powers<P>(dx, px);
+    scaled_powers<P>(dy, py);
+    scaled_powers<P>(dz, pz);
+#pragma unroll
+    for (int k = 0; k < nterms(P); k++) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j <= k; j++) {
+            constexpr int dummy = 0;
+            (void)dummy;
+            if (ma(j) <= ma(k) && mb(j) <= mb(k) && mc(j) <= mc(k))
+                s += M[j] * px[ma(k) - ma(j)] * py[mb(k) - mb(j)] * pz[mc(k) - mc(j)];
+        }
+        tM[k] += s;
+    }
+}
+
+// ----------------------------------------------------------------------------- L2L / L2P
+
+// tL += shift(L), d = c_child - c_parent
+template <int P, typename T>
+__device__ __forceinline__ void l2l_add(T dx, T dy, T dz, const T *L, T *tL)
+{
+    T px[P + 1], py[P + 1], pz[P + 1];
+    scaled_powers<P>(dx, px);
+    scaled_powers<P>(dy, py);
+    scaled_powers<P>(dz, pz);
+#pragma unroll
+    for (int k = 0; k < nterms(P); k++) {
+        T s = T(0);
+#pragma unroll
+        for (int g = 0; g < nterms(P); g++) {
+            if (mdeg(k) + mdeg(g) <= P)
+                s += L[mindex(ma(k) + ma(g), mb(k) + mb(g), mc(k) + mc(g))] * px[ma(g)] * py[mb(g)] * pz[mc(g)];
+        }
+        tL[k] += s;
+    }
+}
+
+// gradient of the local expansion at offset d from its centre
+template <int P, typename T>
+__device__ __forceinline__ void l2p_eval(const T *L, T dx, T dy, T dz, T &ax, T &ay, T &az)
+{
+    T px[P + 1], py[P + 1], pz[P + 1];
+    scaled_powers<P>(dx, px);
+    scaled_powers<P>(dy, py);
+    scaled_powers<P>(dz, pz);
+    T fx = T(0), fy = T(0), fz = T(0);
+#pragma unroll
+    for (int k = 0; k < nterms(P - 1); k++) {
+        T w = px[ma(k)] * py[mb(k)] * pz[mc(k)];
+        fx += L[mindex(ma(k) + 1, mb(k), mc(k))] * w;
+        fy += L[mindex(ma(k), mb(k) + 1, mc(k))] * w;
+        fz += L[mindex(ma(k), mb(k), mc(k) + 1)] * w;
+    }
+    ax = fx;
+    ay = fy;
+    az = fz;
+}
+
+// ----------------------------------------------------------------------------- split kernel
+
+// Single precision erfc(u) for u >= 0 as exp(-u^2) * t * Q(t), t = 1/(1 + u/2): the
+// variable of the Numerical Recipes erfcc form (Press et al., sec. 6.2), with Q a
+// degree-10 polynomial fitted here by least squares on the relative error of
+// erfcx(u) over [0, 1e4].  Relative error of t*Q(t) against erfcx: 1.1e-8 in exact
+// arithmetic, 3.2e-7 when evaluated in float (rounding limited).
+__device__ __forceinline__ float erfcx_tq(float t)
+{
+    float q = 4.242083445e-02f;
+    q = __builtin_fmaf(q, t, -2.311667988e-01f);
+    q = __builtin_fmaf(q, t, 4.867675762e-01f);
+    q = __builtin_fmaf(q, t, -4.475895600e-01f);
+    q = __builtin_fmaf(q, t, 1.274673754e-01f);
+    q = __builtin_fmaf(q, t, -5.737388979e-02f);
+    q = __builtin_fmaf(q, t, 9.303219725e-02f);
+    q = __builtin_fmaf(q, t, 1.753727219e-01f);
+    q = __builtin_fmaf(q, t, 2.468809031e-01f);
+    q = __builtin_fmaf(q, t, 2.820938369e-01f);
+    q = __builtin_fmaf(q, t, 2.820947951e-01f);
+    return t * q;
+}
+
+// exp(-u^2) in float via the hardware base-2 exponential
+__device__ __forceinline__ float gauss_f32(float u)
+{
+    return __builtin_amdgcn_exp2f(-1.4426950408889634f * u * u);
+}
+
+// G(u) = erfc(u) + 2/sqrt(pi) u exp(-u^2), the factor of the split pair force
+__device__ __forceinline__ float split_factor_f32(float u)
+{
+    float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f, u, 1.0f));
+    return gauss_f32(u) * __builtin_fmaf(1.1283791670955126f, u, erfcx_tq(t));
+}
+
+__device__ __forceinline__ double split_factor_f64(double u)
+{
+    return erfc(u) + (2.0 / 1.7724538509055160273) * u * exp(-u * u);
+}
+
+// radial factors f_n = (r^-1 d/dr)^n [erfc(r/2rs)/r], n = 0..P, by the upward
+// recurrence B_n = ((2n-1) B_{n-1} + (2a^2)^n exp(-a^2 r^2)/(a sqrt(pi))) / r^2,
+// f_n = (-1)^n B_n, a = 1/(2 rs).
+template <int P>
+__device__ __forceinline__ void radial_f64(double r, double inv2rs, double *f)
+{
+    double u = r * inv2rs;
+    double e = exp(-u * u) * 0.56418958354775628695;  // 1/sqrt(pi)
+    double ir = 1.0 / r;
+    double ir2 = ir * ir;
+    double b = erfc(u) * ir;
+    double t = e / inv2rs;
+    double a2 = 2.0 * inv2rs * inv2rs;
+    f[0] = b;
+#pragma unroll
+    for (int n = 1; n <= P; n++) {
+        t *= a2;
+        b = ((2 * n - 1) * b + t) * ir2;
+        f[n] = (n & 1) ? -b : b;
+    }
+}
+
+// same in float, in units where rs = 1 (so a = 1/2)
+template <int P>
+__device__ __forceinline__ void radial_f32_unit(float r, float *f)
+{
+    float u = 0.5f * r;
+    float e = gauss_f32(u);
+    float ir = __builtin_amdgcn_rcpf(r);
+    float ir2 = ir * ir;
+    float tt = __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f, u, 1.0f));
+    float b = e * erfcx_tq(tt) * ir;
+    float t = e * (2.0f * 0.56418958354775628695f);
+    f[0] = b;
+#pragma unroll
+    for (int n = 1; n <= P; n++) {
+        t *= 0.5f;
+        b = ((2 * n - 1) * b + t) * ir2;
+        f[n] = (n & 1) ? -b : b;
+    }
+}
+
+// D_a(R) for |a| <= P from the radial factors
+template <int P, typename T>
+__device__ __forceinline__ void dtensor(T x, T y, T z, const T *f, T *D)
+{
+    T px[P + 1], py[P + 1], pz[P + 1];
+    plain_powers<P>(x, px);
+    plain_powers<P>(y, py);
+    plain_powers<P>(z, pz);
+#pragma unroll
+    for (int k = 0; k < nterms(P); k++) {
+        T s = T(0);
+#pragma unroll
+        for (int kx = 0; kx <= P / 2; kx++)
+#pragma unroll
+            for (int ky = 0; ky <= P / 2; ky++)
+#pragma unroll
+                for (int kz = 0; kz <= P / 2; kz++) {
+                    if (2 * kx <= ma(k) && 2 * ky <= mb(k) && 2 * kz <= mc(k)) {
+                        constexpr double one = 1.0;
+                        T coef = T(one * hcoef(ma(k), kx) * hcoef(mb(k), ky) * hcoef(mc(k), kz));
+                        s += f[mdeg(k) - kx - ky - kz] * coef * px[ma(k) - 2 * kx] * py[mb(k) - 2 * ky] *
+                             pz[mc(k) - 2 * kz];
+                    }
+                }
+        D[k] = s;
+    }
+}
+
+// L += sum_{|k|+|j|<=P} M_j D_{k+j}
+template <int P, typename T>
+__device__ __forceinline__ void m2l_contract(const T *D, const T *M, T *L)
+{
+#pragma unroll
+    for (int k = 0; k < nterms(P); k++) {
+        T s = T(0);
+#pragma unroll
+        for (int j = 0; j < nterms(P); j++) {
+            if (mdeg(k) + mdeg(j) <= P) s += M[j] * D[mindex(ma(k) + ma(j), mb(k) + mb(j), mc(k) + mc(j))];
+        }
+        L[k] += s;
+    }
+}
+
+// ----------------------------------------------------------------------------- acceptance
+
+// acceptance() of src/fmm.c:267-326 (LONGSHORT): 1 accept (M2L), 0 open, -1 beyond
+// the cutoff.  Boxes whose gap sums to less than 1e-4 (kpc/h) count as touching.
+__host__ __device__ inline int mac(const double *wi, const double *wj, double d0, double d1, double d2,
+                                   double theta2, double rc2)
+{
+    double w0 = (wi[0] + wj[0]) * 0.5, w1 = (wi[1] + wj[1]) * 0.5, w2 = (wi[2] + wj[2]) * 0.5;
+    double dd2 = d0 * d0 + d1 * d1 + d2 * d2;
+    double g0 = (d0 < 0.0 ? -d0 : d0) - w0;
+    double g1 = (d1 < 0.0 ? -d1 : d1) - w1;
+    double g2 = (d2 < 0.0 ? -d2 : d2) - w2;
+    g0 = g0 <= 0.0 ? 0.0 : g0;
+    g1 = g1 <= 0.0 ? 0.0 : g1;
+    g2 = g2 <= 0.0 ? 0.0 : g2;
+    if (g0 + g1 + g2 < 0.0001) return 0;
+    double dm2 = g0 * g0 + g1 * g1 + g2 * g2;
+    if (dm2 >= rc2) return -1;
+    if (dd2 > rc2) return 0;
+    double wmax = w0;
+    if (w1 > wmax) wmax = w1;
+    if (w2 > wmax) wmax = w2;
+    wmax *= 2;
+    return (wmax * wmax < theta2 * dd2) ? 1 : 0;
+}
+
+}  // namespace photons

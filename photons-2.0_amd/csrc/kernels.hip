@@ -1,0 +1,655 @@
+// kernels.hip -- P2M, M2M, M2L, L2L and the fused P2P + L2P kernel.
+//
+// FP64 path: every operator in double on global coordinates (the reference's
+// arithmetic, src/operator.c, src/fmm.c:796-872, src/remotes.c:14-57).
+// FP32-mixed path: P2M and M2M in double; M2L, L2L, L2P and P2P in float on
+// relative coordinates formed in double, lengths in units of rs so that r^-(2p+1)
+// stays inside the float range:
+//   M'_a = M_a / rs^|a|,  L_a = L'_a / rs^(|a|+1),  acc = rs^-2 * grad L'(d/rs).
+//
+// P2P: one wavefront per target leaf.  Lane l handles target l mod nT and every
+// nsub-th source particle (nsub = 64 / nT), so a 22-particle leaf keeps 44 lanes
+// busy; each source leaf is staged in LDS once for the whole wavefront, and the
+// per-lane partial sums are combined in a fixed order (deterministic results).
+#include <stdexcept>
+
+#include "engine.h"
+#include "fmm_math.h"
+
+namespace photons {
+
+constexpr int kP2PMaxLeaf = 64;
+
+// ---------------------------------------------------------------------------- upward
+
+template <int P>
+__global__ void k_p2m(int nleaf, const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
+                      const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
+                      double mass, double *__restrict__ M)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nleaf) return;
+    constexpr int NT = nterms(P);
+    double m[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) m[k] = 0.0;
+    const CellInfo I = info[l];
+    const CellGeom g = geom[l];
+    for (int i = I.ipart; i < I.ipart + I.npart; i++)
+        p2m_add<P, double>(mass, px[i] - g.c[0], py[i] - g.c[1], pz[i] - g.c[2], m);
+#pragma unroll
+    for (int k = 0; k < NT; k++) M[(size_t)l * NT + k] = m[k];
+}
+
+template <int P>
+__global__ void k_m2m_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
+                            const CellInfo *__restrict__ info, double *__restrict__ M)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    constexpr int NT = nterms(P);
+    const int v = nodes[t];
+    const CellInfo I = info[v];
+    const CellGeom gv = geom[v];
+    double m[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) m[k] = 0.0;
+    const int sons[2] = {I.son0, I.son1};
+    for (int s = 0; s < 2; s++) {
+        const int c = sons[s];
+        const CellGeom gc = geom[c];
+        double mc[NT];
+#pragma unroll
+        for (int k = 0; k < NT; k++) mc[k] = M[(size_t)c * NT + k];
+        m2m_add<P, double>(gv.c[0] - gc.c[0], gv.c[1] - gc.c[1], gv.c[2] - gc.c[2], mc, m);
+    }
+#pragma unroll
+    for (int k = 0; k < NT; k++) M[(size_t)v * NT + k] = m[k];
+}
+
+template <int P>
+__global__ void k_scale_down(int ncell, const double *__restrict__ M, double inv_rs, float *__restrict__ Ms)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncell) return;
+    constexpr int NT = nterms(P);
+    double s[P + 1];
+    s[0] = 1.0;
+#pragma unroll
+    for (int e = 1; e <= P; e++) s[e] = s[e - 1] * inv_rs;
+#pragma unroll
+    for (int k = 0; k < NT; k++) Ms[(size_t)c * NT + k] = (float)(M[(size_t)c * NT + k] * s[mdeg(k)]);
+}
+
+// ---------------------------------------------------------------------------- M2L
+
+template <int P>
+__global__ void __launch_bounds__(128) k_m2l_f64(int ncell, const CellGeom *__restrict__ geom,
+                                                 const int64_t *__restrict__ off, const int32_t *__restrict__ src,
+                                                 const double *__restrict__ M, double inv2rs, double boxsize,
+                                                 double *__restrict__ Lout, int accumulate)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    if (X >= ncell) return;
+    constexpr int NT = nterms(P);
+    double L[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) L[k] = accumulate ? Lout[(size_t)X * NT + k] : 0.0;
+    const CellGeom gX = geom[X];
+    for (int64_t e = off[X]; e < off[X + 1]; e++) {
+        const int32_t s = src[e];
+        const int B = entry_cell(s), sid = entry_shift(s);
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (sid) {
+            int k = sid <= 13 ? sid - 1 : sid;
+            sx = (double)(k / 9 - 1) * boxsize;
+            sy = (double)((k / 3) % 3 - 1) * boxsize;
+            sz = (double)(k % 3 - 1) * boxsize;
+        }
+        const CellGeom gB = geom[B];
+        const double rx = gX.c[0] - (gB.c[0] + sx);
+        const double ry = gX.c[1] - (gB.c[1] + sy);
+        const double rz = gX.c[2] - (gB.c[2] + sz);
+        const double r = sqrt(rx * rx + ry * ry + rz * rz);
+        double f[P + 1], D[NT], m[NT];
+        radial_f64<P>(r, inv2rs, f);
+        dtensor<P, double>(rx, ry, rz, f, D);
+#pragma unroll
+        for (int k = 0; k < NT; k++) m[k] = M[(size_t)B * NT + k];
+        m2l_contract<P, double>(D, m, L);
+    }
+#pragma unroll
+    for (int k = 0; k < NT; k++) Lout[(size_t)X * NT + k] = L[k];
+}
+
+template <int P>
+__global__ void __launch_bounds__(128) k_m2l_f32(int ncell, const CellGeom *__restrict__ geom,
+                                                 const int64_t *__restrict__ off, const int32_t *__restrict__ src,
+                                                 const float *__restrict__ Ms, double inv_rs, double boxsize,
+                                                 float *__restrict__ Lout, int accumulate)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    if (X >= ncell) return;
+    constexpr int NT = nterms(P);
+    float L[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) L[k] = accumulate ? Lout[(size_t)X * NT + k] : 0.0f;
+    const CellGeom gX = geom[X];
+    for (int64_t e = off[X]; e < off[X + 1]; e++) {
+        const int32_t s = src[e];
+        const int B = entry_cell(s), sid = entry_shift(s);
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (sid) {
+            int k = sid <= 13 ? sid - 1 : sid;
+            sx = (double)(k / 9 - 1) * boxsize;
+            sy = (double)((k / 3) % 3 - 1) * boxsize;
+            sz = (double)(k % 3 - 1) * boxsize;
+        }
+        const CellGeom gB = geom[B];
+        // relative coordinates formed in double, then to float in units of rs
+        const float rx = (float)((gX.c[0] - (gB.c[0] + sx)) * inv_rs);
+        const float ry = (float)((gX.c[1] - (gB.c[1] + sy)) * inv_rs);
+        const float rz = (float)((gX.c[2] - (gB.c[2] + sz)) * inv_rs);
+        const float r = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
+        float f[P + 1], D[NT], m[NT];
+        radial_f32_unit<P>(r, f);
+        dtensor<P, float>(rx, ry, rz, f, D);
+#pragma unroll
+        for (int k = 0; k < NT; k++) m[k] = Ms[(size_t)B * NT + k];
+        m2l_contract<P, float>(D, m, L);
+    }
+#pragma unroll
+    for (int k = 0; k < NT; k++) Lout[(size_t)X * NT + k] = L[k];
+}
+
+// ---------------------------------------------------------------------------- L2L
+
+template <int P, typename T>
+__global__ void k_l2l_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
+                            const CellInfo *__restrict__ info, double scale, T *__restrict__ L)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= count) return;
+    constexpr int NT = nterms(P);
+    const int v = nodes[t];
+    const CellInfo I = info[v];
+    const CellGeom gv = geom[v];
+    T lv[NT];
+#pragma unroll
+    for (int k = 0; k < NT; k++) lv[k] = L[(size_t)v * NT + k];
+    const int sons[2] = {I.son0, I.son1};
+    for (int s = 0; s < 2; s++) {
+        const int c = sons[s];
+        const CellGeom gc = geom[c];
+        T lc[NT];
+#pragma unroll
+        for (int k = 0; k < NT; k++) lc[k] = L[(size_t)c * NT + k];
+        l2l_add<P, T>((T)((gc.c[0] - gv.c[0]) * scale), (T)((gc.c[1] - gv.c[1]) * scale),
+                      (T)((gc.c[2] - gv.c[2]) * scale), lv, lc);
+#pragma unroll
+        for (int k = 0; k < NT; k++) L[(size_t)c * NT + k] = lc[k];
+    }
+}
+
+// ---------------------------------------------------------------------------- offsets
+
+__global__ void k_offsets_f32(int nleaf, const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
+                              const double *__restrict__ px, const double *__restrict__ py,
+                              const double *__restrict__ pz, float *__restrict__ ox, float *__restrict__ oy,
+                              float *__restrict__ oz)
+{
+    // one wavefront per leaf
+    int l = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
+    int lane = threadIdx.x % 64;
+    if (l >= nleaf) return;
+    const CellInfo I = info[l];
+    const CellGeom g = geom[l];
+    for (int q = lane; q < I.npart; q += 64) {
+        int i = I.ipart + q;
+        ox[i] = (float)(px[i] - g.c[0]);
+        oy[i] = (float)(py[i] - g.c[1]);
+        oz[i] = (float)(pz[i] - g.c[2]);
+    }
+}
+
+// ---------------------------------------------------------------------------- P2P + L2P
+
+struct P2PArgs {
+    const CellGeom *geom;
+    const CellInfo *info;
+    const int64_t *off;
+    const int32_t *src;
+    const double *px, *py, *pz;
+    const float *ox, *oy, *oz;
+    const int32_t *pidx;
+    const void *L;  // leaf expansions (double or float)
+    double mass, soft, inv2rs, inv_rs, boxsize;
+    int nleaf;
+    int with_l2p;
+    int accumulate;       // out += instead of out =
+    int out_caller_order; // scatter through pidx into out[3*pidx]; else tree order
+    double *out;          // [n][3]
+};
+
+__device__ __forceinline__ void shift_vec(int sid, double L, double &sx, double &sy, double &sz)
+{
+    sx = sy = sz = 0.0;
+    if (sid) {
+        int k = sid <= 13 ? sid - 1 : sid;
+        sx = (double)(k / 9 - 1) * L;
+        sy = (double)((k / 3) % 3 - 1) * L;
+        sz = (double)(k % 3 - 1) * L;
+    }
+}
+
+__device__ __forceinline__ void store_acc(const P2PArgs &a, int i, double ax, double ay, double az)
+{
+    size_t o = 3 * (size_t)(a.out_caller_order ? a.pidx[i] : i);
+    if (a.accumulate) {
+        ax += a.out[o];
+        ay += a.out[o + 1];
+        az += a.out[o + 2];
+    }
+    a.out[o] = ax;
+    a.out[o + 1] = ay;
+    a.out[o + 2] = az;
+}
+
+template <int P>
+__global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
+{
+    __shared__ double sx_[kP2PMaxLeaf], sy_[kP2PMaxLeaf], sz_[kP2PMaxLeaf];
+    __shared__ double red[3][64];
+    const int T = blockIdx.x;
+    const int lane = threadIdx.x;
+    const CellInfo IT = a.info[T];
+    const int nT = IT.npart;
+    if (nT == 0) return;
+    const int nsub = nT >= 32 ? (nT > 64 ? 1 : 64 / nT) : min(64 / nT, 32);
+    const int t = lane % nT, k = lane / nT;
+    const bool active = k < nsub && t < nT;
+    const int it = IT.ipart + t;
+    const double xt = a.px[it], yt = a.py[it], zt = a.pz[it];
+    const double m = a.mass, eps = a.soft, meps3 = m / (eps * eps * eps);
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int64_t e = a.off[T]; e < a.off[T + 1]; e++) {
+        const int32_t s = a.src[e];
+        const int S = entry_cell(s), sid = entry_shift(s);
+        const CellInfo IS = a.info[S];
+        double shx, shy, shz;
+        shift_vec(sid, a.boxsize, shx, shy, shz);
+        if (lane < IS.npart) {
+            const int j = IS.ipart + lane;
+            sx_[lane] = a.px[j] + shx;  // image body = pos + shift (src/remotes.c:84-86)
+            sy_[lane] = a.py[j] + shy;
+            sz_[lane] = a.pz[j] + shz;
+        }
+        __syncthreads();
+        if (active) {
+            for (int q = k; q < IS.npart; q += nsub) {
+                const double dx = sx_[q] - xt, dy = sy_[q] - yt, dz = sz_[q] - zt;
+                const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+                double f = dr < eps ? meps3 : m / (dr * dr * dr);
+                f *= split_factor_f64(dr * a.inv2rs);
+                ax += dx * f;
+                ay += dy * f;
+                az += dz * f;
+            }
+        }
+        __syncthreads();
+    }
+    red[0][lane] = ax;
+    red[1][lane] = ay;
+    red[2][lane] = az;
+    __syncthreads();
+    if (lane < nT) {
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        for (int kk = 0; kk < nsub; kk++) {
+            bx += red[0][lane + kk * nT];
+            by += red[1][lane + kk * nT];
+            bz += red[2][lane + kk * nT];
+        }
+        if (a.with_l2p) {
+            constexpr int NT = nterms(P);
+            const double *Lp = reinterpret_cast<const double *>(a.L) + (size_t)T * NT;
+            double Lr[NT];
+#pragma unroll
+            for (int q = 0; q < NT; q++) Lr[q] = Lp[q];
+            const CellGeom g = a.geom[T];
+            double lx, ly, lz;
+            l2p_eval<P, double>(Lr, xt - g.c[0], yt - g.c[1], zt - g.c[2], lx, ly, lz);
+            bx += lx;
+            by += ly;
+            bz += lz;
+        }
+        store_acc(a, it, bx, by, bz);
+    }
+}
+
+template <int P>
+__global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
+{
+    __shared__ float sx_[kP2PMaxLeaf], sy_[kP2PMaxLeaf], sz_[kP2PMaxLeaf];
+    __shared__ float red[3][64];
+    const int T = blockIdx.x;
+    const int lane = threadIdx.x;
+    const CellInfo IT = a.info[T];
+    const int nT = IT.npart;
+    if (nT == 0) return;
+    const int nsub = nT >= 32 ? (nT > 64 ? 1 : 64 / nT) : min(64 / nT, 32);
+    const int t = lane % nT, k = lane / nT;
+    const bool active = k < nsub && t < nT;
+    const int it = IT.ipart + t;
+    const float oxt = a.ox[it], oyt = a.oy[it], ozt = a.oz[it];
+    const CellGeom gT = a.geom[T];
+    const float m = (float)a.mass, eps = (float)a.soft;
+    const float meps3 = (float)(a.mass / (a.soft * a.soft * a.soft));
+    const float inv2rs = (float)a.inv2rs;
+    float ax = 0.f, ay = 0.f, az = 0.f;
+    for (int64_t e = a.off[T]; e < a.off[T + 1]; e++) {
+        const int32_t s = a.src[e];
+        const int S = entry_cell(s), sid = entry_shift(s);
+        const CellInfo IS = a.info[S];
+        const CellGeom gS = a.geom[S];
+        double shx, shy, shz;
+        shift_vec(sid, a.boxsize, shx, shy, shz);
+        // centre-to-centre offset in double, then float
+        const float bx = (float)((gS.c[0] + shx) - gT.c[0]) - oxt;
+        const float by = (float)((gS.c[1] + shy) - gT.c[1]) - oyt;
+        const float bz = (float)((gS.c[2] + shz) - gT.c[2]) - ozt;
+        if (lane < IS.npart) {
+            const int j = IS.ipart + lane;
+            sx_[lane] = a.ox[j];
+            sy_[lane] = a.oy[j];
+            sz_[lane] = a.oz[j];
+        }
+        __syncthreads();
+        if (active) {
+            for (int q = k; q < IS.npart; q += nsub) {
+                const float dx = bx + sx_[q], dy = by + sy_[q], dz = bz + sz_[q];
+                const float r2 = dx * dx + dy * dy + dz * dz;
+                const float ri = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
+                const float r = r2 * ri;
+                float f = r < eps ? meps3 : m * ri * ri * ri;
+                f *= split_factor_f32(r * inv2rs);
+                ax += dx * f;
+                ay += dy * f;
+                az += dz * f;
+            }
+        }
+        __syncthreads();
+    }
+    red[0][lane] = ax;
+    red[1][lane] = ay;
+    red[2][lane] = az;
+    __syncthreads();
+    if (lane < nT) {
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        for (int kk = 0; kk < nsub; kk++) {
+            bx += red[0][lane + kk * nT];
+            by += red[1][lane + kk * nT];
+            bz += red[2][lane + kk * nT];
+        }
+        if (a.with_l2p) {
+            constexpr int NT = nterms(P);
+            const float *Lp = reinterpret_cast<const float *>(a.L) + (size_t)T * NT;
+            float Lr[NT];
+#pragma unroll
+            for (int q = 0; q < NT; q++) Lr[q] = Lp[q];
+            const float is = (float)a.inv_rs;
+            float lx, ly, lz;
+            l2p_eval<P, float>(Lr, oxt * is, oyt * is, ozt * is, lx, ly, lz);
+            const float is2 = is * is;
+            bx += lx * is2;
+            by += ly * is2;
+            bz += lz * is2;
+        }
+        store_acc(a, it, (double)bx, (double)by, (double)bz);
+    }
+}
+
+// ---------------------------------------------------------------------------- host side
+
+static inline int nblk(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+int nterms_of(int order) { return nterms(order); }
+
+#define PH_DISPATCH(order, F)                                      \
+    switch (order) {                                               \
+    case 3: F(3); break;                                           \
+    case 4: F(4); break;                                           \
+    case 5: F(5); break;                                           \
+    default: throw std::runtime_error("expansion order must be 3, 4 or 5"); \
+    }
+
+void compute_offsets_f32(Ctx &c)
+{
+    float *ox = reinterpret_cast<float *>(grow(c.ox, sizeof(float) * c.n));
+    float *oy = reinterpret_cast<float *>(grow(c.oy, sizeof(float) * c.n));
+    float *oz = reinterpret_cast<float *>(grow(c.oz, sizeof(float) * c.n));
+    k_offsets_f32<<<nblk(c.nleaf, 4), 256, 0, c.stream>>>(c.nleaf, dptr<CellGeom>(c.geom), dptr<CellInfo>(c.info),
+                                                          dptr<double>(c.px), dptr<double>(c.py), dptr<double>(c.pz),
+                                                          ox, oy, oz);
+}
+
+void upward_pass(Ctx &c)
+{
+    const int NT = nterms(c.prm.order);
+    const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+    // FP64 multipoles always; the FP32-mixed path stores a scaled float copy after them
+    size_t mbytes = sizeof(double) * (size_t)c.ncell * NT;
+    double *M = reinterpret_cast<double *>(grow(c.M, mbytes + (f32 ? sizeof(float) * (size_t)c.ncell * NT : 0)));
+    hipStream_t st = c.stream;
+    const CellGeom *geom = dptr<CellGeom>(c.geom);
+    const CellInfo *info = dptr<CellInfo>(c.info);
+    const int *lv = dptr<int>(c.level_nodes);
+#define P2M_L(PP)                                                                                          \
+    k_p2m<PP><<<nblk(c.nleaf, 128), 128, 0, st>>>(c.nleaf, geom, info, dptr<double>(c.px), dptr<double>(c.py), \
+                                                  dptr<double>(c.pz), c.prm.mass, M);                       \
+    for (int d = c.nlevel - 1; d >= 0; d--) {                                                              \
+        int b = c.level_off[d], e = c.level_off[d + 1];                                                    \
+        k_m2m_level<PP><<<nblk(e - b, 128), 128, 0, st>>>(lv + b, e - b, geom, info, M);                  \
+    }                                                                                                      \
+    if (f32)                                                                                               \
+        k_scale_down<PP><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, M, 1.0 / c.prm.rs,                  \
+                                                             reinterpret_cast<float *>(M + (size_t)c.ncell * NT));
+    PH_DISPATCH(c.prm.order, P2M_L)
+#undef P2M_L
+    PH_CHECK(hipGetLastError());
+}
+
+void m2l_pass(Ctx &c, bool accumulate)
+{
+    const int NT = nterms(c.prm.order);
+    const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+    size_t lbytes = (f32 ? sizeof(float) : sizeof(double)) * (size_t)c.ncell * NT;
+    void *L = grow(c.L, lbytes);
+    hipStream_t st = c.stream;
+    const CellGeom *geom = dptr<CellGeom>(c.geom);
+    const int64_t *off = dptr<int64_t>(c.m2l_off);
+    const int32_t *src = dptr<int32_t>(c.m2l_src);
+    double *M = dptr<double>(c.M);
+#define M2L_L(PP)                                                                                                 \
+    if (f32)                                                                                                      \
+        k_m2l_f32<PP><<<nblk(c.ncell, 128), 128, 0, st>>>(                                                        \
+            c.ncell, geom, off, src, reinterpret_cast<const float *>(M + (size_t)c.ncell * NT), 1.0 / c.prm.rs,    \
+            c.prm.boxsize, reinterpret_cast<float *>(L), accumulate);                                             \
+    else                                                                                                          \
+        k_m2l_f64<PP><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, geom, off, src, M, 0.5 / c.prm.rs,            \
+                                                          c.prm.boxsize, reinterpret_cast<double *>(L), accumulate);
+    PH_DISPATCH(c.prm.order, M2L_L)
+#undef M2L_L
+    PH_CHECK(hipGetLastError());
+}
+
+void downward_pass(Ctx &c)
+{
+    const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+    hipStream_t st = c.stream;
+    const CellGeom *geom = dptr<CellGeom>(c.geom);
+    const CellInfo *info = dptr<CellInfo>(c.info);
+    const int *lv = dptr<int>(c.level_nodes);
+#define L2L_L(PP)                                                                                              \
+    for (int d = 0; d < c.nlevel; d++) {                                                                       \
+        int b = c.level_off[d], e = c.level_off[d + 1];                                                        \
+        if (f32)                                                                                               \
+            k_l2l_level<PP, float><<<nblk(e - b, 128), 128, 0, st>>>(lv + b, e - b, geom, info, 1.0 / c.prm.rs, \
+                                                                     dptr<float>(c.L));                        \
+        else                                                                                                   \
+            k_l2l_level<PP, double><<<nblk(e - b, 128), 128, 0, st>>>(lv + b, e - b, geom, info, 1.0,          \
+                                                                      dptr<double>(c.L));                      \
+    }
+    PH_DISPATCH(c.prm.order, L2L_L)
+#undef L2L_L
+    PH_CHECK(hipGetLastError());
+}
+
+void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, double *out)
+{
+    if (c.prm.maxleaf > kP2PMaxLeaf) throw std::runtime_error("maxleaf above 64 is not supported");
+    P2PArgs a;
+    a.geom = dptr<CellGeom>(c.geom);
+    a.info = dptr<CellInfo>(c.info);
+    a.off = dptr<int64_t>(c.p2p_off);
+    a.src = dptr<int32_t>(c.p2p_src);
+    a.px = dptr<double>(c.px);
+    a.py = dptr<double>(c.py);
+    a.pz = dptr<double>(c.pz);
+    a.ox = dptr<float>(c.ox);
+    a.oy = dptr<float>(c.oy);
+    a.oz = dptr<float>(c.oz);
+    a.pidx = dptr<int32_t>(c.pidx);
+    a.L = c.L.p;
+    a.mass = c.prm.mass;
+    a.soft = c.prm.soft;
+    a.inv2rs = 0.5 / c.prm.rs;
+    a.inv_rs = 1.0 / c.prm.rs;
+    a.boxsize = c.prm.boxsize;
+    a.nleaf = c.nleaf;
+    a.with_l2p = with_l2p;
+    a.accumulate = accumulate;
+    a.out_caller_order = caller_order;
+    a.out = out;
+    const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+    hipStream_t st = c.stream;
+#define P2P_L(PP)                                              \
+    if (f32) k_p2p_f32<PP><<<c.nleaf, 64, 0, st>>>(a);         \
+    else k_p2p_f64<PP><<<c.nleaf, 64, 0, st>>>(a);
+    if (c.nleaf > 0) PH_DISPATCH(c.prm.order, P2P_L)
+#undef P2P_L
+    PH_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------- batched operators
+
+template <int P>
+__global__ void k_op_p2m(int count, const double *pos, const int32_t *first, const int32_t *npart,
+                         const double *center, double mass, double *M)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    constexpr int NT = nterms(P);
+    double m[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) m[q] = 0.0;
+    for (int i = first[k]; i < first[k] + npart[k]; i++)
+        p2m_add<P, double>(mass, pos[3 * i] - center[3 * k], pos[3 * i + 1] - center[3 * k + 1],
+                           pos[3 * i + 2] - center[3 * k + 2], m);
+#pragma unroll
+    for (int q = 0; q < NT; q++) M[(size_t)k * NT + q] = m[q];
+}
+
+template <int P, int KIND>  // 0 m2m, 1 l2l
+__global__ void k_op_shift(int count, const double *d, const double *X, double *Y)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    constexpr int NT = nterms(P);
+    double x[NT], y[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        x[q] = X[(size_t)k * NT + q];
+        y[q] = Y[(size_t)k * NT + q];
+    }
+    if (KIND == 0) m2m_add<P, double>(d[3 * k], d[3 * k + 1], d[3 * k + 2], x, y);
+    else l2l_add<P, double>(d[3 * k], d[3 * k + 1], d[3 * k + 2], x, y);
+#pragma unroll
+    for (int q = 0; q < NT; q++) Y[(size_t)k * NT + q] = y[q];
+}
+
+template <int P>
+__global__ void k_op_m2l(int count, double inv2rs, const double *R, const double *M, double *L)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    constexpr int NT = nterms(P);
+    double m[NT], l[NT], f[P + 1], D[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) {
+        m[q] = M[(size_t)k * NT + q];
+        l[q] = L[(size_t)k * NT + q];
+    }
+    const double rx = R[3 * k], ry = R[3 * k + 1], rz = R[3 * k + 2];
+    radial_f64<P>(sqrt(rx * rx + ry * ry + rz * rz), inv2rs, f);
+    dtensor<P, double>(rx, ry, rz, f, D);
+    m2l_contract<P, double>(D, m, l);
+#pragma unroll
+    for (int q = 0; q < NT; q++) L[(size_t)k * NT + q] = l[q];
+}
+
+template <int P>
+__global__ void k_op_l2p(int n, const double *pos, const int32_t *owner, const double *center, const double *L,
+                         double *acc)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    constexpr int NT = nterms(P);
+    const int o = owner[i];
+    double l[NT];
+#pragma unroll
+    for (int q = 0; q < NT; q++) l[q] = L[(size_t)o * NT + q];
+    double ax, ay, az;
+    l2p_eval<P, double>(l, pos[3 * i] - center[3 * o], pos[3 * i + 1] - center[3 * o + 1],
+                        pos[3 * i + 2] - center[3 * o + 2], ax, ay, az);
+    acc[3 * i] += ax;
+    acc[3 * i + 1] += ay;
+    acc[3 * i + 2] += az;
+}
+
+void op_p2m(hipStream_t st, int order, int count, const double *pos, const int32_t *first, const int32_t *npart,
+            const double *center, double mass, double *M)
+{
+#define L_(PP) k_op_p2m<PP><<<nblk(count, 64), 64, 0, st>>>(count, pos, first, npart, center, mass, M);
+    PH_DISPATCH(order, L_)
+#undef L_
+    PH_CHECK(hipGetLastError());
+}
+
+void op_shift(hipStream_t st, int order, int kind, int count, const double *d, const double *X, double *Y)
+{
+#define L_(PP)                                                                              \
+    if (kind == 0) k_op_shift<PP, 0><<<nblk(count, 64), 64, 0, st>>>(count, d, X, Y);       \
+    else k_op_shift<PP, 1><<<nblk(count, 64), 64, 0, st>>>(count, d, X, Y);
+    PH_DISPATCH(order, L_)
+#undef L_
+    PH_CHECK(hipGetLastError());
+}
+
+void op_m2l(hipStream_t st, int order, int count, double rs, const double *R, const double *M, double *L)
+{
+#define L_(PP) k_op_m2l<PP><<<nblk(count, 64), 64, 0, st>>>(count, 0.5 / rs, R, M, L);
+    PH_DISPATCH(order, L_)
+#undef L_
+    PH_CHECK(hipGetLastError());
+}
+
+void op_l2p(hipStream_t st, int order, int n, const double *pos, const int32_t *owner, const double *center,
+            const double *L, double *acc)
+{
+#define L_(PP) k_op_l2p<PP><<<nblk(n, 64), 64, 0, st>>>(n, pos, owner, center, L, acc);
+    PH_DISPATCH(order, L_)
+#undef L_
+    PH_CHECK(hipGetLastError());
+}
+
+}  // namespace photons

@@ -1,0 +1,570 @@
+// tree.hip -- kd tree of the short-range force on the GPU.
+//
+// Same tree as build_kdtree/bksort_inplace/center_kdtree (src/fmm.c:30-177): a node
+// is split at the mean of its particles' coordinate along the axis of its depth
+// (x at the root, cycling), particles <= mean go left; a child with at most maxleaf
+// particles is a leaf (possibly empty), boxes are cut at the split planes from
+// the domain box [0, L)^3.  When no particle lies above the mean the last one goes
+// right, as the reference partition does.
+//
+// MI355X formulation: one level of the tree per step over all nodes of that
+// level.  Each node's particle range is cut into chunks of kChunk elements, one
+// wavefront per chunk: a deterministic per-chunk sum (fixed lane order and
+// butterfly), per-node mean from the chunk partials in chunk order, a stable
+// partition whose per-chunk offsets come from one exclusive scan.  After the last
+// level, subtree sizes give the depth-first numbering of the reference (leaves
+// in creation order, nodes in pre-order).  The mean's rounding differs from the
+// reference's sequential sum by a few ulps; the partition (which particles go
+// left) is the same except for particles within those ulps of the plane.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <stdexcept>
+
+#include "engine.h"
+
+namespace photons {
+
+constexpr int kChunk = 2048;
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 4;
+
+__device__ __forceinline__ int find_seg(const int *__restrict__ seg_chunk, int S, int c)
+{
+    int lo = 0, hi = S;  // seg_chunk[lo] <= c < seg_chunk[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (seg_chunk[mid] <= c) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+struct LevelArgs {
+    const double *coord;  // the split axis of this level
+    const int *t_start, *t_count;
+    int seg0, S, C;
+    const int *seg_chunk;  // [S+1]
+};
+
+__global__ void __launch_bounds__(256) k_chunk_sum(LevelArgs a, double *__restrict__ partial)
+{
+    int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    int lane = threadIdx.x % kWave;
+    if (c >= a.C) return;
+    int s = find_seg(a.seg_chunk, a.S, c);
+    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
+    int b = st + (c - a.seg_chunk[s]) * kChunk;
+    int e = min(b + kChunk, st + ct);
+    double v = 0.0;
+    for (int i = b + lane; i < e; i += kWave) v += a.coord[i];
+    v = wave_sum(v);
+    if (lane == 0) partial[c] = v;
+}
+
+__global__ void k_nchunks(const int *__restrict__ cnt, int seg0, int S, int *__restrict__ out)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > S) return;
+    out[s] = (s == S) ? 0 : (cnt[seg0 + s] + kChunk - 1) / kChunk;
+}
+
+__global__ void k_mean(LevelArgs a, const double *__restrict__ partial, double *__restrict__ t_split)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.S) return;
+    double v = 0.0;
+    for (int c = a.seg_chunk[s]; c < a.seg_chunk[s + 1]; c++) v += partial[c];
+    t_split[a.seg0 + s] = v / (double)a.t_count[a.seg0 + s];
+}
+
+__global__ void __launch_bounds__(256) k_chunk_count(LevelArgs a, const double *__restrict__ t_split,
+                                                     int *__restrict__ rcnt)
+{
+    int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    int lane = threadIdx.x % kWave;
+    if (c > a.C) return;
+    if (c == a.C) {  // sentinel so that the exclusive scan ends with the total
+        if (lane == 0) rcnt[c] = 0;
+        return;
+    }
+    int s = find_seg(a.seg_chunk, a.S, c);
+    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
+    double split = t_split[a.seg0 + s];
+    int b = st + (c - a.seg_chunk[s]) * kChunk;
+    int e = min(b + kChunk, st + ct);
+    int n = 0;
+    for (int i = b + lane; i < e; i += kWave) n += a.coord[i] > split;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) n += __shfl_xor(n, o, kWave);
+    if (lane == 0) rcnt[c] = n;
+}
+
+__global__ void k_seg_counts(LevelArgs a, const int *__restrict__ rpre, int *__restrict__ nleft,
+                             int *__restrict__ flag)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.S) return;
+    int ct = a.t_count[a.seg0 + s];
+    int nr = rpre[a.seg_chunk[s + 1]] - rpre[a.seg_chunk[s]];
+    int nl = ct - nr, f = 0;
+    if (nr == 0) {  // nothing above the mean: the last particle goes right
+        nl = ct - 1;
+        f = 1;
+    } else if (nl == 0) {  // nothing at or below it (rounding): the first goes left
+        nl = 1;
+        f = 2;
+    }
+    nleft[s] = nl;
+    flag[s] = f;
+}
+
+struct Particles {
+    double *x, *y, *z;
+    int *idx;
+};
+
+__global__ void __launch_bounds__(256) k_scatter(LevelArgs a, const double *__restrict__ t_split,
+                                                 const int *__restrict__ rpre, const int *__restrict__ nleft,
+                                                 const int *__restrict__ flag, Particles src, Particles dst)
+{
+    int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    int lane = threadIdx.x % kWave;
+    if (c >= a.C) return;
+    int s = find_seg(a.seg_chunk, a.S, c);
+    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
+    int ci = c - a.seg_chunk[s];
+    int b = st + ci * kChunk;
+    int e = min(b + kChunk, st + ct);
+    double split = t_split[a.seg0 + s];
+    int f = flag[s];
+    int nl = nleft[s];
+    int rbase = rpre[c] - rpre[a.seg_chunk[s]];
+    int lbase = ci * kChunk - rbase;
+    for (int i0 = b; i0 < e; i0 += kWave) {
+        int i = i0 + lane;
+        bool active = i < e;
+        bool right = false;
+        if (active) right = (f == 0) ? (a.coord[i] > split) : false;
+        uint64_t bal = __ballot(active && right);
+        uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+        int rr = __popcll(bal & lt);
+        int rl = lane - rr;
+        if (active) {
+            int d = (f != 0) ? i : (right ? st + nl + rbase + rr : st + lbase + rl);
+            dst.x[d] = src.x[i];
+            dst.y[d] = src.y[i];
+            dst.z[d] = src.z[i];
+            dst.idx[d] = src.idx[i];
+        }
+        int nact = min(kWave, e - i0);
+        int nr = __popcll(bal);
+        rbase += nr;
+        lbase += nact - nr;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_copy_back(LevelArgs a, Particles src, Particles dst)
+{
+    int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    int lane = threadIdx.x % kWave;
+    if (c >= a.C) return;
+    int s = find_seg(a.seg_chunk, a.S, c);
+    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
+    int b = st + (c - a.seg_chunk[s]) * kChunk;
+    int e = min(b + kChunk, st + ct);
+    for (int i = b + lane; i < e; i += kWave) {
+        dst.x[i] = src.x[i];
+        dst.y[i] = src.y[i];
+        dst.z[i] = src.z[i];
+        dst.idx[i] = src.idx[i];
+    }
+}
+
+__global__ void k_child_kinds(LevelArgs a, const int *__restrict__ nleft, int maxleaf, int *__restrict__ nn,
+                              int *__restrict__ nl)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > a.S) return;
+    if (s == a.S) {
+        nn[s] = 0;
+        nl[s] = 0;
+        return;
+    }
+    int ct = a.t_count[a.seg0 + s];
+    int c0 = nleft[s], c1 = ct - c0;
+    int k = (c0 > maxleaf) + (c1 > maxleaf);
+    nn[s] = k;
+    nl[s] = 2 - k;
+}
+
+struct TreeTmp {
+    int *t_start, *t_count, *t_son, *t_depth, *t_parent;
+    double *t_split, *t_lo, *t_hi;  // lo/hi: [3][cap] SoA
+    int *l_start, *l_count, *l_parent;
+    double *l_lo, *l_hi;
+    int tcap, lcap;
+};
+
+__global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, const int *__restrict__ nleft,
+                                const int *__restrict__ non, const int *__restrict__ lon, int node_base,
+                                int leaf_base)
+{
+    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= a.S) return;
+    int v = a.seg0 + s;
+    int st = T.t_start[v], ct = T.t_count[v];
+    int cnt[2] = {nleft[s], ct - nleft[s]};
+    int beg[2] = {st, st + nleft[s]};
+    double split = T.t_split[v];
+    int depth = T.t_depth[v];
+    int nextn = node_base + non[s], nextl = leaf_base + lon[s];
+    for (int k = 0; k < 2; k++) {
+        double lo[3], hi[3];
+        for (int d = 0; d < 3; d++) {
+            lo[d] = T.t_lo[d * T.tcap + v];
+            hi[d] = T.t_hi[d * T.tcap + v];
+        }
+        if (k == 0) hi[axis] = split;
+        else lo[axis] = split;
+        if (cnt[k] > maxleaf) {
+            int id = nextn++;
+            T.t_start[id] = beg[k];
+            T.t_count[id] = cnt[k];
+            T.t_depth[id] = depth + 1;
+            T.t_parent[id] = v;
+            for (int d = 0; d < 3; d++) {
+                T.t_lo[d * T.tcap + id] = lo[d];
+                T.t_hi[d * T.tcap + id] = hi[d];
+            }
+            T.t_son[2 * v + k] = id;
+        } else {
+            int id = nextl++;
+            T.l_start[id] = beg[k];
+            T.l_count[id] = cnt[k];
+            T.l_parent[id] = v;
+            for (int d = 0; d < 3; d++) {
+                T.l_lo[d * T.lcap + id] = lo[d];
+                T.l_hi[d * T.lcap + id] = hi[d];
+            }
+            T.t_son[2 * v + k] = -(id + 1);
+        }
+    }
+}
+
+// ---- final numbering
+
+__global__ void k_sizes(const int *__restrict__ t_son, int b, int e, int *__restrict__ size, int *__restrict__ lsize)
+{
+    int v = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= e) return;
+    int sz = 1, ls = 0;
+    for (int k = 0; k < 2; k++) {
+        int c = t_son[2 * v + k];
+        if (c < 0) {
+            sz += 1;
+            ls += 1;
+        } else {
+            sz += size[c];
+            ls += lsize[c];
+        }
+    }
+    size[v] = sz;
+    lsize[v] = ls;
+}
+
+__global__ void k_number(const int *__restrict__ t_son, int b, int e, const int *__restrict__ size,
+                         const int *__restrict__ lsize, int *__restrict__ dfs, int *__restrict__ lfirst,
+                         int *__restrict__ l_final, int *__restrict__ l_dfs)
+{
+    int v = b + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= e) return;
+    int d = dfs[v], lf = lfirst[v];
+    int c0 = t_son[2 * v], c1 = t_son[2 * v + 1];
+    int s0 = c0 < 0 ? 1 : size[c0];
+    int ls0 = c0 < 0 ? 1 : lsize[c0];
+    int dd[2] = {d + 1, d + 1 + s0};
+    int ll[2] = {lf, lf + ls0};
+    int cs[2] = {c0, c1};
+    for (int k = 0; k < 2; k++) {
+        int c = cs[k];
+        if (c < 0) {
+            l_final[-c - 1] = ll[k];
+            l_dfs[-c - 1] = dd[k];
+        } else {
+            dfs[c] = dd[k];
+            lfirst[c] = ll[k];
+        }
+    }
+}
+
+__global__ void k_emit_nodes(int ntmp, int nleaf, const int *__restrict__ dfs, const int *__restrict__ lfirst,
+                             const int *__restrict__ size, TreeTmp T, const int *__restrict__ l_final,
+                             CellGeom *__restrict__ geom, CellInfo *__restrict__ info, int *__restrict__ level_nodes)
+{
+    int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= ntmp) return;
+    int f = nleaf + dfs[v] - lfirst[v];
+    CellGeom g;
+    for (int d = 0; d < 3; d++) {
+        double lo = T.t_lo[d * T.tcap + v], hi = T.t_hi[d * T.tcap + v];
+        g.c[d] = 0.5 * (hi + lo);
+        g.w[d] = hi - lo;
+    }
+    geom[f] = g;
+    CellInfo I;
+    I.npart = T.t_count[v];
+    I.ipart = T.t_start[v];
+    int cs[2];
+    for (int k = 0; k < 2; k++) {
+        int c = T.t_son[2 * v + k];
+        cs[k] = c < 0 ? l_final[-c - 1] : nleaf + dfs[c] - lfirst[c];
+    }
+    I.son0 = cs[0];
+    I.son1 = cs[1];
+    I.dfs = dfs[v];
+    I.dsize = size[v];
+    int p = T.t_parent[v];
+    I.parent = p < 0 ? -1 : nleaf + dfs[p] - lfirst[p];
+    I.depth = T.t_depth[v];
+    info[f] = I;
+    level_nodes[v] = f;
+}
+
+__global__ void k_emit_leaves(int nl, int nleaf, const int *__restrict__ l_final, const int *__restrict__ l_dfs,
+                              const int *__restrict__ dfs, const int *__restrict__ lfirst, TreeTmp T,
+                              CellGeom *__restrict__ geom, CellInfo *__restrict__ info)
+{
+    int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= nl) return;
+    int f = l_final[l];
+    CellGeom g;
+    for (int d = 0; d < 3; d++) {
+        double lo = T.l_lo[d * T.lcap + l], hi = T.l_hi[d * T.lcap + l];
+        g.c[d] = 0.5 * (hi + lo);
+        g.w[d] = hi - lo;
+    }
+    geom[f] = g;
+    CellInfo I;
+    I.npart = T.l_count[l];
+    I.ipart = T.l_start[l];
+    I.son0 = I.son1 = -1;
+    I.dfs = l_dfs[l];
+    I.dsize = 1;
+    int p = T.l_parent[l];
+    I.parent = nleaf + dfs[p] - lfirst[p];
+    I.depth = T.t_depth[p] + 1;
+    info[f] = I;
+}
+
+__global__ void k_init_particles(int64_t n, const double *__restrict__ pos, Particles p)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    p.x[i] = pos[3 * i];
+    p.y[i] = pos[3 * i + 1];
+    p.z[i] = pos[3 * i + 2];
+    p.idx[i] = (int)i;
+}
+
+// ---------------------------------------------------------------------------- host side
+
+static inline int blocks(int64_t n, int b) { return (int)((n + b - 1) / b); }
+
+template <typename T>
+static T *ensure(DevBuf &b, size_t count)
+{
+    return reinterpret_cast<T *>(grow(b, count * sizeof(T)));
+}
+
+// grow a level-order array keeping its first `keep` elements (SoA width `width`)
+template <typename T>
+static T *regrow(DevBuf &b, size_t oldcap, size_t newcap, int width, hipStream_t s)
+{
+    T *old = reinterpret_cast<T *>(b.p);
+    T *nw = nullptr;
+    PH_CHECK(hipMalloc(&nw, newcap * width * sizeof(T)));
+    if (old) {
+        for (int w = 0; w < width; w++)
+            PH_CHECK(hipMemcpyAsync(nw + w * newcap, old + w * oldcap, oldcap * sizeof(T), hipMemcpyDeviceToDevice, s));
+        PH_CHECK(hipStreamSynchronize(s));
+        PH_CHECK(hipFree(old));
+    }
+    b.p = nw;
+    b.cap = newcap * width * sizeof(T);
+    return nw;
+}
+
+static void exclusive_scan(Ctx &c, const int *in, int *out, int n)
+{
+    size_t bytes = 0;
+    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, c.stream));
+    void *tmp = grow(c.scan_tmp, bytes);
+    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, c.stream));
+}
+
+void upload_positions(Ctx &c, const double *d_pos_aos)
+{
+    int64_t n = c.n;
+    Particles p{ensure<double>(c.px, n), ensure<double>(c.py, n), ensure<double>(c.pz, n), ensure<int>(c.pidx, n)};
+    if (n > 0) k_init_particles<<<blocks(n, 256), 256, 0, c.stream>>>(n, d_pos_aos, p);
+}
+
+void build_tree(Ctx &c)
+{
+    const int64_t n = c.n;
+    const int maxleaf = c.prm.maxleaf;
+    hipStream_t st = c.stream;
+    Particles A{dptr<double>(c.px), dptr<double>(c.py), dptr<double>(c.pz), dptr<int>(c.pidx)};
+    Particles B{ensure<double>(c.qx, n), ensure<double>(c.qy, n), ensure<double>(c.qz, n), ensure<int>(c.qidx, n)};
+
+    TreeTmp T;
+    auto alloc_nodes = [&](int newcap, int oldcap) {
+        T.t_start = regrow<int>(c.t_start, oldcap, newcap, 1, st);
+        T.t_count = regrow<int>(c.t_count, oldcap, newcap, 1, st);
+        T.t_son = regrow<int>(c.t_son, 2 * (size_t)oldcap, 2 * (size_t)newcap, 1, st);
+        T.t_depth = regrow<int>(c.t_depth, oldcap, newcap, 1, st);
+        T.t_parent = regrow<int>(c.t_parent, oldcap, newcap, 1, st);
+        T.t_split = regrow<double>(c.t_split, oldcap, newcap, 1, st);
+        T.t_lo = regrow<double>(c.t_lo, oldcap, newcap, 3, st);
+        T.t_hi = regrow<double>(c.t_hi, oldcap, newcap, 3, st);
+        T.tcap = c.tcap = newcap;
+    };
+    auto alloc_leaves = [&](int newcap, int oldcap) {
+        T.l_start = regrow<int>(c.l_start, oldcap, newcap, 1, st);
+        T.l_count = regrow<int>(c.l_count, oldcap, newcap, 1, st);
+        T.l_parent = regrow<int>(c.l_parent, oldcap, newcap, 1, st);
+        T.l_lo = regrow<double>(c.l_lo, oldcap, newcap, 3, st);
+        T.l_hi = regrow<double>(c.l_hi, oldcap, newcap, 3, st);
+        T.lcap = c.lcap = newcap;
+    };
+    // level-order temporaries: keep the capacity of earlier builds, start at ~4N/maxleaf
+    const int want = (int)std::max<int64_t>(1024, 4 * n / std::max(1, maxleaf) + 16);
+    if (c.tcap < want) {
+        alloc_nodes(want, 0);
+    } else {
+        T.t_start = dptr<int>(c.t_start); T.t_count = dptr<int>(c.t_count); T.t_son = dptr<int>(c.t_son);
+        T.t_depth = dptr<int>(c.t_depth); T.t_parent = dptr<int>(c.t_parent); T.t_split = dptr<double>(c.t_split);
+        T.t_lo = dptr<double>(c.t_lo); T.t_hi = dptr<double>(c.t_hi); T.tcap = c.tcap;
+    }
+    if (c.lcap < want + 1) {
+        alloc_leaves(want + 1, 0);
+    } else {
+        T.l_start = dptr<int>(c.l_start); T.l_count = dptr<int>(c.l_count); T.l_parent = dptr<int>(c.l_parent);
+        T.l_lo = dptr<double>(c.l_lo); T.l_hi = dptr<double>(c.l_hi); T.lcap = c.lcap;
+    }
+
+    // root
+    {
+        int zero = 0, cnt = (int)n, parent = -1;
+        double lo = 0.0, hi = c.prm.boxsize;
+        PH_CHECK(hipMemcpyAsync(T.t_start, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+        PH_CHECK(hipMemcpyAsync(T.t_count, &cnt, sizeof(int), hipMemcpyHostToDevice, st));
+        PH_CHECK(hipMemcpyAsync(T.t_depth, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+        PH_CHECK(hipMemcpyAsync(T.t_parent, &parent, sizeof(int), hipMemcpyHostToDevice, st));
+        for (int d = 0; d < 3; d++) {
+            PH_CHECK(hipMemcpyAsync(T.t_lo + d * T.tcap, &lo, sizeof(double), hipMemcpyHostToDevice, st));
+            PH_CHECK(hipMemcpyAsync(T.t_hi + d * T.tcap, &hi, sizeof(double), hipMemcpyHostToDevice, st));
+        }
+        PH_CHECK(hipStreamSynchronize(st));
+    }
+
+    int *pin = reinterpret_cast<int *>(c.pinned.p);
+    std::vector<int> lvl = {0, 1};  // temp node ids of each level
+    int ntmp = 1, nleaf = 0;
+    int depth = 0;
+    std::vector<int> seg_chunk_h;
+    while (lvl[depth + 1] > lvl[depth]) {
+        int seg0 = lvl[depth], S = lvl[depth + 1] - lvl[depth];
+        int axis = depth % 3;
+        int *seg_chunk = ensure<int>(c.seg_chunk, S + 1);
+        {
+            int *nch = ensure<int>(c.seg_nn, S + 1);
+            k_nchunks<<<blocks(S + 1, 256), 256, 0, st>>>(T.t_count, seg0, S, nch);
+            exclusive_scan(c, nch, seg_chunk, S + 1);
+            PH_CHECK(hipMemcpyAsync(pin, seg_chunk + S, sizeof(int), hipMemcpyDeviceToHost, st));
+            PH_CHECK(hipStreamSynchronize(st));
+        }
+        int C = pin[0];
+        const double *coord = axis == 0 ? A.x : (axis == 1 ? A.y : A.z);
+        LevelArgs a{coord, T.t_start, T.t_count, seg0, S, C, seg_chunk};
+        double *partial = ensure<double>(c.chunk_part, C + 1);
+        int *rcnt = ensure<int>(c.chunk_rcnt, 2 * (size_t)(C + 1));
+        int *rpre = rcnt + (C + 1);
+        int *nleft = ensure<int>(c.seg_nleft, S + 1);
+        int *flag = ensure<int>(c.seg_flag, S + 1);
+        int wb = blocks(C, kWavesPerBlock);
+        k_chunk_sum<<<wb, 256, 0, st>>>(a, partial);
+        k_mean<<<blocks(S, 256), 256, 0, st>>>(a, partial, T.t_split);
+        k_chunk_count<<<blocks(C + 1, kWavesPerBlock), 256, 0, st>>>(a, T.t_split, rcnt);
+        exclusive_scan(c, rcnt, rpre, C + 1);
+        k_seg_counts<<<blocks(S, 256), 256, 0, st>>>(a, rpre, nleft, flag);
+        k_scatter<<<wb, 256, 0, st>>>(a, T.t_split, rpre, nleft, flag, A, B);
+        k_copy_back<<<wb, 256, 0, st>>>(a, B, A);
+        int *nn = ensure<int>(c.seg_nn, S + 1);
+        int *nl = ensure<int>(c.seg_nl, S + 1);
+        int *non = ensure<int>(c.seg_non, S + 1);
+        int *lon = ensure<int>(c.seg_lon, S + 1);
+        k_child_kinds<<<blocks(S + 1, 256), 256, 0, st>>>(a, nleft, maxleaf, nn, nl);
+        exclusive_scan(c, nn, non, S + 1);
+        exclusive_scan(c, nl, lon, S + 1);
+        PH_CHECK(hipMemcpyAsync(pin, non + S, sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipMemcpyAsync(pin + 1, lon + S, sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipStreamSynchronize(st));
+        int new_nodes = pin[0], new_leaves = pin[1];
+        if (ntmp + new_nodes > T.tcap) {
+            int ncap = std::max(2 * T.tcap, ntmp + new_nodes + 1024);
+            alloc_nodes(ncap, T.tcap);
+            a.t_start = T.t_start;
+            a.t_count = T.t_count;
+        }
+        if (nleaf + new_leaves > T.lcap) alloc_leaves(std::max(2 * T.lcap, nleaf + new_leaves + 1024), T.lcap);
+        k_make_children<<<blocks(S, 256), 256, 0, st>>>(a, T, axis, maxleaf, nleft, non, lon, ntmp, nleaf);
+        ntmp += new_nodes;
+        nleaf += new_leaves;
+        lvl.push_back(ntmp);
+        depth++;
+        if (depth > kMaxDepth) throw std::runtime_error("tree deeper than the walk stack supports");
+    }
+    const int nlevels = depth;  // levels holding nodes: 0 .. depth-1
+
+    // subtree sizes, bottom-up
+    int *size = ensure<int>(c.t_size, ntmp);
+    int *lsize = ensure<int>(c.t_lsize, ntmp);
+    int *dfs = ensure<int>(c.t_dfs, ntmp);
+    int *lfirst = ensure<int>(c.t_lfirst, ntmp);
+    int *l_final = ensure<int>(c.l_final, 2 * (size_t)nleaf);
+    int *l_dfs = l_final + nleaf;
+    for (int d = nlevels - 1; d >= 0; d--) {
+        int b = lvl[d], e = lvl[d + 1];
+        k_sizes<<<blocks(e - b, 256), 256, 0, st>>>(T.t_son, b, e, size, lsize);
+    }
+    PH_CHECK(hipMemsetAsync(dfs, 0, sizeof(int), st));
+    PH_CHECK(hipMemsetAsync(lfirst, 0, sizeof(int), st));
+    for (int d = 0; d < nlevels; d++) {
+        int b = lvl[d], e = lvl[d + 1];
+        k_number<<<blocks(e - b, 256), 256, 0, st>>>(T.t_son, b, e, size, lsize, dfs, lfirst, l_final, l_dfs);
+    }
+    c.nleaf = nleaf;
+    c.nnode = ntmp;
+    c.ncell = nleaf + ntmp;
+    c.nlevel = nlevels;
+    c.maxdepth = nlevels;  // leaves sit at most at depth nlevels
+    CellGeom *geom = ensure<CellGeom>(c.geom, c.ncell);
+    CellInfo *info = ensure<CellInfo>(c.info, c.ncell);
+    int *level_nodes = ensure<int>(c.level_nodes, ntmp);
+    k_emit_nodes<<<blocks(ntmp, 256), 256, 0, st>>>(ntmp, nleaf, dfs, lfirst, size, T, l_final, geom, info,
+                                                     level_nodes);
+    k_emit_leaves<<<blocks(nleaf, 256), 256, 0, st>>>(nleaf, nleaf, l_final, l_dfs, dfs, lfirst, T, geom, info);
+    c.level_off.assign(lvl.begin(), lvl.end());
+    PH_CHECK(hipGetLastError());
+}
+
+}  // namespace photons

@@ -1,0 +1,61 @@
+"""The C-ABI library: it loads, exports every entry point include/photons_fmm.h declares,
+and refuses to run without a GPU instead of computing anything on the CPU."""
+import ctypes as C
+import os
+import subprocess
+
+import pytest
+
+import photons_gpu as pg
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.fixture(scope="module")
+def lib():
+    if not os.path.exists(pg.LIB_PATH):
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "photons-2.0_amd")], check=True)
+    return pg.lib()
+
+
+def test_every_declared_symbol_is_exported(lib):
+    names = pg.declared_symbols()
+    # the drop-in layer and the native API are both there
+    for must in ("fmm_prepare", "fmm_task", "fmm_ext", "p2m", "m2l", "l2p", "p2p_kernel", "walk_task_p2p_ext",
+                 "photons_create", "photons_force_device", "photons_p2m_batch"):
+        assert must in names
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_reference_globals_are_exported(lib):
+    for name in ("NPART", "MAXLEAF", "first_leaf", "first_node", "last_leaf", "last_node"):
+        C.c_int.in_dll(lib, name)
+    for name in ("MASSPART", "SoftenScale", "splitRadius", "cutoffRadius", "open_angle", "BOXSIZE"):
+        C.c_double.in_dll(lib, name)
+    for name in ("part", "leaf", "btree"):
+        C.c_void_p.in_dll(lib, name)
+
+
+def test_parameters_are_validated(lib):
+    import ic
+    _, prm = ic.workload_lattice32()
+    bad = dict(prm.as_dict())
+    bad["order"] = 7
+    h = C.c_void_p()
+    p = pg.make_params(bad)
+    assert lib.photons_create(C.byref(h), C.byref(p), 0, None) != 0
+    assert b"order" in lib.photons_last_error()
+
+
+def test_no_gpu_means_an_error_not_a_fallback(lib):
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    import ic
+    _, prm = ic.workload_lattice32()
+    with pytest.raises(pg.PhotonsError):
+        pg.Context(prm)
