@@ -30,8 +30,7 @@ __host__ __device__ inline int32_t pack_entry(int cell, int sid)
 }
 __host__ __device__ inline int entry_cell(int32_t e) { return (int)((uint32_t)e & (uint32_t)kCellMask); }
 __host__ __device__ inline int entry_shift(int32_t e) { return (int)((uint32_t)e >> kShiftBits); }
-constexpr int kWalkStack = 160;  // entries per thread; needs 2*depth + 2
-constexpr int kMaxDepth = (kWalkStack - 2) / 2;
+constexpr int kMaxDepth = 240;  // levels of the kd tree
 
 struct CellGeom {
     double c[3];
@@ -76,6 +75,8 @@ struct Ctx {
     DevBuf level_nodes;            // node cell ids by depth
     DevBuf M, L;                   // expansions
     DevBuf p2p_off, p2p_src, m2l_off, m2l_src, cnt_tmp;
+    DevBuf bfs_a, bfs_b, p2p_keys, m2l_keys;  // walk frontier and task keys
+    DevBuf m2l_item_off, m2l_item_tgt, m2l_part;  // balanced M2L work items
     DevBuf scan_tmp;
     // build temporaries (level order)
     int tcap = 0, lcap = 0;        // capacities of the level-order temporaries

@@ -19,6 +19,9 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+#include <utility>
+
 namespace photons {
 
 __host__ __device__ constexpr int nterms(int p) { return (p + 1) * (p + 2) * (p + 3) / 6; }
@@ -69,21 +72,30 @@ __host__ __device__ constexpr double hcoef(int a, int k)
     return num / den;
 }
 
+// Compile-time loop: f(std::integral_constant<int, I>) for I = B .. E-1, so that every
+// multi-index below is a constant expression and the term arrays stay in registers.
+template <int B, int E, typename F>
+__host__ __device__ __forceinline__ void static_for(F &&f)
+{
+    if constexpr (B < E) {
+        f(std::integral_constant<int, B>{});
+        static_for<B + 1, E>(f);
+    }
+}
+
 // Powers x^e / e! for e = 0..P, i.e. the monomial weights of a Taylor shift.
 template <int P, typename T>
 __device__ __forceinline__ void scaled_powers(T x, T *px)
 {
     px[0] = T(1);
-#pragma unroll
-    for (int e = 1; e <= P; e++) px[e] = px[e - 1] * x * T(1.0 / e);
+    static_for<1, P + 1>([&](auto e) { px[e.value] = px[e.value - 1] * x * T(1.0 / e.value); });
 }
 
 template <int P, typename T>
 __device__ __forceinline__ void plain_powers(T x, T *px)
 {
     px[0] = T(1);
-#pragma unroll
-    for (int e = 1; e <= P; e++) px[e] = px[e - 1] * x;
+    static_for<1, P + 1>([&](auto e) { px[e.value] = px[e.value - 1] * x; });
 }
 
 // ----------------------------------------------------------------------------- P2M / M2M
@@ -96,8 +108,10 @@ __device__ __forceinline__ void p2m_add(T m, T dx, T dy, T dz, T *M)
     scaled_powers<P>(-dx, px);
     scaled_powers<P>(-dy, py);
     scaled_powers<P>(-dz, pz);
-#pragma unroll
-    for (int k = 0; k < nterms(P); k++) M[k] += m * px[ma(k)] * py[mb(k)] * pz[mc(k)];
+    static_for<0, nterms(P)>([&](auto k) {
+        constexpr int K = k.value;
+        M[K] += m * px[ma(K)] * py[mb(K)] * pz[mc(K)];
+    });
 }
 
 // tM += shift(M), d = c_parent - c_child
@@ -108,18 +122,16 @@ __device__ __forceinline__ void m2m_add(T dx, T dy, T dz, const T *M, T *tM)
     scaled_powers<P>(dx, px);
     scaled_powers<P>(dy, py);
     scaled_powers<P>(dz, pz);
-#pragma unroll
-    for (int k = 0; k < nterms(P); k++) {
-        T s = T(0);
-#pragma unroll
-        for (int j = 0; j <= k; j++) {
-            constexpr int dummy = 0;
-            (void)dummy;
-            if (ma(j) <= ma(k) && mb(j) <= mb(k) && mc(j) <= mc(k))
-                s += M[j] * px[ma(k) - ma(j)] * py[mb(k) - mb(j)] * pz[mc(k) - mc(j)];
-        }
-        tM[k] += s;
-    }
+    static_for<0, nterms(P)>([&](auto k) {
+        constexpr int K = k.value;
+        T s = M[K];
+        static_for<0, K>([&](auto j) {
+            constexpr int J = j.value;
+            if constexpr (ma(J) <= ma(K) && mb(J) <= mb(K) && mc(J) <= mc(K))
+                s += M[J] * px[ma(K) - ma(J)] * py[mb(K) - mb(J)] * pz[mc(K) - mc(J)];
+        });
+        tM[K] += s;
+    });
 }
 
 // ----------------------------------------------------------------------------- L2L / L2P
@@ -132,16 +144,15 @@ __device__ __forceinline__ void l2l_add(T dx, T dy, T dz, const T *L, T *tL)
     scaled_powers<P>(dx, px);
     scaled_powers<P>(dy, py);
     scaled_powers<P>(dz, pz);
-#pragma unroll
-    for (int k = 0; k < nterms(P); k++) {
-        T s = T(0);
-#pragma unroll
-        for (int g = 0; g < nterms(P); g++) {
-            if (mdeg(k) + mdeg(g) <= P)
-                s += L[mindex(ma(k) + ma(g), mb(k) + mb(g), mc(k) + mc(g))] * px[ma(g)] * py[mb(g)] * pz[mc(g)];
-        }
-        tL[k] += s;
-    }
+    static_for<0, nterms(P)>([&](auto k) {
+        constexpr int K = k.value;
+        T s = L[K];
+        static_for<1, nterms(P - mdeg(K))>([&](auto g) {
+            constexpr int G = g.value;
+            s += L[mindex(ma(K) + ma(G), mb(K) + mb(G), mc(K) + mc(G))] * px[ma(G)] * py[mb(G)] * pz[mc(G)];
+        });
+        tL[K] += s;
+    });
 }
 
 // gradient of the local expansion at offset d from its centre
@@ -152,14 +163,14 @@ __device__ __forceinline__ void l2p_eval(const T *L, T dx, T dy, T dz, T &ax, T 
     scaled_powers<P>(dx, px);
     scaled_powers<P>(dy, py);
     scaled_powers<P>(dz, pz);
-    T fx = T(0), fy = T(0), fz = T(0);
-#pragma unroll
-    for (int k = 0; k < nterms(P - 1); k++) {
-        T w = px[ma(k)] * py[mb(k)] * pz[mc(k)];
-        fx += L[mindex(ma(k) + 1, mb(k), mc(k))] * w;
-        fy += L[mindex(ma(k), mb(k) + 1, mc(k))] * w;
-        fz += L[mindex(ma(k), mb(k), mc(k) + 1)] * w;
-    }
+    T fx = L[1], fy = L[2], fz = L[3];
+    static_for<1, nterms(P - 1)>([&](auto k) {
+        constexpr int K = k.value;
+        T w = px[ma(K)] * py[mb(K)] * pz[mc(K)];
+        fx += L[mindex(ma(K) + 1, mb(K), mc(K))] * w;
+        fy += L[mindex(ma(K), mb(K) + 1, mc(K))] * w;
+        fz += L[mindex(ma(K), mb(K), mc(K) + 1)] * w;
+    });
     ax = fx;
     ay = fy;
     az = fz;
@@ -220,12 +231,12 @@ __device__ __forceinline__ void radial_f64(double r, double inv2rs, double *f)
     double t = e / inv2rs;
     double a2 = 2.0 * inv2rs * inv2rs;
     f[0] = b;
-#pragma unroll
-    for (int n = 1; n <= P; n++) {
+    static_for<1, P + 1>([&](auto nn) {
+        constexpr int n = nn.value;
         t *= a2;
         b = ((2 * n - 1) * b + t) * ir2;
         f[n] = (n & 1) ? -b : b;
-    }
+    });
 }
 
 // same in float, in units where rs = 1 (so a = 1/2)
@@ -240,12 +251,12 @@ __device__ __forceinline__ void radial_f32_unit(float r, float *f)
     float b = e * erfcx_tq(tt) * ir;
     float t = e * (2.0f * 0.56418958354775628695f);
     f[0] = b;
-#pragma unroll
-    for (int n = 1; n <= P; n++) {
+    static_for<1, P + 1>([&](auto nn) {
+        constexpr int n = nn.value;
         t *= 0.5f;
         b = ((2 * n - 1) * b + t) * ir2;
         f[n] = (n & 1) ? -b : b;
-    }
+    });
 }
 
 // D_a(R) for |a| <= P from the radial factors
@@ -256,39 +267,36 @@ __device__ __forceinline__ void dtensor(T x, T y, T z, const T *f, T *D)
     plain_powers<P>(x, px);
     plain_powers<P>(y, py);
     plain_powers<P>(z, pz);
-#pragma unroll
-    for (int k = 0; k < nterms(P); k++) {
+    static_for<0, nterms(P)>([&](auto k) {
+        constexpr int K = k.value;
+        constexpr int A = ma(K), B = mb(K), C = mc(K);
         T s = T(0);
-#pragma unroll
-        for (int kx = 0; kx <= P / 2; kx++)
-#pragma unroll
-            for (int ky = 0; ky <= P / 2; ky++)
-#pragma unroll
-                for (int kz = 0; kz <= P / 2; kz++) {
-                    if (2 * kx <= ma(k) && 2 * ky <= mb(k) && 2 * kz <= mc(k)) {
-                        constexpr double one = 1.0;
-                        T coef = T(one * hcoef(ma(k), kx) * hcoef(mb(k), ky) * hcoef(mc(k), kz));
-                        s += f[mdeg(k) - kx - ky - kz] * coef * px[ma(k) - 2 * kx] * py[mb(k) - 2 * ky] *
-                             pz[mc(k) - 2 * kz];
-                    }
-                }
-        D[k] = s;
-    }
+        static_for<0, A / 2 + 1>([&](auto kx) {
+            static_for<0, B / 2 + 1>([&](auto ky) {
+                static_for<0, C / 2 + 1>([&](auto kz) {
+                    constexpr int X = kx.value, Y = ky.value, Z = kz.value;
+                    constexpr double coef = hcoef(A, X) * hcoef(B, Y) * hcoef(C, Z);
+                    s += f[A + B + C - X - Y - Z] * T(coef) * px[A - 2 * X] * py[B - 2 * Y] * pz[C - 2 * Z];
+                });
+            });
+        });
+        D[K] = s;
+    });
 }
 
 // L += sum_{|k|+|j|<=P} M_j D_{k+j}
 template <int P, typename T>
 __device__ __forceinline__ void m2l_contract(const T *D, const T *M, T *L)
 {
-#pragma unroll
-    for (int k = 0; k < nterms(P); k++) {
-        T s = T(0);
-#pragma unroll
-        for (int j = 0; j < nterms(P); j++) {
-            if (mdeg(k) + mdeg(j) <= P) s += M[j] * D[mindex(ma(k) + ma(j), mb(k) + mb(j), mc(k) + mc(j))];
-        }
-        L[k] += s;
-    }
+    static_for<0, nterms(P)>([&](auto k) {
+        constexpr int K = k.value;
+        T s = L[K];
+        static_for<0, nterms(P - mdeg(K))>([&](auto j) {
+            constexpr int J = j.value;
+            s += M[J] * D[mindex(ma(K) + ma(J), mb(K) + mb(J), mc(K) + mc(J))];
+        });
+        L[K] = s;
+    });
 }
 
 // ----------------------------------------------------------------------------- acceptance

@@ -11,6 +11,9 @@
 // nsub-th source particle (nsub = 64 / nT), so a 22-particle leaf keeps 44 lanes
 // busy; each source leaf is staged in LDS once for the whole wavefront, and the
 // per-lane partial sums are combined in a fixed order (deterministic results).
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
 #include <stdexcept>
 
 #include "engine.h"
@@ -23,7 +26,7 @@ constexpr int kP2PMaxLeaf = 64;
 // ---------------------------------------------------------------------------- upward
 
 template <int P>
-__global__ void k_p2m(int nleaf, const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
+__global__ void __launch_bounds__(128) k_p2m(int nleaf, const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
                       const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
                       double mass, double *__restrict__ M)
 {
@@ -42,7 +45,7 @@ __global__ void k_p2m(int nleaf, const CellGeom *__restrict__ geom, const CellIn
 }
 
 template <int P>
-__global__ void k_m2m_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
+__global__ void __launch_bounds__(128) k_m2m_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
                             const CellInfo *__restrict__ info, double *__restrict__ M)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -68,7 +71,7 @@ __global__ void k_m2m_level(const int *__restrict__ nodes, int count, const Cell
 }
 
 template <int P>
-__global__ void k_scale_down(int ncell, const double *__restrict__ M, double inv_rs, float *__restrict__ Ms)
+__global__ void __launch_bounds__(128) k_scale_down(int ncell, const double *__restrict__ M, double inv_rs, float *__restrict__ Ms)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= ncell) return;
@@ -83,89 +86,94 @@ __global__ void k_scale_down(int ncell, const double *__restrict__ M, double inv
 
 // ---------------------------------------------------------------------------- M2L
 
-template <int P>
-__global__ void __launch_bounds__(128) k_m2l_f64(int ncell, const CellGeom *__restrict__ geom,
-                                                 const int64_t *__restrict__ off, const int32_t *__restrict__ src,
-                                                 const double *__restrict__ M, double inv2rs, double boxsize,
-                                                 double *__restrict__ Lout, int accumulate)
+// M2L work is cut into items of at most kM2LChunk list entries so that targets with
+// thousands of sources (cells inside dense clusters) do not serialise one thread;
+// each item writes a partial expansion and a second pass adds the partials of a
+// target in item order (deterministic).
+constexpr int kM2LChunk = 8;
+
+__device__ __forceinline__ void shift_xyz(int sid, double L, double &sx, double &sy, double &sz)
 {
-    int X = blockIdx.x * blockDim.x + threadIdx.x;
-    if (X >= ncell) return;
-    constexpr int NT = nterms(P);
-    double L[NT];
-#pragma unroll
-    for (int k = 0; k < NT; k++) L[k] = accumulate ? Lout[(size_t)X * NT + k] : 0.0;
-    const CellGeom gX = geom[X];
-    for (int64_t e = off[X]; e < off[X + 1]; e++) {
-        const int32_t s = src[e];
-        const int B = entry_cell(s), sid = entry_shift(s);
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        if (sid) {
-            int k = sid <= 13 ? sid - 1 : sid;
-            sx = (double)(k / 9 - 1) * boxsize;
-            sy = (double)((k / 3) % 3 - 1) * boxsize;
-            sz = (double)(k % 3 - 1) * boxsize;
-        }
-        const CellGeom gB = geom[B];
-        const double rx = gX.c[0] - (gB.c[0] + sx);
-        const double ry = gX.c[1] - (gB.c[1] + sy);
-        const double rz = gX.c[2] - (gB.c[2] + sz);
-        const double r = sqrt(rx * rx + ry * ry + rz * rz);
-        double f[P + 1], D[NT], m[NT];
-        radial_f64<P>(r, inv2rs, f);
-        dtensor<P, double>(rx, ry, rz, f, D);
-#pragma unroll
-        for (int k = 0; k < NT; k++) m[k] = M[(size_t)B * NT + k];
-        m2l_contract<P, double>(D, m, L);
+    sx = sy = sz = 0.0;
+    if (sid) {
+        int k = sid <= 13 ? sid - 1 : sid;
+        sx = (double)(k / 9 - 1) * L;
+        sy = (double)((k / 3) % 3 - 1) * L;
+        sz = (double)(k % 3 - 1) * L;
     }
-#pragma unroll
-    for (int k = 0; k < NT; k++) Lout[(size_t)X * NT + k] = L[k];
 }
 
-template <int P>
-__global__ void __launch_bounds__(128) k_m2l_f32(int ncell, const CellGeom *__restrict__ geom,
-                                                 const int64_t *__restrict__ off, const int32_t *__restrict__ src,
-                                                 const float *__restrict__ Ms, double inv_rs, double boxsize,
-                                                 float *__restrict__ Lout, int accumulate)
+__global__ void k_m2l_nitems(const int64_t *__restrict__ off, int ncell, int *__restrict__ nitems)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    if (X > ncell) return;
+    nitems[X] = X == ncell ? 0 : (int)((off[X + 1] - off[X] + kM2LChunk - 1) / kM2LChunk);
+}
+
+__global__ void k_m2l_item_targets(const int *__restrict__ item_off, int ncell, int *__restrict__ item_tgt)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    if (X >= ncell) return;
+    for (int j = item_off[X]; j < item_off[X + 1]; j++) item_tgt[j] = X;
+}
+
+template <int P, typename T>
+__global__ void __launch_bounds__(128) k_m2l_items(int nitems, const int *__restrict__ item_tgt,
+                                                   const int *__restrict__ item_off, const CellGeom *__restrict__ geom,
+                                                   const int64_t *__restrict__ off, const int32_t *__restrict__ src,
+                                                   const T *__restrict__ M, double scale, double inv2rs,
+                                                   double boxsize, T *__restrict__ part)
+{
+    int it = blockIdx.x * blockDim.x + threadIdx.x;
+    if (it >= nitems) return;
+    constexpr int NT = nterms(P);
+    const int X = item_tgt[it];
+    const int64_t e0 = off[X] + (int64_t)(it - item_off[X]) * kM2LChunk;
+    const int64_t e1 = min(off[X + 1], e0 + kM2LChunk);
+    T L[NT];
+    static_for<0, NT>([&](auto k) { L[k.value] = T(0); });
+    const CellGeom gX = geom[X];
+    for (int64_t e = e0; e < e1; e++) {
+        const int32_t s = src[e];
+        const int B = entry_cell(s), sid = entry_shift(s);
+        double sx, sy, sz;
+        shift_xyz(sid, boxsize, sx, sy, sz);
+        const CellGeom gB = geom[B];
+        // relative coordinates formed in double (then scaled to units of rs for float)
+        const T rx = (T)((gX.c[0] - (gB.c[0] + sx)) * scale);
+        const T ry = (T)((gX.c[1] - (gB.c[1] + sy)) * scale);
+        const T rz = (T)((gX.c[2] - (gB.c[2] + sz)) * scale);
+        T f[P + 1], D[NT], m[NT];
+        if constexpr (sizeof(T) == 8) {
+            radial_f64<P>(sqrt(rx * rx + ry * ry + rz * rz), inv2rs, f);
+        } else {
+            radial_f32_unit<P>(__builtin_sqrtf(rx * rx + ry * ry + rz * rz), f);
+        }
+        dtensor<P, T>(rx, ry, rz, f, D);
+        static_for<0, NT>([&](auto k) { m[k.value] = M[(size_t)B * NT + k.value]; });
+        m2l_contract<P, T>(D, m, L);
+    }
+    static_for<0, NT>([&](auto k) { part[(size_t)it * NT + k.value] = L[k.value]; });
+}
+
+template <int P, typename T>
+__global__ void __launch_bounds__(128) k_m2l_reduce(int ncell, const int *__restrict__ item_off,
+                                                    const T *__restrict__ part, T *__restrict__ Lout, int accumulate)
 {
     int X = blockIdx.x * blockDim.x + threadIdx.x;
     if (X >= ncell) return;
     constexpr int NT = nterms(P);
-    float L[NT];
-#pragma unroll
-    for (int k = 0; k < NT; k++) L[k] = accumulate ? Lout[(size_t)X * NT + k] : 0.0f;
-    const CellGeom gX = geom[X];
-    for (int64_t e = off[X]; e < off[X + 1]; e++) {
-        const int32_t s = src[e];
-        const int B = entry_cell(s), sid = entry_shift(s);
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        if (sid) {
-            int k = sid <= 13 ? sid - 1 : sid;
-            sx = (double)(k / 9 - 1) * boxsize;
-            sy = (double)((k / 3) % 3 - 1) * boxsize;
-            sz = (double)(k % 3 - 1) * boxsize;
-        }
-        const CellGeom gB = geom[B];
-        // relative coordinates formed in double, then to float in units of rs
-        const float rx = (float)((gX.c[0] - (gB.c[0] + sx)) * inv_rs);
-        const float ry = (float)((gX.c[1] - (gB.c[1] + sy)) * inv_rs);
-        const float rz = (float)((gX.c[2] - (gB.c[2] + sz)) * inv_rs);
-        const float r = __builtin_sqrtf(rx * rx + ry * ry + rz * rz);
-        float f[P + 1], D[NT], m[NT];
-        radial_f32_unit<P>(r, f);
-        dtensor<P, float>(rx, ry, rz, f, D);
-#pragma unroll
-        for (int k = 0; k < NT; k++) m[k] = Ms[(size_t)B * NT + k];
-        m2l_contract<P, float>(D, m, L);
-    }
-#pragma unroll
-    for (int k = 0; k < NT; k++) Lout[(size_t)X * NT + k] = L[k];
+    T L[NT];
+    static_for<0, NT>([&](auto k) { L[k.value] = accumulate ? Lout[(size_t)X * NT + k.value] : T(0); });
+    for (int it = item_off[X]; it < item_off[X + 1]; it++)
+        static_for<0, NT>([&](auto k) { L[k.value] += part[(size_t)it * NT + k.value]; });
+    static_for<0, NT>([&](auto k) { Lout[(size_t)X * NT + k.value] = L[k.value]; });
 }
 
 // ---------------------------------------------------------------------------- L2L
 
 template <int P, typename T>
-__global__ void k_l2l_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
+__global__ void __launch_bounds__(128) k_l2l_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
                             const CellInfo *__restrict__ info, double scale, T *__restrict__ L)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -458,25 +466,55 @@ void upward_pass(Ctx &c)
     PH_CHECK(hipGetLastError());
 }
 
+static void exclusive_scan_int(Ctx &c, const int *in, int *out, int n)
+{
+    size_t bytes = 0;
+    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, c.stream));
+    void *tmp = grow(c.scan_tmp, bytes);
+    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, c.stream));
+}
+
 void m2l_pass(Ctx &c, bool accumulate)
 {
     const int NT = nterms(c.prm.order);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
-    size_t lbytes = (f32 ? sizeof(float) : sizeof(double)) * (size_t)c.ncell * NT;
-    void *L = grow(c.L, lbytes);
+    const size_t tsize = f32 ? sizeof(float) : sizeof(double);
+    void *L = grow(c.L, tsize * (size_t)c.ncell * NT);
     hipStream_t st = c.stream;
     const CellGeom *geom = dptr<CellGeom>(c.geom);
     const int64_t *off = dptr<int64_t>(c.m2l_off);
     const int32_t *src = dptr<int32_t>(c.m2l_src);
     double *M = dptr<double>(c.M);
-#define M2L_L(PP)                                                                                                 \
-    if (f32)                                                                                                      \
-        k_m2l_f32<PP><<<nblk(c.ncell, 128), 128, 0, st>>>(                                                        \
-            c.ncell, geom, off, src, reinterpret_cast<const float *>(M + (size_t)c.ncell * NT), 1.0 / c.prm.rs,    \
-            c.prm.boxsize, reinterpret_cast<float *>(L), accumulate);                                             \
-    else                                                                                                          \
-        k_m2l_f64<PP><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, geom, off, src, M, 0.5 / c.prm.rs,            \
-                                                          c.prm.boxsize, reinterpret_cast<double *>(L), accumulate);
+    // work items
+    int *item_off = reinterpret_cast<int *>(grow(c.m2l_item_off, sizeof(int) * 2 * (size_t)(c.ncell + 1)));
+    int *nitems = item_off + (c.ncell + 1);
+    k_m2l_nitems<<<nblk(c.ncell + 1, 256), 256, 0, st>>>(off, c.ncell, nitems);
+    exclusive_scan_int(c, nitems, item_off, c.ncell + 1);
+    int *pin = reinterpret_cast<int *>(c.pinned.p) + 64;
+    PH_CHECK(hipMemcpyAsync(pin, item_off + c.ncell, sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipStreamSynchronize(st));
+    const int n_items = *pin;
+    int *item_tgt = reinterpret_cast<int *>(grow(c.m2l_item_tgt, sizeof(int) * (size_t)std::max(1, n_items)));
+    void *part = grow(c.m2l_part, tsize * (size_t)std::max(1, n_items) * NT);
+    k_m2l_item_targets<<<nblk(c.ncell, 256), 256, 0, st>>>(item_off, c.ncell, item_tgt);
+#define M2L_L(PP)                                                                                                  \
+    if (f32) {                                                                                                     \
+        if (n_items)                                                                                               \
+            k_m2l_items<PP, float><<<nblk(n_items, 128), 128, 0, st>>>(                                            \
+                n_items, item_tgt, item_off, geom, off, src, reinterpret_cast<const float *>(M + (size_t)c.ncell * NT), \
+                1.0 / c.prm.rs, 0.5, c.prm.boxsize, reinterpret_cast<float *>(part));                              \
+        k_m2l_reduce<PP, float><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, item_off,                           \
+                                                                    reinterpret_cast<const float *>(part),         \
+                                                                    reinterpret_cast<float *>(L), accumulate);     \
+    } else {                                                                                                       \
+        if (n_items)                                                                                               \
+            k_m2l_items<PP, double><<<nblk(n_items, 128), 128, 0, st>>>(n_items, item_tgt, item_off, geom, off, src, \
+                                                                        M, 1.0, 0.5 / c.prm.rs, c.prm.boxsize,     \
+                                                                        reinterpret_cast<double *>(part));         \
+        k_m2l_reduce<PP, double><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, item_off,                          \
+                                                                     reinterpret_cast<const double *>(part),       \
+                                                                     reinterpret_cast<double *>(L), accumulate);   \
+    }
     PH_DISPATCH(c.prm.order, M2L_L)
 #undef M2L_L
     PH_CHECK(hipGetLastError());
@@ -543,7 +581,7 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
 // ---------------------------------------------------------------------------- batched operators
 
 template <int P>
-__global__ void k_op_p2m(int count, const double *pos, const int32_t *first, const int32_t *npart,
+__global__ void __launch_bounds__(128) k_op_p2m(int count, const double *pos, const int32_t *first, const int32_t *npart,
                          const double *center, double mass, double *M)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -560,7 +598,7 @@ __global__ void k_op_p2m(int count, const double *pos, const int32_t *first, con
 }
 
 template <int P, int KIND>  // 0 m2m, 1 l2l
-__global__ void k_op_shift(int count, const double *d, const double *X, double *Y)
+__global__ void __launch_bounds__(128) k_op_shift(int count, const double *d, const double *X, double *Y)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
@@ -578,7 +616,7 @@ __global__ void k_op_shift(int count, const double *d, const double *X, double *
 }
 
 template <int P>
-__global__ void k_op_m2l(int count, double inv2rs, const double *R, const double *M, double *L)
+__global__ void __launch_bounds__(128) k_op_m2l(int count, double inv2rs, const double *R, const double *M, double *L)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
@@ -598,7 +636,7 @@ __global__ void k_op_m2l(int count, double inv2rs, const double *R, const double
 }
 
 template <int P>
-__global__ void k_op_l2p(int n, const double *pos, const int32_t *owner, const double *center, const double *L,
+__global__ void __launch_bounds__(128) k_op_l2p(int n, const double *pos, const int32_t *owner, const double *center, const double *L,
                          double *acc)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

@@ -531,7 +531,7 @@ void build_tree(Ctx &c)
         nleaf += new_leaves;
         lvl.push_back(ntmp);
         depth++;
-        if (depth > kMaxDepth) throw std::runtime_error("tree deeper than the walk stack supports");
+        if (depth > kMaxDepth) throw std::runtime_error("tree deeper than the supported number of levels");
     }
     const int nlevels = depth;  // levels holding nodes: 0 .. depth-1
 

@@ -4,24 +4,24 @@
 // and emits P2P and M2L tasks (src/fmm.c:406-712 for the home box,
 // src/remotes.c:213-552 against each of the 26 shifted, pruned copies of the tree
 // built by prepare_sendtree2, src/remotes.c:60-169).  Which pairs it emits depends
-// only on the rules applied to each pair, so the same lists come out of any
-// traversal that applies them.
+// only on the rule applied to each pair, so any traversal that applies the same
+// rules emits the same set.
 //
-// Here one thread owns one target cell X and replays the walk restricted to the
-// pairs whose target is an ancestor-or-self of X: opening a target keeps only the
-// child on the way to X, and a pair is emitted only when its target is X.  Each
-// thread therefore writes its own list, in the reference's emission order for
-// that target, with no atomics and no sort.  A count pass sizes the lists, an
-// exclusive scan places them, a fill pass writes them.
+// MI355X formulation: a breadth-first walk over the whole pair space.  Each level
+// is one launch with one thread per pair: the thread applies the rule and appends
+// its children (1, 2 or 4 pairs) or its emitted task, with one atomic per
+// wavefront per output.  No stack, no per-thread divergence beyond one rule, full
+// occupancy.  The emitted tasks are then sorted by (target, shift, source rank in
+// depth-first order): for one target the reference's depth-first walk emits its
+// sources in exactly that order (a source subtree is finished before its sibling's
+// starts), so the lists come out in the reference's order and are deterministic.
 //
 // Image walks: a shifted node keeps no children when its box is rc or more from
-// the domain box or when 0.95*theta*distance exceeds its largest width; that rule
-// is evaluated on the fly.  An image whose shifted domain box is rc or more from
-// the box of X's parent cannot reach X and is skipped (every pair reaching X
-// passes through a pair (parent, source) that opened the target, which needs a
-// gap below rc).
+// the domain box or when 0.95*theta*distance exceeds its largest width
+// (src/remotes.c:145-158); the rule is evaluated on the fly, no copy is built.
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <stdexcept>
 
 #include "engine.h"
@@ -29,17 +29,28 @@
 
 namespace photons {
 
-struct WalkArgs {
+// pair: A in bits 0-26, B in bits 27-53, shift id in bits 54-58
+__device__ __forceinline__ uint64_t pair_of(uint32_t A, uint32_t B, uint32_t sid)
+{
+    return (uint64_t)A | ((uint64_t)B << 27) | ((uint64_t)sid << 54);
+}
+
+// task key: target in bits 37-63, shift id in bits 32-36, source rank in bits 0-31
+__device__ __forceinline__ uint64_t task_key(uint32_t target, uint32_t sid, uint32_t rank)
+{
+    return ((uint64_t)target << 37) | ((uint64_t)sid << 32) | (uint64_t)rank;
+}
+
+struct BfsArgs {
     const CellGeom *geom;
     const CellInfo *info;
-    int nleaf, ncell;
+    int nleaf;
     double theta, theta2, rc, rc2, boxsize;
-    int shift_lo, shift_hi;
-    int start_target, start_source;
-    int64_t *p2p_cnt, *m2l_cnt;          // pass 0
-    const int64_t *p2p_off, *m2l_off;    // pass 1
-    int32_t *p2p_src, *m2l_src;          // pass 1
-    int *err;
+    CellGeom root;  // domain box
+    const uint64_t *in;
+    int64_t n_in;
+    uint64_t *out, *p2p, *m2l;
+    int64_t p2p_base, m2l_base;  // tasks emitted by earlier levels
 };
 
 __host__ __device__ inline void shift_of(int sid, double L, double &sx, double &sy, double &sz)
@@ -50,10 +61,9 @@ __host__ __device__ inline void shift_of(int sid, double L, double &sx, double &
         return;
     }
     int k = sid <= 13 ? sid - 1 : sid;  // skip the centre (index 13 of the 27)
-    int mi = k / 9 - 1, mj = (k / 3) % 3 - 1, mk = k % 3 - 1;
-    sx = (double)mi * L;
-    sy = (double)mj * L;
-    sz = (double)mk * L;
+    sx = (double)(k / 9 - 1) * L;
+    sy = (double)((k / 3) % 3 - 1) * L;
+    sz = (double)(k % 3 - 1) * L;
 }
 
 __device__ __forceinline__ bool let_pruned(const CellGeom &root, const CellGeom &g, double sx, double sy, double sz,
@@ -76,134 +86,139 @@ __device__ __forceinline__ bool let_pruned(const CellGeom &root, const CellGeom 
     return wmax < 0.95 * theta * dr;
 }
 
-__device__ __forceinline__ double box_gap(const CellGeom &a, const CellGeom &b, double sx, double sy, double sz)
+// The rule applied to one pair (src/fmm.c:406-712, src/remotes.c:213-552): its
+// children, or the P2P or M2L task it emits.
+__device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, uint64_t *kids, int &nkid, uint64_t &ptask,
+                                         int &np, uint64_t &mtask, int &nm)
 {
-    double s[3] = {sx, sy, sz}, g2 = 0.0;
-#pragma unroll
-    for (int i = 0; i < 3; i++) {
-        double d = a.c[i] - (b.c[i] + s[i]);
-        if (d < 0.0) d = -d;
-        d -= (a.w[i] + b.w[i]) * 0.5;
-        if (d > 0.0) g2 += d * d;
-    }
-    return sqrt(g2);
-}
-
-template <int PASS>
-__global__ void __launch_bounds__(128) k_walk(WalkArgs a)
-{
-    const int X = blockIdx.x * blockDim.x + threadIdx.x;
-    if (X >= a.ncell) return;
+    nkid = np = nm = 0;
+    const int A = (int)(pr & 0x7ffffff), B = (int)((pr >> 27) & 0x7ffffff), sid = (int)(pr >> 54);
     const int nleaf = a.nleaf;
-    const CellInfo IX = a.info[X];
-    const CellInfo IT = a.info[a.start_target];
-    const bool inside = IX.dfs >= IT.dfs && IX.dfs < IT.dfs + IT.dsize;
-    int64_t np = 0, nm = 0;
-    int64_t po = 0, mo = 0;
-    if (PASS == 1) {
-        if (X < nleaf) po = a.p2p_off[X];
-        mo = a.m2l_off[X];
+    const bool aleaf = A < nleaf, bleaf = B < nleaf;
+    if (sid == 0 && A == B) {  // self pair
+        if (aleaf) {
+            ptask = task_key(A, 0, B);
+            np = 1;
+        } else {
+            const CellInfo IA = a.info[A];
+            kids[0] = pair_of(IA.son0, IA.son0, 0);
+            kids[1] = pair_of(IA.son0, IA.son1, 0);
+            kids[2] = pair_of(IA.son1, IA.son0, 0);
+            kids[3] = pair_of(IA.son1, IA.son1, 0);
+            nkid = 4;
+        }
+        return;
     }
-    const int dX = IX.dfs;
-    int2 stack[kWalkStack];
-    const int root = a.start_target;
-    const CellGeom groot = a.geom[nleaf];  // domain box (tree root)
+    if (aleaf && bleaf) {  // leaf-leaf pairs are always P2P
+        ptask = task_key(A, sid, B);
+        np = 1;
+        return;
+    }
+    double sx, sy, sz;
+    shift_of(sid, a.boxsize, sx, sy, sz);
+    const CellGeom gA = a.geom[A], gB = a.geom[B];
+    const double d0 = gA.c[0] - (gB.c[0] + sx);
+    const double d1 = gA.c[1] - (gB.c[1] + sy);
+    const double d2 = gA.c[2] - (gB.c[2] + sz);
+    const int flag = mac(gA.w, gB.w, d0, d1, d2, a.theta2, a.rc2);
+    if (flag < 0) return;
+    const bool pruned = sid != 0 && !bleaf && let_pruned(a.root, gB, sx, sy, sz, a.rc, a.theta);
+    int open;  // 1 target, 2 source
+    if (flag == 1 || (aleaf && pruned)) {
+        // accepted; in the image walk a target leaf against a pruned node takes M2L
+        // whatever the acceptance says (src/remotes.c:442)
+        mtask = task_key(A, sid, a.info[B].dfs);
+        nm = 1;
+        return;
+    } else if (aleaf) {
+        open = 2;
+    } else if (bleaf) {
+        open = 1;
+    } else {
+        const bool bigger = (gA.w[0] + gA.w[1] + gA.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]);
+        open = (bigger || pruned) ? 1 : 2;  // ties open the source
+    }
+    if (open == 1) {
+        const CellInfo IA = a.info[A];
+        kids[0] = pair_of(IA.son0, B, sid);
+        kids[1] = pair_of(IA.son1, B, sid);
+    } else {
+        const CellInfo IB = a.info[B];
+        kids[0] = pair_of(A, IB.son0, sid);
+        kids[1] = pair_of(A, IB.son1, sid);
+    }
+    nkid = 2;
+}
 
-    for (int sid = inside ? a.shift_lo : a.shift_hi; sid < a.shift_hi; sid++) {
-        double sx, sy, sz;
-        shift_of(sid, a.boxsize, sx, sy, sz);
-        if (sid > 0 && X != root && IX.parent >= 0) {
-            const CellGeom gp = a.geom[IX.parent];
-            if (box_gap(gp, groot, sx, sy, sz) >= a.rc * (1.0 + 1e-9) + 1e-9 * a.boxsize) continue;
-        }
-        int sp = 0;
-        stack[sp++] = make_int2(root, a.start_source);
-        while (sp > 0) {
-            const int2 e = stack[--sp];
-            const int A = e.x, B = e.y;
-            const bool aleaf = A < nleaf, bleaf = B < nleaf;
-            if (sp + 2 > kWalkStack) {
-                atomicOr(a.err, 1);
-                break;
-            }
-            if (sid == 0 && A == B) {  // self pair
-                if (aleaf) {
-                    if (PASS == 1) a.p2p_src[po + np] = B;
-                    np++;
-                } else if (A != X) {
-                    const CellInfo IA = a.info[A];
-                    const CellInfo I0 = a.info[IA.son0];
-                    int toward = (dX >= I0.dfs && dX < I0.dfs + I0.dsize) ? IA.son0 : IA.son1;
-                    stack[sp++] = make_int2(toward, IA.son1);
-                    stack[sp++] = make_int2(toward, IA.son0);
-                }
-                continue;
-            }
-            if (aleaf && bleaf) {  // leaf-leaf pairs are always P2P
-                if (PASS == 1) a.p2p_src[po + np] = pack_entry(B, sid);
-                np++;
-                continue;
-            }
-            const CellGeom gA = a.geom[A], gB = a.geom[B];
-            const double d0 = gA.c[0] - (gB.c[0] + sx);
-            const double d1 = gA.c[1] - (gB.c[1] + sy);
-            const double d2 = gA.c[2] - (gB.c[2] + sz);
-            const int flag = mac(gA.w, gB.w, d0, d1, d2, a.theta2, a.rc2);
-            if (flag < 0) continue;
-            bool open_target;
-            if (sid == 0) {
-                if (flag == 1) {
-                    if (A == X) {
-                        if (PASS == 1) a.m2l_src[mo + nm] = B;
-                        nm++;
-                    }
-                    continue;
-                }
-                open_target = aleaf ? false : (bleaf ? true : (gA.w[0] + gA.w[1] + gA.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]));
-            } else {
-                const bool pruned = !bleaf && let_pruned(groot, gB, sx, sy, sz, a.rc, a.theta);
-                if (aleaf) {  // target leaf against a shifted node
-                    if (flag == 1 || pruned) {
-                        if (PASS == 1) a.m2l_src[mo + nm] = pack_entry(B, sid);
-                        nm++;
-                    } else {
-                        const CellInfo IB = a.info[B];
-                        stack[sp++] = make_int2(A, IB.son1);
-                        stack[sp++] = make_int2(A, IB.son0);
-                    }
-                    continue;
-                }
-                if (flag == 1) {
-                    if (A == X) {
-                        if (PASS == 1) a.m2l_src[mo + nm] = pack_entry(B, sid);
-                        nm++;
-                    }
-                    continue;
-                }
-                open_target = bleaf ? true
-                                    : ((gA.w[0] + gA.w[1] + gA.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]) || pruned);
-            }
-            if (open_target) {
-                if (A != X) {
-                    const CellInfo IA = a.info[A];
-                    const CellInfo I0 = a.info[IA.son0];
-                    int toward = (dX >= I0.dfs && dX < I0.dfs + I0.dsize) ? IA.son0 : IA.son1;
-                    stack[sp++] = make_int2(toward, B);
-                }
-            } else {
-                const CellInfo IB = a.info[B];
-                stack[sp++] = make_int2(A, IB.son1);
-                stack[sp++] = make_int2(A, IB.son0);
-            }
+// Each level runs in two passes without atomics: k_bfs_count sums the outputs of
+// every tile of kBfsTile pairs, the host scans those sums (exact output sizes), and
+// k_bfs_write re-applies the rule and writes each output at its scanned position.
+constexpr int kBfsThreads = 256;
+constexpr int kBfsPerThread = 4;
+constexpr int kBfsTile = kBfsThreads * kBfsPerThread;
+
+__global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, int64_t *__restrict__ tile_cnt, int ntile)
+{
+    typedef hipcub::BlockReduce<int, kBfsThreads> Reduce;
+    __shared__ typename Reduce::TempStorage tmp;
+    int ck = 0, cp = 0, cm = 0;
+    for (int k = 0; k < kBfsPerThread; k++) {
+        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
+        if (i < a.n_in) {
+            uint64_t kids[4], pt, mt;
+            int nk, np, nm;
+            bfs_rule(a, a.in[i], kids, nk, pt, np, mt, nm);
+            ck += nk;
+            cp += np;
+            cm += nm;
         }
     }
-    if (PASS == 0) {
-        if (X < nleaf) a.p2p_cnt[X] = np;
-        a.m2l_cnt[X] = nm;
+    int tk = Reduce(tmp).Sum(ck);
+    __syncthreads();
+    int tp = Reduce(tmp).Sum(cp);
+    __syncthreads();
+    int tm = Reduce(tmp).Sum(cm);
+    if (threadIdx.x == 0) {
+        tile_cnt[blockIdx.x] = tk;
+        tile_cnt[(ntile + 1) + blockIdx.x] = tp;
+        tile_cnt[2 * (ntile + 1) + blockIdx.x] = tm;
     }
 }
 
-__global__ void k_zero_tail(int64_t *p, int idx) { p[idx] = 0; }
+__global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const int64_t *__restrict__ tile_off, int ntile)
+{
+    typedef hipcub::BlockScan<int, kBfsThreads> Scan;
+    __shared__ typename Scan::TempStorage tmp;
+    uint64_t kids[kBfsPerThread][4], pt[kBfsPerThread], mt[kBfsPerThread];
+    int nk[kBfsPerThread], np[kBfsPerThread], nm[kBfsPerThread];
+    int ck = 0, cp = 0, cm = 0;
+#pragma unroll
+    for (int k = 0; k < kBfsPerThread; k++) {
+        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
+        nk[k] = np[k] = nm[k] = 0;
+        if (i < a.n_in) bfs_rule(a, a.in[i], kids[k], nk[k], pt[k], np[k], mt[k], nm[k]);
+        ck += nk[k];
+        cp += np[k];
+        cm += nm[k];
+    }
+    int ok, op, om;
+    Scan(tmp).ExclusiveSum(ck, ok);
+    __syncthreads();
+    Scan(tmp).ExclusiveSum(cp, op);
+    __syncthreads();
+    Scan(tmp).ExclusiveSum(cm, om);
+    int64_t bk = tile_off[blockIdx.x] + ok;
+    int64_t bp = a.p2p_base + tile_off[(ntile + 1) + blockIdx.x] + op;
+    int64_t bm = a.m2l_base + tile_off[2 * (ntile + 1) + blockIdx.x] + om;
+#pragma unroll
+    for (int k = 0; k < kBfsPerThread; k++) {
+        for (int j = 0; j < nk[k]; j++) a.out[bk++] = kids[k][j];
+        if (np[k]) a.p2p[bp++] = pt[k];
+        if (nm[k]) a.m2l[bm++] = mt[k];
+    }
+}
+
+__global__ void k_zero_i64(int64_t *p) { *p = 0; }
 
 static void scan64(Ctx &c, const int64_t *in, int64_t *out, int n)
 {
@@ -213,54 +228,161 @@ static void scan64(Ctx &c, const int64_t *in, int64_t *out, int n)
     PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, c.stream));
 }
 
+__global__ void k_init_frontier(uint64_t *f, int A, int B, int shift_lo, int shift_hi)
+{
+    int s = shift_lo + threadIdx.x;
+    if (s < shift_hi) f[threadIdx.x] = pair_of(A, B, s);
+}
+
+// CSR offsets of the sorted task keys: off[t] = first key with target >= t
+__global__ void k_csr_offsets(const uint64_t *__restrict__ keys, int64_t n, int ntarget, int64_t *__restrict__ off)
+{
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t > ntarget) return;
+    uint64_t want = (uint64_t)t << 37;
+    int64_t lo = 0, hi = n;
+    while (lo < hi) {
+        int64_t mid = (lo + hi) >> 1;
+        if (keys[mid] < want) lo = mid + 1;
+        else hi = mid;
+    }
+    off[t] = lo;
+}
+
+__global__ void k_unpack_tasks(const uint64_t *__restrict__ keys, int64_t n, const int *__restrict__ inv_dfs,
+                               int32_t *__restrict__ src)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint64_t k = keys[i];
+    int rank = (int)(k & 0xffffffffu), sid = (int)((k >> 32) & 31);
+    src[i] = pack_entry(inv_dfs ? inv_dfs[rank] : rank, sid);
+}
+
+__global__ void k_inv_dfs(const CellInfo *__restrict__ info, int ncell, int *__restrict__ inv)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < ncell) inv[info[c].dfs] = c;
+}
+
+// ---------------------------------------------------------------------------- host side
+
+static int bits_for(int64_t v)
+{
+    int b = 1;
+    while (b < 40 && ((int64_t)1 << b) <= v) b++;
+    return b;
+}
+
+// grow keeping the first `keep` elements
+static uint64_t *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
+{
+    if (need * sizeof(uint64_t) <= b.cap) return reinterpret_cast<uint64_t *>(b.p);
+    size_t want = need + need / 2;
+    uint64_t *nw = nullptr;
+    PH_CHECK(hipMalloc(&nw, want * sizeof(uint64_t)));
+    if (b.p && keep) PH_CHECK(hipMemcpyAsync(nw, b.p, keep * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+    PH_CHECK(hipStreamSynchronize(st));
+    if (b.p) PH_CHECK(hipFree(b.p));
+    b.p = nw;
+    b.cap = want * sizeof(uint64_t);
+    return nw;
+}
+
+static void sort_keys(Ctx &c, DevBuf &keys, DevBuf &alt, int64_t n, int end_bit)
+{
+    if (n <= 1) return;
+    uint64_t *k0 = reinterpret_cast<uint64_t *>(keys.p);
+    uint64_t *k1 = reinterpret_cast<uint64_t *>(grow(alt, sizeof(uint64_t) * n));
+    hipcub::DoubleBuffer<uint64_t> db(k0, k1);
+    size_t bytes = 0;
+    PH_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, db, (int)n, 0, end_bit, c.stream));
+    void *tmp = grow(c.scan_tmp, bytes);
+    PH_CHECK(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, db, (int)n, 0, end_bit, c.stream));
+    if (db.Current() != k0) std::swap(keys, alt);
+}
+
 void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int start_source)
 {
     hipStream_t st = c.stream;
     const int nleaf = c.nleaf, ncell = c.ncell;
-    if (2 * c.maxdepth + 2 > kWalkStack) throw std::runtime_error("tree too deep for the walk stack");
-    int64_t *cnt = reinterpret_cast<int64_t *>(grow(c.cnt_tmp, sizeof(int64_t) * (size_t)(nleaf + 1 + ncell + 1)));
-    int64_t *p2p_cnt = cnt, *m2l_cnt = cnt + nleaf + 1;
-    int64_t *p2p_off = reinterpret_cast<int64_t *>(grow(c.p2p_off, sizeof(int64_t) * (size_t)(nleaf + 1)));
-    int64_t *m2l_off = reinterpret_cast<int64_t *>(grow(c.m2l_off, sizeof(int64_t) * (size_t)(ncell + 1)));
-    int *err = reinterpret_cast<int *>(c.err.p);
-    PH_CHECK(hipMemsetAsync(err, 0, sizeof(int), st));
+    if (ncell >= (1 << 27)) throw std::runtime_error("more than 2^27 cells");
+    const int64_t n = std::max<int64_t>(c.n, 1);
 
-    WalkArgs a{};
+    BfsArgs a{};
     a.geom = dptr<CellGeom>(c.geom);
     a.info = dptr<CellInfo>(c.info);
     a.nleaf = nleaf;
-    a.ncell = ncell;
     a.theta = c.prm.theta;
     a.theta2 = c.prm.theta * c.prm.theta;
     a.rc = c.prm.rc;
     a.rc2 = c.prm.rc * c.prm.rc;
     a.boxsize = c.prm.boxsize;
-    a.shift_lo = shift_lo;
-    a.shift_hi = shift_hi;
-    a.start_target = start_target;
-    a.start_source = start_source;
-    a.p2p_cnt = p2p_cnt;
-    a.m2l_cnt = m2l_cnt;
-    a.err = err;
-    const int bs = 128, nb = (ncell + bs - 1) / bs;
-    k_walk<0><<<nb, bs, 0, st>>>(a);
-    k_zero_tail<<<1, 1, 0, st>>>(p2p_cnt, nleaf);
-    k_zero_tail<<<1, 1, 0, st>>>(m2l_cnt, ncell);
-    scan64(c, p2p_cnt, p2p_off, nleaf + 1);
-    scan64(c, m2l_cnt, m2l_off, ncell + 1);
-    int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 8;
-    PH_CHECK(hipMemcpyAsync(pin, p2p_off + nleaf, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    PH_CHECK(hipMemcpyAsync(pin + 1, m2l_off + ncell, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    PH_CHECK(hipMemcpyAsync(pin + 2, err, sizeof(int), hipMemcpyDeviceToHost, st));
+    PH_CHECK(hipMemcpyAsync(&a.root, a.geom + nleaf, sizeof(CellGeom), hipMemcpyDeviceToHost, st));
     PH_CHECK(hipStreamSynchronize(st));
-    if (*reinterpret_cast<int *>(pin + 2)) throw std::runtime_error("walk stack overflow");
-    c.n_p2p = pin[0];
-    c.n_m2l = pin[1];
-    a.p2p_off = p2p_off;
-    a.m2l_off = m2l_off;
-    a.p2p_src = reinterpret_cast<int32_t *>(grow(c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_p2p)));
-    a.m2l_src = reinterpret_cast<int32_t *>(grow(c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_m2l)));
-    k_walk<1><<<nb, bs, 0, st>>>(a);
+
+    // frontier ping-pong and task buffers, kept from the last call and grown exactly
+    grow_keep(c.bfs_a, 64, 0, st);
+    int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 16;
+
+    const int nshift = shift_hi - shift_lo;
+    k_init_frontier<<<1, 32, 0, st>>>(reinterpret_cast<uint64_t *>(c.bfs_a.p), start_target, start_source, shift_lo,
+                                      shift_hi);
+    int64_t n_in = nshift, n_p2p = 0, n_m2l = 0;
+    int levels = 0;
+    while (n_in > 0) {
+        const int ntile = (int)((n_in + kBfsTile - 1) / kBfsTile);
+        int64_t *tile_cnt = reinterpret_cast<int64_t *>(grow(c.cnt_tmp, sizeof(int64_t) * 6 * (size_t)(ntile + 1)));
+        int64_t *tile_off = tile_cnt + 3 * (ntile + 1);
+        a.in = reinterpret_cast<uint64_t *>(c.bfs_a.p);
+        a.n_in = n_in;
+        k_bfs_count<<<ntile, kBfsThreads, 0, st>>>(a, tile_cnt, ntile);
+        for (int q = 0; q < 3; q++) {
+            k_zero_i64<<<1, 1, 0, st>>>(tile_cnt + q * (ntile + 1) + ntile);
+            scan64(c, tile_cnt + q * (ntile + 1), tile_off + q * (ntile + 1), ntile + 1);
+        }
+        for (int q = 0; q < 3; q++)
+            PH_CHECK(hipMemcpyAsync(pin + q, tile_off + q * (ntile + 1) + ntile, sizeof(int64_t), hipMemcpyDeviceToHost,
+                                    st));
+        PH_CHECK(hipStreamSynchronize(st));
+        const int64_t n_out = pin[0], add_p2p = pin[1], add_m2l = pin[2];
+        grow_keep(c.bfs_b, std::max<int64_t>(n_out, 1), 0, st);
+        grow_keep(c.p2p_keys, std::max<int64_t>(n_p2p + add_p2p, 1), n_p2p, st);
+        grow_keep(c.m2l_keys, std::max<int64_t>(n_m2l + add_m2l, 1), n_m2l, st);
+        a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
+        a.p2p = reinterpret_cast<uint64_t *>(c.p2p_keys.p);
+        a.m2l = reinterpret_cast<uint64_t *>(c.m2l_keys.p);
+        a.p2p_base = n_p2p;
+        a.m2l_base = n_m2l;
+        k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off, ntile);
+        n_in = n_out;
+        n_p2p += add_p2p;
+        n_m2l += add_m2l;
+        std::swap(c.bfs_a, c.bfs_b);
+        if (++levels > 4 * kMaxDepth + 8) throw std::runtime_error("dual-tree walk does not terminate");
+    }
+    c.n_p2p = n_p2p;
+    c.n_m2l = n_m2l;
+
+    // sort into per-target lists in the reference's order, then CSR
+    sort_keys(c, c.p2p_keys, c.bfs_b, n_p2p, 37 + bits_for(nleaf));
+    sort_keys(c, c.m2l_keys, c.bfs_b, n_m2l, 37 + bits_for(ncell));
+    int64_t *p2p_off = reinterpret_cast<int64_t *>(grow(c.p2p_off, sizeof(int64_t) * (size_t)(nleaf + 1)));
+    int64_t *m2l_off = reinterpret_cast<int64_t *>(grow(c.m2l_off, sizeof(int64_t) * (size_t)(ncell + 1)));
+    int32_t *p2p_src = reinterpret_cast<int32_t *>(grow(c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, n_p2p)));
+    int32_t *m2l_src = reinterpret_cast<int32_t *>(grow(c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, n_m2l)));
+    int *inv = reinterpret_cast<int *>(grow(c.t_final, sizeof(int) * (size_t)ncell));
+    k_inv_dfs<<<(ncell + 255) / 256, 256, 0, st>>>(a.info, ncell, inv);
+    k_csr_offsets<<<(nleaf + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.p2p_keys.p), n_p2p, nleaf,
+                                                           p2p_off);
+    k_csr_offsets<<<(ncell + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.m2l_keys.p), n_m2l, ncell,
+                                                           m2l_off);
+    if (n_p2p)
+        k_unpack_tasks<<<(int)((n_p2p + 255) / 256), 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.p2p_keys.p), n_p2p,
+                                                                   nullptr, p2p_src);
+    if (n_m2l)
+        k_unpack_tasks<<<(int)((n_m2l + 255) / 256), 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.m2l_keys.p), n_m2l,
+                                                                   inv, m2l_src);
     PH_CHECK(hipGetLastError());
 }
 
