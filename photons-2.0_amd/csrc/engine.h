@@ -89,6 +89,7 @@ struct Ctx {
     DevBuf pinned;                 // small host-pinned staging
 
     int64_t n_p2p = 0, n_m2l = 0;  // list lengths
+    bool lists_valid = false;      // CSR lists built by a walk on the current tree
     int64_t n_p2p_local = 0, n_m2l_local = 0;
 
     Timing timing;
@@ -108,6 +109,8 @@ struct Ctx {
     } while (0)
 
 void *grow(DevBuf &b, size_t bytes);
+// throws unless b holds at least `bytes` (guards every launch that reads b)
+void need(const DevBuf &b, size_t bytes, const char *what);
 void upload_positions(Ctx &c, const double *d_pos_aos);  // AoS [n][3] -> px, py, pz, pidx
 template <typename T>
 T *dptr(DevBuf &b)

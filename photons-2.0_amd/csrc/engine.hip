@@ -40,6 +40,11 @@ void *grow(DevBuf &b, size_t bytes)
     return b.p;
 }
 
+void need(const DevBuf &b, size_t bytes, const char *what)
+{
+    if (!b.p || b.cap < bytes) throw std::runtime_error(std::string(what) + " is not available for this tree");
+}
+
 static void release(DevBuf &b)
 {
     if (b.p) (void)hipFree(b.p);
@@ -189,10 +194,15 @@ static void prepare(Ctx &c, const double *d_pos)
     phase_mark(c);
     upload_positions(c, d_pos);
     phase_mark(c);
+    c.lists_valid = false;
     build_tree(c);
     if (c.prm.precision == PHOTONS_FP32_MIXED) compute_offsets_f32(c);
     phase_mark(c);
     upward_pass(c);
+    // local expansions start at zero for this tree (M2L may accumulate into them)
+    const size_t lbytes = (c.prm.precision == PHOTONS_FP32_MIXED ? sizeof(float) : sizeof(double)) *
+                          (size_t)c.ncell * nterms_of(c.prm.order);
+    PH_CHECK(hipMemsetAsync(grow(c.L, lbytes), 0, lbytes, c.stream));
     phase_mark(c);
 }
 
@@ -377,7 +387,11 @@ int photons_get_stats(const photons_ctx *xc, photons_stats *st)
         st->depth = c.maxdepth;
         st->p2p_pairs = c.n_p2p;
         st->m2l_pairs = c.n_m2l;
-        if (c.n == 0 || c.ncell == 0) return 0;
+        if (c.n == 0 || c.ncell == 0 || !c.lists_valid) return 0;
+        need(c.p2p_off, sizeof(int64_t) * (size_t)(c.nleaf + 1), "P2P list");
+        need(c.p2p_src, sizeof(int32_t) * (size_t)c.n_p2p, "P2P list");
+        need(c.m2l_off, sizeof(int64_t) * (size_t)(c.ncell + 1), "M2L list");
+        need(c.m2l_src, sizeof(int32_t) * (size_t)c.n_m2l, "M2L list");
         unsigned long long *d = reinterpret_cast<unsigned long long *>(grow(c.stats, 4 * sizeof(unsigned long long)));
         PH_CHECK(hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), c.stream));
         k_list_stats<<<(c.nleaf + 127) / 128, 128, 0, c.stream>>>(c.nleaf, dptr<CellInfo>(c.info),
@@ -444,8 +458,11 @@ int photons_tree_cells(const photons_ctx *xc, int32_t *npart, int32_t *ipart, in
 static int export_list(Ctx &c, int ntarget, DevBuf &offb, DevBuf &srcb, int64_t total, int64_t cap, int32_t *tgt,
                        int32_t *src, int8_t *shift, int64_t *count)
 {
+    if (!c.lists_valid) total = 0;
     *count = total;
     if (!tgt || total == 0) return 0;
+    need(offb, sizeof(int64_t) * (size_t)(ntarget + 1), "list offsets");
+    need(srcb, sizeof(int32_t) * (size_t)total, "list entries");
     std::vector<int64_t> off(ntarget + 1);
     std::vector<int32_t> s(total);
     PH_CHECK(hipMemcpyAsync(off.data(), offb.p, sizeof(int64_t) * (ntarget + 1), hipMemcpyDeviceToHost, c.stream));
@@ -489,6 +506,8 @@ int photons_expansions(const photons_ctx *xc, double *M, double *L)
         size_t cnt = (size_t)c.ncell * NT;
         if (cnt == 0) return 0;
         const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+        need(c.M, sizeof(double) * cnt, "multipoles");
+        need(c.L, (f32 ? sizeof(float) : sizeof(double)) * cnt, "local expansions");
         if (M) PH_CHECK(hipMemcpyAsync(M, c.M.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, c.stream));
         std::vector<float> lf;
         if (L) {
@@ -659,6 +678,7 @@ int photons_p2p_pairs(photons_ctx *x, int count, const int32_t *tgt, const int32
         PH_CHECK(hipMemcpyAsync(doff, off.data(), sizeof(int64_t) * (c.nleaf + 1), hipMemcpyHostToDevice, c.stream));
         PH_CHECK(hipMemcpyAsync(dsrc, ent.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, c.stream));
         c.n_p2p = count;
+        c.lists_valid = false;  // the M2L list no longer matches this P2P list
         double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)c.n));
         PH_CHECK(hipMemcpyAsync(da, acc_tree, sizeof(double) * 3 * c.n, hipMemcpyHostToDevice, c.stream));
         int saved = c.prm.precision;
@@ -688,6 +708,7 @@ int photons_local_expansions(photons_ctx *x, double *L, int write)
         size_t cnt = (size_t)c.ncell * NT;
         if (cnt == 0) return 0;
         if (c.prm.precision == PHOTONS_FP32_MIXED) return fail_msg("local expansions are float in this context");
+        need(c.L, sizeof(double) * cnt, "local expansions");
         if (write) {
             grow(c.L, sizeof(double) * cnt);
             PH_CHECK(hipMemcpyAsync(c.L.p, L, sizeof(double) * cnt, hipMemcpyHostToDevice, c.stream));

@@ -479,7 +479,11 @@ void m2l_pass(Ctx &c, bool accumulate)
     const int NT = nterms(c.prm.order);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
     const size_t tsize = f32 ? sizeof(float) : sizeof(double);
-    void *L = grow(c.L, tsize * (size_t)c.ncell * NT);
+    need(c.L, tsize * (size_t)c.ncell * NT, "local expansions");
+    need(c.M, (sizeof(double) + (f32 ? sizeof(float) : 0)) * (size_t)c.ncell * NT, "multipoles");
+    need(c.m2l_off, sizeof(int64_t) * (size_t)(c.ncell + 1), "M2L list");
+    need(c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_m2l), "M2L list");
+    void *L = c.L.p;
     hipStream_t st = c.stream;
     const CellGeom *geom = dptr<CellGeom>(c.geom);
     const int64_t *off = dptr<int64_t>(c.m2l_off);
@@ -523,6 +527,7 @@ void m2l_pass(Ctx &c, bool accumulate)
 void downward_pass(Ctx &c)
 {
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+    need(c.L, (f32 ? sizeof(float) : sizeof(double)) * (size_t)c.ncell * nterms(c.prm.order), "local expansions");
     hipStream_t st = c.stream;
     const CellGeom *geom = dptr<CellGeom>(c.geom);
     const CellInfo *info = dptr<CellInfo>(c.info);
@@ -545,6 +550,14 @@ void downward_pass(Ctx &c)
 void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, double *out)
 {
     if (c.prm.maxleaf > kP2PMaxLeaf) throw std::runtime_error("maxleaf above 64 is not supported");
+    const bool f32p = c.prm.precision == PHOTONS_FP32_MIXED;
+    need(c.p2p_off, sizeof(int64_t) * (size_t)(c.nleaf + 1), "P2P list");
+    need(c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_p2p), "P2P list");
+    need(c.geom, sizeof(CellGeom) * (size_t)c.ncell, "cells");
+    need(c.px, sizeof(double) * (size_t)c.n, "positions");
+    if (f32p) need(c.ox, sizeof(float) * (size_t)c.n, "float offsets");
+    if (with_l2p) need(c.L, (f32p ? sizeof(float) : sizeof(double)) * (size_t)c.ncell * nterms(c.prm.order),
+                       "local expansions");
     P2PArgs a;
     a.geom = dptr<CellGeom>(c.geom);
     a.info = dptr<CellInfo>(c.info);

@@ -363,6 +363,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     }
     c.n_p2p = n_p2p;
     c.n_m2l = n_m2l;
+    c.lists_valid = true;
 
     // sort into per-target lists in the reference's order, then CSR
     sort_keys(c, c.p2p_keys, c.bfs_b, n_p2p, 37 + bits_for(nleaf));
