@@ -82,6 +82,9 @@ def main():
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
+                    help="first copy MiB mebibytes device to device (a plain copy kernel of known size, "
+                         "to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (each box or domain runs on one GPU);"
@@ -89,6 +92,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src = torch.empty(nbytes, dtype=torch.uint8, device="cuda").fill_(1)
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy calibration: {nbytes} bytes read and written once")
+        del src, dst
     log("generating particles")
     pos, prm = ic.workload_clustered(nside=args.nside, order=args.order)
     n = len(pos)

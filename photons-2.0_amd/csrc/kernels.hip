@@ -334,70 +334,206 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     }
 }
 
+// FP32 pair force per interaction, about 30 VALU instructions:
+//   r2 = |d|^2 + 1e-30 (keeps rsqrt finite for coincident particles, whose d = 0)
+//   1/max(r, eps)^3 = min(rsqrt(r2), 1/eps)^3   (the reference's softening branch)
+//   G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2) = exp(-u^2) (erfcx(u) + 2u/sqrt(pi)),
+//          u = r/(2 rs), exp(-u^2) straight from r2, erfcx(u) = t Q7(t), t = 1/(1+u/2)
+// The particle mass multiplies the sum once at the end.
+struct PairConsts {
+    float ieps, inv4rs, c2u, kexp;
+};
+
+__device__ __forceinline__ void pair_f32(const PairConsts &k, float dx, float dy, float dz, float &ax, float &ay,
+                                         float &az)
+{
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, 1e-30f)));
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    const float rc = fminf(ri, k.ieps);
+    const float r = r2 * ri;
+    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(r, k.inv4rs, 1.0f));
+    const float R = __builtin_fmaf(r, k.c2u, erfcx_tq7(t));
+    const float e = __builtin_amdgcn_exp2f(r2 * k.kexp);
+    const float f = rc * rc * rc * (e * R);
+    ax = __builtin_fmaf(dx, f, ax);
+    ay = __builtin_fmaf(dy, f, ay);
+    az = __builtin_fmaf(dz, f, az);
+}
+
+// Source particles staged in LDS per batch, as float offsets from the target leaf's
+// centre (3 x 4 bytes each).
+constexpr int kP2PStage = 512;
+
+// Lane layout for a target leaf of n particles (1 <= n <= 64); a = 64 / n, e = 64 - a n.
+//   pass A: lane l serves target l % n and the sources q = l / n + i sA of the staged
+//           batch, sA = a when e = 0, else a + 1 (the extra class a then exists only
+//           for targets 0 .. e-1);
+//   pass B: (e > 0 only) the m = n - e targets e .. n-1 still miss class a: each gets
+//           b = 64 / m lanes, lane l < m b serving target e + l % m and the sources
+//           q = a + (a + 1)(l / m + i b).
+// Every (target, source) pair of the batch is computed once, and all 64 lanes run the
+// same number of iterations to within one (about n * sources / 64 in total).
+struct LaneMap {
+    int a, e, m, b;
+    int sA, kA, tA;      // pass A: stride, first source, target
+    int sB, kB, tB;      // pass B: stride, first source (or a huge value: idle), target
+};
+
+__device__ __forceinline__ LaneMap lane_map(int n, int lane)
+{
+    LaneMap L;
+    L.a = 64 / n;
+    L.e = 64 - L.a * n;
+    L.sA = L.e ? L.a + 1 : L.a;
+    L.kA = lane / n;
+    L.tA = lane % n;
+    L.m = n - L.e;
+    L.b = L.e ? 64 / L.m : 0;
+    L.sB = (L.a + 1) * L.b;
+    L.tB = L.e ? L.e + lane % L.m : 0;
+    L.kB = (L.e && lane < L.m * L.b) ? L.a + (L.a + 1) * (lane / L.m) : (1 << 30);
+    return L;
+}
+
+// One wavefront per target leaf.  The source list is consumed in batches: 64 list
+// entries are read at once, one per lane (entry -> cell -> centre offset, formed in
+// double), a wave prefix sum of their particle counts picks the leading entries that
+// fit kP2PStage particles, and their particles are staged in LDS as float offsets
+// from the target centre, all loads of a batch issued together.  The pairs of the
+// batch are then spread over the lanes by the two-pass layout above.  Partial sums
+// are combined per target in a fixed lane order, so results are deterministic.
 template <int P>
 __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
 {
-    __shared__ float sx_[kP2PMaxLeaf], sy_[kP2PMaxLeaf], sz_[kP2PMaxLeaf];
-    __shared__ float red[3][64];
+    __shared__ float sx[kP2PStage], sy[kP2PStage], sz[kP2PStage];
+    __shared__ float ecx[64], ecy[64], ecz[64];
+    __shared__ int eincl[64], ecnt[64], eip[64];
     const int T = blockIdx.x;
     const int lane = threadIdx.x;
     const CellInfo IT = a.info[T];
-    const int nT = IT.npart;
-    if (nT == 0) return;
-    const int nsub = nT >= 32 ? (nT > 64 ? 1 : 64 / nT) : min(64 / nT, 32);
-    const int t = lane % nT, k = lane / nT;
-    const bool active = k < nsub && t < nT;
-    const int it = IT.ipart + t;
-    const float oxt = a.ox[it], oyt = a.oy[it], ozt = a.oz[it];
+    const int n = IT.npart;
+    if (n == 0) return;
+    const LaneMap L = lane_map(n, lane);
+    const float txA = a.ox[IT.ipart + L.tA], tyA = a.oy[IT.ipart + L.tA], tzA = a.oz[IT.ipart + L.tA];
+    const float txB = a.ox[IT.ipart + L.tB], tyB = a.oy[IT.ipart + L.tB], tzB = a.oz[IT.ipart + L.tB];
     const CellGeom gT = a.geom[T];
-    const float m = (float)a.mass, eps = (float)a.soft;
-    const float meps3 = (float)(a.mass / (a.soft * a.soft * a.soft));
-    const float inv2rs = (float)a.inv2rs;
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    for (int64_t e = a.off[T]; e < a.off[T + 1]; e++) {
-        const int32_t s = a.src[e];
-        const int S = entry_cell(s), sid = entry_shift(s);
-        const CellInfo IS = a.info[S];
-        const CellGeom gS = a.geom[S];
-        double shx, shy, shz;
-        shift_vec(sid, a.boxsize, shx, shy, shz);
-        // centre-to-centre offset in double, then float
-        const float bx = (float)((gS.c[0] + shx) - gT.c[0]) - oxt;
-        const float by = (float)((gS.c[1] + shy) - gT.c[1]) - oyt;
-        const float bz = (float)((gS.c[2] + shz) - gT.c[2]) - ozt;
-        if (lane < IS.npart) {
-            const int j = IS.ipart + lane;
-            sx_[lane] = a.ox[j];
-            sy_[lane] = a.oy[j];
-            sz_[lane] = a.oz[j];
+    PairConsts pc;
+    pc.ieps = (float)(1.0 / a.soft);
+    pc.inv4rs = (float)(0.5 * a.inv2rs);
+    pc.c2u = (float)(1.1283791670955126 * a.inv2rs);               // 2/sqrt(pi) * u / r
+    pc.kexp = (float)(-1.4426950408889634 * a.inv2rs * a.inv2rs);  // -log2(e) u^2 / r^2
+    float axA = 0.f, ayA = 0.f, azA = 0.f, axB = 0.f, ayB = 0.f, azB = 0.f;
+    const int64_t e1 = a.off[T + 1];
+    for (int64_t base = a.off[T]; base < e1;) {
+        const int nb = (int)min((int64_t)64, e1 - base);
+        int cnt = 0;
+        if (lane < nb) {
+            const int32_t s = a.src[base + lane];
+            const int S = entry_cell(s), sid = entry_shift(s);
+            const CellInfo IS = a.info[S];
+            const CellGeom gS = a.geom[S];
+            double shx, shy, shz;
+            shift_vec(sid, a.boxsize, shx, shy, shz);
+            ecx[lane] = (float)((gS.c[0] + shx) - gT.c[0]);
+            ecy[lane] = (float)((gS.c[1] + shy) - gT.c[1]);
+            ecz[lane] = (float)((gS.c[2] + shz) - gT.c[2]);
+            eip[lane] = IS.ipart;
+            cnt = IS.npart;
         }
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        // the counts are at most 64 <= kP2PStage, so at least one entry fits
+        const int ne = __popcll(__ballot(lane < nb && incl <= kP2PStage));
+        const int tot = __shfl(incl, ne - 1, 64);
+        ecnt[lane] = cnt;
+        eincl[lane] = lane < ne ? incl : 0x7fffffff;
         __syncthreads();
-        if (active) {
-            for (int q = k; q < IS.npart; q += nsub) {
-                const float dx = bx + sx_[q], dy = by + sy_[q], dz = bz + sz_[q];
-                const float r2 = dx * dx + dy * dy + dz * dz;
-                const float ri = r2 > 0.f ? __builtin_amdgcn_rsqf(r2) : 0.f;
-                const float r = r2 * ri;
-                float f = r < eps ? meps3 : m * ri * ri * ri;
-                f *= split_factor_f32(r * inv2rs);
-                ax += dx * f;
-                ay += dy * f;
-                az += dz * f;
+        // stage: slot p belongs to the first entry whose inclusive count exceeds p
+        constexpr int R = kP2PStage / 64;
+        float vx[R], vy[R], vz[R];
+        int js[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int p = lane + 64 * r;
+            int j = 0;
+#pragma unroll
+            for (int s = 32; s >= 1; s >>= 1)
+                if (eincl[j + s - 1] <= p) j += s;
+            js[r] = j;
+            if (p < tot) {
+                const int gi = eip[j] + p - (eincl[j] - ecnt[j]);
+                vx[r] = a.ox[gi];
+                vy[r] = a.oy[gi];
+                vz[r] = a.oz[gi];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int p = lane + 64 * r;
+            if (p < tot) {
+                const int j = js[r];
+                sx[p] = ecx[j] + vx[r];
+                sy[p] = ecy[j] + vy[r];
+                sz[p] = ecz[j] + vz[r];
             }
         }
         __syncthreads();
-    }
-    red[0][lane] = ax;
-    red[1][lane] = ay;
-    red[2][lane] = az;
-    __syncthreads();
-    if (lane < nT) {
-        float bx = 0.f, by = 0.f, bz = 0.f;
-        for (int kk = 0; kk < nsub; kk++) {
-            bx += red[0][lane + kk * nT];
-            by += red[1][lane + kk * nT];
-            bz += red[2][lane + kk * nT];
+        {
+            int q = L.kA;
+            for (; q + L.sA < tot; q += 2 * L.sA) {
+                pair_f32(pc, sx[q] - txA, sy[q] - tyA, sz[q] - tzA, axA, ayA, azA);
+                const int q2 = q + L.sA;
+                pair_f32(pc, sx[q2] - txA, sy[q2] - tyA, sz[q2] - tzA, axA, ayA, azA);
+            }
+            if (q < tot) pair_f32(pc, sx[q] - txA, sy[q] - tyA, sz[q] - tzA, axA, ayA, azA);
         }
+        if (L.e) {
+            int q = L.kB;
+            for (; q + L.sB < tot; q += 2 * L.sB) {
+                pair_f32(pc, sx[q] - txB, sy[q] - tyB, sz[q] - tzB, axB, ayB, azB);
+                const int q2 = q + L.sB;
+                pair_f32(pc, sx[q2] - txB, sy[q2] - tyB, sz[q2] - tzB, axB, ayB, azB);
+            }
+            if (q < tot) pair_f32(pc, sx[q] - txB, sy[q] - tyB, sz[q] - tzB, axB, ayB, azB);
+        }
+        base += ne;
+        __syncthreads();  // the staging arrays are rewritten by the next batch
+    }
+    // per-target sums in a fixed order: pass A lanes t + n k, then pass B lanes
+    float *red = sx;  // 6 x 64 floats, the staging array is free now
+    red[lane] = axA;
+    red[64 + lane] = ayA;
+    red[128 + lane] = azA;
+    red[192 + lane] = axB;
+    red[256 + lane] = ayB;
+    red[320 + lane] = azB;
+    __syncthreads();
+    if (lane < n) {
+        const int t = lane;
+        float bx = 0.f, by = 0.f, bz = 0.f;
+        const int kmax = t < L.e ? L.a + 1 : L.a;
+        for (int k = 0; k < kmax; k++) {
+            bx += red[t + n * k];
+            by += red[64 + t + n * k];
+            bz += red[128 + t + n * k];
+        }
+        if (t >= L.e) {
+            for (int j = 0; j < L.b; j++) {
+                const int l = (t - L.e) + L.m * j;
+                bx += red[192 + l];
+                by += red[256 + l];
+                bz += red[320 + l];
+            }
+        }
+        const float m = (float)a.mass;
+        bx *= m;
+        by *= m;
+        bz *= m;
+        const int it = IT.ipart + t;
         if (a.with_l2p) {
             constexpr int NT = nterms(P);
             const float *Lp = reinterpret_cast<const float *>(a.L) + (size_t)T * NT;
@@ -406,7 +542,7 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
             for (int q = 0; q < NT; q++) Lr[q] = Lp[q];
             const float is = (float)a.inv_rs;
             float lx, ly, lz;
-            l2p_eval<P, float>(Lr, oxt * is, oyt * is, ozt * is, lx, ly, lz);
+            l2p_eval<P, float>(Lr, a.ox[it] * is, a.oy[it] * is, a.oz[it] * is, lx, ly, lz);
             const float is2 = is * is;
             bx += lx * is2;
             by += ly * is2;
