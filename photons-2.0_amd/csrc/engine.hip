@@ -9,6 +9,7 @@
 //   l2l       L2L level by level
 //   p2p       P2P + L2P, one wavefront per target leaf, result in caller order
 // The tree build and the walk read a few counts back to size the next step.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -455,6 +456,8 @@ int photons_tree_cells(const photons_ctx *xc, int32_t *npart, int32_t *ipart, in
     }
 }
 
+// Lists are exported in the reference's order: per target by (shift id, depth-first
+// rank of the source cell), the order its depth-first walks emit them in.
 static int export_list(Ctx &c, int ntarget, DevBuf &offb, DevBuf &srcb, int64_t total, int64_t cap, int32_t *tgt,
                        int32_t *src, int8_t *shift, int64_t *count)
 {
@@ -463,17 +466,24 @@ static int export_list(Ctx &c, int ntarget, DevBuf &offb, DevBuf &srcb, int64_t 
     if (!tgt || total == 0) return 0;
     need(offb, sizeof(int64_t) * (size_t)(ntarget + 1), "list offsets");
     need(srcb, sizeof(int32_t) * (size_t)total, "list entries");
+    need(c.info, sizeof(CellInfo) * (size_t)c.ncell, "cells");
     std::vector<int64_t> off(ntarget + 1);
     std::vector<int32_t> s(total);
+    std::vector<CellInfo> info(c.ncell);
     PH_CHECK(hipMemcpyAsync(off.data(), offb.p, sizeof(int64_t) * (ntarget + 1), hipMemcpyDeviceToHost, c.stream));
     PH_CHECK(hipMemcpyAsync(s.data(), srcb.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost, c.stream));
+    PH_CHECK(hipMemcpyAsync(info.data(), c.info.p, sizeof(CellInfo) * c.ncell, hipMemcpyDeviceToHost, c.stream));
     PH_CHECK(hipStreamSynchronize(c.stream));
-    for (int t = 0; t < ntarget; t++)
+    auto rank = [&](int32_t e) { return ((int64_t)entry_shift(e) << 32) | (int64_t)info[entry_cell(e)].dfs; };
+    for (int t = 0; t < ntarget; t++) {
+        std::sort(s.begin() + off[t], s.begin() + off[t + 1],
+                  [&](int32_t x, int32_t y) { return rank(x) < rank(y); });
         for (int64_t e = off[t]; e < off[t + 1] && e < cap; e++) {
             tgt[e] = t;
             src[e] = entry_cell(s[e]);
             shift[e] = (int8_t)entry_shift(s[e]);
         }
+    }
     return 0;
 }
 

@@ -171,23 +171,6 @@ __global__ void __launch_bounds__(256) k_scatter(LevelArgs a, const double *__re
     }
 }
 
-__global__ void __launch_bounds__(256) k_copy_back(LevelArgs a, Particles src, Particles dst)
-{
-    int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
-    int lane = threadIdx.x % kWave;
-    if (c >= a.C) return;
-    int s = find_seg(a.seg_chunk, a.S, c);
-    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
-    int b = st + (c - a.seg_chunk[s]) * kChunk;
-    int e = min(b + kChunk, st + ct);
-    for (int i = b + lane; i < e; i += kWave) {
-        dst.x[i] = src.x[i];
-        dst.y[i] = src.y[i];
-        dst.z[i] = src.z[i];
-        dst.idx[i] = src.idx[i];
-    }
-}
-
 __global__ void k_child_kinds(LevelArgs a, const int *__restrict__ nleft, int maxleaf, int *__restrict__ nn,
                               int *__restrict__ nl)
 {
@@ -256,6 +239,24 @@ __global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, c
             }
             T.t_son[2 * v + k] = -(id + 1);
         }
+    }
+}
+
+// The levels scatter alternately into the two particle buffers and never copy back:
+// the particles of a leaf at depth D are left in buffer D % 2.  This pass moves the
+// leaves held by the second buffer into the first, one wavefront per leaf.
+__global__ void __launch_bounds__(256) k_gather_leaves(TreeTmp T, int nleaf, Particles src, Particles dst)
+{
+    int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    int lane = threadIdx.x % kWave;
+    // leaf depth = parent depth + 1; an even leaf depth is already in the first buffer
+    if (l >= nleaf || (T.t_depth[T.l_parent[l]] & 1) == 1) return;
+    int b = T.l_start[l], e = b + T.l_count[l];
+    for (int i = b + lane; i < e; i += kWave) {
+        dst.x[i] = src.x[i];
+        dst.y[i] = src.y[i];
+        dst.z[i] = src.z[i];
+        dst.idx[i] = src.idx[i];
     }
 }
 
@@ -507,7 +508,7 @@ void build_tree(Ctx &c)
         exclusive_scan(c, rcnt, rpre, C + 1);
         k_seg_counts<<<blocks(S, 256), 256, 0, st>>>(a, rpre, nleft, flag);
         k_scatter<<<wb, 256, 0, st>>>(a, T.t_split, rpre, nleft, flag, A, B);
-        k_copy_back<<<wb, 256, 0, st>>>(a, B, A);
+        std::swap(A, B);  // the children's particles are in B now
         int *nn = ensure<int>(c.seg_nn, S + 1);
         int *nl = ensure<int>(c.seg_nl, S + 1);
         int *non = ensure<int>(c.seg_non, S + 1);
@@ -534,6 +535,12 @@ void build_tree(Ctx &c)
         if (depth > kMaxDepth) throw std::runtime_error("tree deeper than the supported number of levels");
     }
     const int nlevels = depth;  // levels holding nodes: 0 .. depth-1
+    // leaves at odd depth sit in the second buffer
+    {
+        Particles P{dptr<double>(c.px), dptr<double>(c.py), dptr<double>(c.pz), dptr<int>(c.pidx)};
+        Particles Q{dptr<double>(c.qx), dptr<double>(c.qy), dptr<double>(c.qz), dptr<int>(c.qidx)};
+        if (nleaf > 0) k_gather_leaves<<<blocks(nleaf, kWavesPerBlock), 256, 0, st>>>(T, nleaf, Q, P);
+    }
 
     // subtree sizes, bottom-up
     int *size = ensure<int>(c.t_size, ntmp);

@@ -8,13 +8,14 @@
 // rules emits the same set.
 //
 // MI355X formulation: a breadth-first walk over the whole pair space.  Each level
-// is one launch with one thread per pair: the thread applies the rule and appends
-// its children (1, 2 or 4 pairs) or its emitted task, with one atomic per
-// wavefront per output.  No stack, no per-thread divergence beyond one rule, full
-// occupancy.  The emitted tasks are then sorted by (target, shift, source rank in
-// depth-first order): for one target the reference's depth-first walk emits its
-// sources in exactly that order (a source subtree is finished before its sibling's
-// starts), so the lists come out in the reference's order and are deterministic.
+// is two launches with one thread per pair (count, then write at scanned offsets):
+// the thread applies the rule and appends its children (1, 2 or 4 pairs) or its
+// emitted task.  No stack, no atomics, no per-thread divergence beyond one rule.
+// The emitted tasks (target, packed source entry) are then grouped by target with a
+// stable radix sort on the target id alone, so every list is deterministic.  Within
+// one target the entries keep the walk's emission order; the list exports restore
+// the reference's depth-first order, which is (shift, source rank in depth-first
+// order): the reference finishes a source subtree before its sibling's starts.
 //
 // Image walks: a shifted node keeps no children when its box is rc or more from
 // the domain box or when 0.95*theta*distance exceeds its largest width
@@ -35,10 +36,10 @@ __device__ __forceinline__ uint64_t pair_of(uint32_t A, uint32_t B, uint32_t sid
     return (uint64_t)A | ((uint64_t)B << 27) | ((uint64_t)sid << 54);
 }
 
-// task key: target in bits 37-63, shift id in bits 32-36, source rank in bits 0-31
-__device__ __forceinline__ uint64_t task_key(uint32_t target, uint32_t sid, uint32_t rank)
+// task: target cell in bits 32-63, packed source entry (cell, shift id) in bits 0-31
+__device__ __forceinline__ uint64_t task_of(uint32_t target, int cell, int sid)
 {
-    return ((uint64_t)target << 37) | ((uint64_t)sid << 32) | (uint64_t)rank;
+    return ((uint64_t)target << 32) | (uint64_t)(uint32_t)pack_entry(cell, sid);
 }
 
 struct BfsArgs {
@@ -49,7 +50,9 @@ struct BfsArgs {
     CellGeom root;  // domain box
     const uint64_t *in;
     int64_t n_in;
-    uint64_t *out, *p2p, *m2l;
+    uint64_t *out;
+    uint32_t *p2p_t, *m2l_t;  // task targets
+    int32_t *p2p_s, *m2l_s;   // task source entries
     int64_t p2p_base, m2l_base;  // tasks emitted by earlier levels
 };
 
@@ -97,7 +100,7 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, uint64_t
     const bool aleaf = A < nleaf, bleaf = B < nleaf;
     if (sid == 0 && A == B) {  // self pair
         if (aleaf) {
-            ptask = task_key(A, 0, B);
+            ptask = task_of(A, B, 0);
             np = 1;
         } else {
             const CellInfo IA = a.info[A];
@@ -110,7 +113,7 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, uint64_t
         return;
     }
     if (aleaf && bleaf) {  // leaf-leaf pairs are always P2P
-        ptask = task_key(A, sid, B);
+        ptask = task_of(A, B, sid);
         np = 1;
         return;
     }
@@ -127,7 +130,7 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, uint64_t
     if (flag == 1 || (aleaf && pruned)) {
         // accepted; in the image walk a target leaf against a pruned node takes M2L
         // whatever the acceptance says (src/remotes.c:442)
-        mtask = task_key(A, sid, a.info[B].dfs);
+        mtask = task_of(A, B, sid);
         nm = 1;
         return;
     } else if (aleaf) {
@@ -213,8 +216,14 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const int6
 #pragma unroll
     for (int k = 0; k < kBfsPerThread; k++) {
         for (int j = 0; j < nk[k]; j++) a.out[bk++] = kids[k][j];
-        if (np[k]) a.p2p[bp++] = pt[k];
-        if (nm[k]) a.m2l[bm++] = mt[k];
+        if (np[k]) {
+            a.p2p_t[bp] = (uint32_t)(pt[k] >> 32);
+            a.p2p_s[bp++] = (int32_t)(uint32_t)pt[k];
+        }
+        if (nm[k]) {
+            a.m2l_t[bm] = (uint32_t)(mt[k] >> 32);
+            a.m2l_s[bm++] = (int32_t)(uint32_t)mt[k];
+        }
     }
 }
 
@@ -234,35 +243,18 @@ __global__ void k_init_frontier(uint64_t *f, int A, int B, int shift_lo, int shi
     if (s < shift_hi) f[threadIdx.x] = pair_of(A, B, s);
 }
 
-// CSR offsets of the sorted task keys: off[t] = first key with target >= t
-__global__ void k_csr_offsets(const uint64_t *__restrict__ keys, int64_t n, int ntarget, int64_t *__restrict__ off)
+// CSR offsets of the target-sorted tasks: off[t] = first task with target >= t
+__global__ void k_csr_offsets(const uint32_t *__restrict__ tgt, int64_t n, int ntarget, int64_t *__restrict__ off)
 {
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t > ntarget) return;
-    uint64_t want = (uint64_t)t << 37;
     int64_t lo = 0, hi = n;
     while (lo < hi) {
         int64_t mid = (lo + hi) >> 1;
-        if (keys[mid] < want) lo = mid + 1;
+        if (tgt[mid] < (uint32_t)t) lo = mid + 1;
         else hi = mid;
     }
     off[t] = lo;
-}
-
-__global__ void k_unpack_tasks(const uint64_t *__restrict__ keys, int64_t n, const int *__restrict__ inv_dfs,
-                               int32_t *__restrict__ src)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    uint64_t k = keys[i];
-    int rank = (int)(k & 0xffffffffu), sid = (int)((k >> 32) & 31);
-    src[i] = pack_entry(inv_dfs ? inv_dfs[rank] : rank, sid);
-}
-
-__global__ void k_inv_dfs(const CellInfo *__restrict__ info, int ncell, int *__restrict__ inv)
-{
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < ncell) inv[info[c].dfs] = c;
 }
 
 // ---------------------------------------------------------------------------- host side
@@ -274,32 +266,39 @@ static int bits_for(int64_t v)
     return b;
 }
 
-// grow keeping the first `keep` elements
-static uint64_t *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
+// grow to `need` bytes keeping the first `keep` bytes
+static void *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
 {
-    if (need * sizeof(uint64_t) <= b.cap) return reinterpret_cast<uint64_t *>(b.p);
+    if (need <= b.cap) return b.p;
     size_t want = need + need / 2;
-    uint64_t *nw = nullptr;
-    PH_CHECK(hipMalloc(&nw, want * sizeof(uint64_t)));
-    if (b.p && keep) PH_CHECK(hipMemcpyAsync(nw, b.p, keep * sizeof(uint64_t), hipMemcpyDeviceToDevice, st));
+    void *nw = nullptr;
+    PH_CHECK(hipMalloc(&nw, want));
+    if (b.p && keep) PH_CHECK(hipMemcpyAsync(nw, b.p, keep, hipMemcpyDeviceToDevice, st));
     PH_CHECK(hipStreamSynchronize(st));
     if (b.p) PH_CHECK(hipFree(b.p));
     b.p = nw;
-    b.cap = want * sizeof(uint64_t);
+    b.cap = want;
     return nw;
 }
 
-static void sort_keys(Ctx &c, DevBuf &keys, DevBuf &alt, int64_t n, int end_bit)
+// stable sort of (target, entry) tasks by target; the spare buffers take the
+// alternate halves and may be swapped with the task buffers
+static void sort_tasks(Ctx &c, DevBuf &tgt, DevBuf &src, DevBuf &spare_t, DevBuf &spare_s, int64_t n, int end_bit)
 {
     if (n <= 1) return;
-    uint64_t *k0 = reinterpret_cast<uint64_t *>(keys.p);
-    uint64_t *k1 = reinterpret_cast<uint64_t *>(grow(alt, sizeof(uint64_t) * n));
-    hipcub::DoubleBuffer<uint64_t> db(k0, k1);
+    if (n > (int64_t)INT32_MAX) throw std::runtime_error("more than 2^31 interaction-list entries in one walk");
+    uint32_t *t0 = reinterpret_cast<uint32_t *>(tgt.p);
+    int32_t *s0 = reinterpret_cast<int32_t *>(src.p);
+    uint32_t *t1 = reinterpret_cast<uint32_t *>(grow(spare_t, sizeof(uint32_t) * n));
+    int32_t *s1 = reinterpret_cast<int32_t *>(grow(spare_s, sizeof(int32_t) * n));
+    hipcub::DoubleBuffer<uint32_t> kb(t0, t1);
+    hipcub::DoubleBuffer<int32_t> vb(s0, s1);
     size_t bytes = 0;
-    PH_CHECK(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, db, (int)n, 0, end_bit, c.stream));
+    PH_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kb, vb, (int)n, 0, end_bit, c.stream));
     void *tmp = grow(c.scan_tmp, bytes);
-    PH_CHECK(hipcub::DeviceRadixSort::SortKeys(tmp, bytes, db, (int)n, 0, end_bit, c.stream));
-    if (db.Current() != k0) std::swap(keys, alt);
+    PH_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kb, vb, (int)n, 0, end_bit, c.stream));
+    if (kb.Current() != t0) std::swap(tgt, spare_t);
+    if (vb.Current() != s0) std::swap(src, spare_s);
 }
 
 void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int start_source)
@@ -322,7 +321,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     PH_CHECK(hipStreamSynchronize(st));
 
     // frontier ping-pong and task buffers, kept from the last call and grown exactly
-    grow_keep(c.bfs_a, 64, 0, st);
+    grow_keep(c.bfs_a, 64 * sizeof(uint64_t), 0, st);
     int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 16;
 
     const int nshift = shift_hi - shift_lo;
@@ -346,12 +345,13 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
                                     st));
         PH_CHECK(hipStreamSynchronize(st));
         const int64_t n_out = pin[0], add_p2p = pin[1], add_m2l = pin[2];
-        grow_keep(c.bfs_b, std::max<int64_t>(n_out, 1), 0, st);
-        grow_keep(c.p2p_keys, std::max<int64_t>(n_p2p + add_p2p, 1), n_p2p, st);
-        grow_keep(c.m2l_keys, std::max<int64_t>(n_m2l + add_m2l, 1), n_m2l, st);
+        grow_keep(c.bfs_b, sizeof(uint64_t) * std::max<int64_t>(n_out, 1), 0, st);
+        const size_t np = std::max<int64_t>(n_p2p + add_p2p, 1), nm = std::max<int64_t>(n_m2l + add_m2l, 1);
+        a.p2p_t = static_cast<uint32_t *>(grow_keep(c.p2p_keys, 4 * np, 4 * n_p2p, st));
+        a.p2p_s = static_cast<int32_t *>(grow_keep(c.p2p_src, 4 * np, 4 * n_p2p, st));
+        a.m2l_t = static_cast<uint32_t *>(grow_keep(c.m2l_keys, 4 * nm, 4 * n_m2l, st));
+        a.m2l_s = static_cast<int32_t *>(grow_keep(c.m2l_src, 4 * nm, 4 * n_m2l, st));
         a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
-        a.p2p = reinterpret_cast<uint64_t *>(c.p2p_keys.p);
-        a.m2l = reinterpret_cast<uint64_t *>(c.m2l_keys.p);
         a.p2p_base = n_p2p;
         a.m2l_base = n_m2l;
         k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off, ntile);
@@ -365,25 +365,17 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     c.n_m2l = n_m2l;
     c.lists_valid = true;
 
-    // sort into per-target lists in the reference's order, then CSR
-    sort_keys(c, c.p2p_keys, c.bfs_b, n_p2p, 37 + bits_for(nleaf));
-    sort_keys(c, c.m2l_keys, c.bfs_b, n_m2l, 37 + bits_for(ncell));
+    // group by target, then CSR offsets
+    sort_tasks(c, c.p2p_keys, c.p2p_src, c.bfs_a, c.bfs_b, n_p2p, bits_for(nleaf));
+    sort_tasks(c, c.m2l_keys, c.m2l_src, c.bfs_a, c.bfs_b, n_m2l, bits_for(ncell));
     int64_t *p2p_off = reinterpret_cast<int64_t *>(grow(c.p2p_off, sizeof(int64_t) * (size_t)(nleaf + 1)));
     int64_t *m2l_off = reinterpret_cast<int64_t *>(grow(c.m2l_off, sizeof(int64_t) * (size_t)(ncell + 1)));
-    int32_t *p2p_src = reinterpret_cast<int32_t *>(grow(c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, n_p2p)));
-    int32_t *m2l_src = reinterpret_cast<int32_t *>(grow(c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, n_m2l)));
-    int *inv = reinterpret_cast<int *>(grow(c.t_final, sizeof(int) * (size_t)ncell));
-    k_inv_dfs<<<(ncell + 255) / 256, 256, 0, st>>>(a.info, ncell, inv);
-    k_csr_offsets<<<(nleaf + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.p2p_keys.p), n_p2p, nleaf,
+    grow(c.p2p_src, sizeof(int32_t));
+    grow(c.m2l_src, sizeof(int32_t));
+    k_csr_offsets<<<(nleaf + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint32_t *>(c.p2p_keys.p), n_p2p, nleaf,
                                                            p2p_off);
-    k_csr_offsets<<<(ncell + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.m2l_keys.p), n_m2l, ncell,
+    k_csr_offsets<<<(ncell + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint32_t *>(c.m2l_keys.p), n_m2l, ncell,
                                                            m2l_off);
-    if (n_p2p)
-        k_unpack_tasks<<<(int)((n_p2p + 255) / 256), 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.p2p_keys.p), n_p2p,
-                                                                   nullptr, p2p_src);
-    if (n_m2l)
-        k_unpack_tasks<<<(int)((n_m2l + 255) / 256), 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.m2l_keys.p), n_m2l,
-                                                                   inv, m2l_src);
     PH_CHECK(hipGetLastError());
 }
 
