@@ -21,6 +21,7 @@
 // the domain box or when 0.95*theta*distance exceeds its largest width
 // (src/remotes.c:145-158); the rule is evaluated on the fly, no copy is built.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <stdexcept>
@@ -89,32 +90,54 @@ __device__ __forceinline__ bool let_pruned(const CellGeom &root, const CellGeom 
     return wmax < 0.95 * theta * dr;
 }
 
-// The rule applied to one pair (src/fmm.c:406-712, src/remotes.c:213-552): its
-// children, or the P2P or M2L task it emits.
-__device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, uint64_t *kids, int &nkid, uint64_t &ptask,
-                                         int &np, uint64_t &mtask, int &nm)
+// Outputs of the rule for one pair, in four fixed slots (static indices keep them in
+// registers): slot c holds a frontier pair when bit c of `kid` is set, a P2P task
+// (target, entry) when bit c of `p2p` is set; `m2l` holds an M2L task when has_m2l.
+struct RuleOut {
+    uint64_t slot[4];
+    unsigned kid, p2p;
+    uint64_t m2l;
+    int has_m2l;
+};
+
+// A child pair of two leaves is always a P2P task (the rule below), so it is emitted
+// as one straight away instead of passing through the next level's frontier.
+__device__ __forceinline__ void put_child(const BfsArgs &a, RuleOut &o, int c, int A, int B, int sid)
 {
-    nkid = np = nm = 0;
+    if (A < a.nleaf && B < a.nleaf) {
+        o.slot[c] = task_of(A, B, sid);
+        o.p2p |= 1u << c;
+    } else {
+        o.slot[c] = pair_of(A, B, sid);
+        o.kid |= 1u << c;
+    }
+}
+
+// The rule applied to one pair (src/fmm.c:406-712, src/remotes.c:213-552): its
+// children (up to 4, leaf-leaf children as P2P tasks), its P2P task or its M2L task.
+__device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, RuleOut &o)
+{
+    o.kid = o.p2p = 0;
+    o.has_m2l = 0;
     const int A = (int)(pr & 0x7ffffff), B = (int)((pr >> 27) & 0x7ffffff), sid = (int)(pr >> 54);
     const int nleaf = a.nleaf;
     const bool aleaf = A < nleaf, bleaf = B < nleaf;
     if (sid == 0 && A == B) {  // self pair
         if (aleaf) {
-            ptask = task_of(A, B, 0);
-            np = 1;
+            o.slot[0] = task_of(A, B, 0);
+            o.p2p = 1;
         } else {
             const CellInfo IA = a.info[A];
-            kids[0] = pair_of(IA.son0, IA.son0, 0);
-            kids[1] = pair_of(IA.son0, IA.son1, 0);
-            kids[2] = pair_of(IA.son1, IA.son0, 0);
-            kids[3] = pair_of(IA.son1, IA.son1, 0);
-            nkid = 4;
+            put_child(a, o, 0, IA.son0, IA.son0, 0);
+            put_child(a, o, 1, IA.son0, IA.son1, 0);
+            put_child(a, o, 2, IA.son1, IA.son0, 0);
+            put_child(a, o, 3, IA.son1, IA.son1, 0);
         }
         return;
     }
     if (aleaf && bleaf) {  // leaf-leaf pairs are always P2P
-        ptask = task_of(A, B, sid);
-        np = 1;
+        o.slot[0] = task_of(A, B, sid);
+        o.p2p = 1;
         return;
     }
     double sx, sy, sz;
@@ -130,8 +153,8 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, uint64_t
     if (flag == 1 || (aleaf && pruned)) {
         // accepted; in the image walk a target leaf against a pruned node takes M2L
         // whatever the acceptance says (src/remotes.c:442)
-        mtask = task_of(A, B, sid);
-        nm = 1;
+        o.m2l = task_of(A, B, sid);
+        o.has_m2l = 1;
         return;
     } else if (aleaf) {
         open = 2;
@@ -143,19 +166,20 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, uint64_t
     }
     if (open == 1) {
         const CellInfo IA = a.info[A];
-        kids[0] = pair_of(IA.son0, B, sid);
-        kids[1] = pair_of(IA.son1, B, sid);
+        put_child(a, o, 0, IA.son0, B, sid);
+        put_child(a, o, 1, IA.son1, B, sid);
     } else {
         const CellInfo IB = a.info[B];
-        kids[0] = pair_of(A, IB.son0, sid);
-        kids[1] = pair_of(A, IB.son1, sid);
+        put_child(a, o, 0, A, IB.son0, sid);
+        put_child(a, o, 1, A, IB.son1, sid);
     }
-    nkid = 2;
 }
 
 // Each level runs in two passes without atomics: k_bfs_count sums the outputs of
 // every tile of kBfsTile pairs, the host scans those sums (exact output sizes), and
 // k_bfs_write re-applies the rule and writes each output at its scanned position.
+// Within a tile the outputs keep the input order (item k * kBfsThreads + thread), so
+// neighbouring threads store to neighbouring addresses.
 constexpr int kBfsThreads = 256;
 constexpr int kBfsPerThread = 4;
 constexpr int kBfsTile = kBfsThreads * kBfsPerThread;
@@ -168,12 +192,11 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, int64_t *_
     for (int k = 0; k < kBfsPerThread; k++) {
         const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
         if (i < a.n_in) {
-            uint64_t kids[4], pt, mt;
-            int nk, np, nm;
-            bfs_rule(a, a.in[i], kids, nk, pt, np, mt, nm);
-            ck += nk;
-            cp += np;
-            cm += nm;
+            RuleOut o;
+            bfs_rule(a, a.in[i], o);
+            ck += __popc(o.kid);
+            cp += __popc(o.p2p);
+            cm += o.has_m2l;
         }
     }
     int tk = Reduce(tmp).Sum(ck);
@@ -188,42 +211,59 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, int64_t *_
     }
 }
 
+// The counts of a thread's kBfsPerThread items sit in 16-bit fields of one 64-bit
+// word (a field's tile sum is at most 4 * kBfsThreads), so one block scan gives the
+// offset of every item within the tile's output.
+__device__ __forceinline__ int field(uint64_t v, int k) { return (int)((v >> (16 * k)) & 0xffff); }
+
 __global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const int64_t *__restrict__ tile_off, int ntile)
 {
-    typedef hipcub::BlockScan<int, kBfsThreads> Scan;
+    static_assert(kBfsPerThread <= 4 && 4 * kBfsThreads < 65536, "16-bit count fields");
+    typedef hipcub::BlockScan<uint64_t, kBfsThreads> Scan;
     __shared__ typename Scan::TempStorage tmp;
-    uint64_t kids[kBfsPerThread][4], pt[kBfsPerThread], mt[kBfsPerThread];
-    int nk[kBfsPerThread], np[kBfsPerThread], nm[kBfsPerThread];
-    int ck = 0, cp = 0, cm = 0;
+    RuleOut o[kBfsPerThread];
+    uint64_t ck = 0, cp = 0, cm = 0;
 #pragma unroll
     for (int k = 0; k < kBfsPerThread; k++) {
         const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
-        nk[k] = np[k] = nm[k] = 0;
-        if (i < a.n_in) bfs_rule(a, a.in[i], kids[k], nk[k], pt[k], np[k], mt[k], nm[k]);
-        ck += nk[k];
-        cp += np[k];
-        cm += nm[k];
+        o[k].kid = o[k].p2p = 0;
+        o[k].has_m2l = 0;
+        if (i < a.n_in) bfs_rule(a, a.in[i], o[k]);
+        ck |= (uint64_t)__popc(o[k].kid) << (16 * k);
+        cp |= (uint64_t)__popc(o[k].p2p) << (16 * k);
+        cm |= (uint64_t)o[k].has_m2l << (16 * k);
     }
-    int ok, op, om;
-    Scan(tmp).ExclusiveSum(ck, ok);
+    uint64_t ok, op, om, tk, tp, tm;
+    Scan(tmp).ExclusiveSum(ck, ok, tk);
     __syncthreads();
-    Scan(tmp).ExclusiveSum(cp, op);
+    Scan(tmp).ExclusiveSum(cp, op, tp);
     __syncthreads();
-    Scan(tmp).ExclusiveSum(cm, om);
-    int64_t bk = tile_off[blockIdx.x] + ok;
-    int64_t bp = a.p2p_base + tile_off[(ntile + 1) + blockIdx.x] + op;
-    int64_t bm = a.m2l_base + tile_off[2 * (ntile + 1) + blockIdx.x] + om;
+    Scan(tmp).ExclusiveSum(cm, om, tm);
+    const int64_t bk = tile_off[blockIdx.x];
+    const int64_t bp = a.p2p_base + tile_off[(ntile + 1) + blockIdx.x];
+    const int64_t bm = a.m2l_base + tile_off[2 * (ntile + 1) + blockIdx.x];
+    int sk = 0, sp = 0, sm = 0;  // outputs of the tile's earlier items
 #pragma unroll
     for (int k = 0; k < kBfsPerThread; k++) {
-        for (int j = 0; j < nk[k]; j++) a.out[bk++] = kids[k][j];
-        if (np[k]) {
-            a.p2p_t[bp] = (uint32_t)(pt[k] >> 32);
-            a.p2p_s[bp++] = (int32_t)(uint32_t)pt[k];
+        const int64_t qk = bk + sk + field(ok, k), qp = bp + sp + field(op, k);
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const unsigned below = (1u << c) - 1u;
+            if ((o[k].kid >> c) & 1u) a.out[qk + __popc(o[k].kid & below)] = o[k].slot[c];
+            if ((o[k].p2p >> c) & 1u) {
+                const int64_t q = qp + __popc(o[k].p2p & below);
+                a.p2p_t[q] = (uint32_t)(o[k].slot[c] >> 32);
+                a.p2p_s[q] = (int32_t)(uint32_t)o[k].slot[c];
+            }
         }
-        if (nm[k]) {
-            a.m2l_t[bm] = (uint32_t)(mt[k] >> 32);
-            a.m2l_s[bm++] = (int32_t)(uint32_t)mt[k];
+        if (o[k].has_m2l) {
+            const int64_t q = bm + sm + field(om, k);
+            a.m2l_t[q] = (uint32_t)(o[k].m2l >> 32);
+            a.m2l_s[q] = (int32_t)(uint32_t)o[k].m2l;
         }
+        sk += field(tk, k);
+        sp += field(tp, k);
+        sm += field(tm, k);
     }
 }
 
@@ -281,24 +321,32 @@ static void *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
     return nw;
 }
 
-// stable sort of (target, entry) tasks by target; the spare buffers take the
-// alternate halves and may be swapped with the task buffers
+// Stable LSD radix sort of (target, entry) tasks by target (20-21 bits at 256^3
+// particles).  8-bit digits: three passes; 11-bit digits (two passes) measured slower
+// on MI355X because their onesweep blocks hold 139 KB of LDS.  The item count is a
+// size_t, so lists beyond 2^31 entries sort too.
+constexpr unsigned kTaskSortBits = 8;
+using TaskSortConfig = rocprim::radix_sort_config<
+    rocprim::default_config, rocprim::default_config,
+    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>,
+                                        kTaskSortBits, rocprim::block_radix_rank_algorithm::match>>;
+
+// the spare buffers take the alternate halves and may be swapped with the task buffers
 static void sort_tasks(Ctx &c, DevBuf &tgt, DevBuf &src, DevBuf &spare_t, DevBuf &spare_s, int64_t n, int end_bit)
 {
     if (n <= 1) return;
-    if (n > (int64_t)INT32_MAX) throw std::runtime_error("more than 2^31 interaction-list entries in one walk");
     uint32_t *t0 = reinterpret_cast<uint32_t *>(tgt.p);
     int32_t *s0 = reinterpret_cast<int32_t *>(src.p);
     uint32_t *t1 = reinterpret_cast<uint32_t *>(grow(spare_t, sizeof(uint32_t) * n));
     int32_t *s1 = reinterpret_cast<int32_t *>(grow(spare_s, sizeof(int32_t) * n));
-    hipcub::DoubleBuffer<uint32_t> kb(t0, t1);
-    hipcub::DoubleBuffer<int32_t> vb(s0, s1);
+    rocprim::double_buffer<uint32_t> kb(t0, t1);
+    rocprim::double_buffer<int32_t> vb(s0, s1);
     size_t bytes = 0;
-    PH_CHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kb, vb, (int)n, 0, end_bit, c.stream));
+    PH_CHECK(rocprim::radix_sort_pairs<TaskSortConfig>(nullptr, bytes, kb, vb, (size_t)n, 0, end_bit, c.stream));
     void *tmp = grow(c.scan_tmp, bytes);
-    PH_CHECK(hipcub::DeviceRadixSort::SortPairs(tmp, bytes, kb, vb, (int)n, 0, end_bit, c.stream));
-    if (kb.Current() != t0) std::swap(tgt, spare_t);
-    if (vb.Current() != s0) std::swap(src, spare_s);
+    PH_CHECK(rocprim::radix_sort_pairs<TaskSortConfig>(tmp, bytes, kb, vb, (size_t)n, 0, end_bit, c.stream));
+    if (kb.current() != t0) std::swap(tgt, spare_t);
+    if (vb.current() != s0) std::swap(src, spare_s);
 }
 
 void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int start_source)
@@ -306,7 +354,6 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     hipStream_t st = c.stream;
     const int nleaf = c.nleaf, ncell = c.ncell;
     if (ncell >= (1 << 27)) throw std::runtime_error("more than 2^27 cells");
-    const int64_t n = std::max<int64_t>(c.n, 1);
 
     BfsArgs a{};
     a.geom = dptr<CellGeom>(c.geom);
