@@ -6,6 +6,13 @@ precisions run on the same particles: FP32-mixed (the headline `value`) and FP64
 Per-phase GPU times come from HIP events the library records on its own stream.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--nside 256] [--order 4]
+
+Two more workloads print their own JSON line (not the default):
+    --workload ensemble   64 independent clustered boxes of 32^3..128^3 particles (log-uniform,
+                          seed 5) in one batched evaluation, leaf size and P2P staging autotuned
+    --workload serving    256 resident boxes of 16^3..192^3 particles (seed 6), single-step
+                          (force + kick/drift) requests with Poisson arrivals at swept rates,
+                          HBM budget 200 GB: throughput and p50/p99 step latency per rate
 """
 from __future__ import annotations
 
@@ -13,6 +20,7 @@ import argparse
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -73,6 +81,131 @@ def cpu_baseline(nside=40):
                       f"as the 256^3 workload (box {prm.boxsize:.0f} kpc/h), one force evaluation, {dt:.1f} s"}
 
 
+def run_ensemble(args, torch):
+    """64 clustered boxes (32^3..128^3, log-uniform, seed 5) as one batched evaluation."""
+    log("generating the ensemble")
+    counts, lengths, pos, prm = ic.workload_ensemble(64, 32 ** 3, 128 ** 3, seed=5, order=args.order)
+    n = int(counts.sum())
+    d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to("cuda")
+    d_acc = torch.empty_like(d_pos)
+    res = {}
+    for name, prec in (("fp32", pg.FP32_MIXED), ("fp64", pg.FP64)):
+        ctx = pg.Context(prm, prec)
+        leaf, stage, table = ctx.autotune_boxes(counts, lengths, d_pos.data_ptr(), d_acc.data_ptr(), reps=1)
+        log(f"[{name}] autotuned leaf {leaf} staging {stage}: {table}")
+        wall = []
+        for s in range(args.warmup + args.steps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.force_boxes_device(counts, lengths, d_pos.data_ptr(), d_acc.data_ptr())
+            ctx.sync()
+            torch.cuda.synchronize()
+            if s >= args.warmup:
+                wall.append(time.perf_counter() - t0)
+                log(f"[{name}] step {s - args.warmup + 1}/{args.steps} {wall[-1] * 1e3:.1f} ms")
+        res[name] = dict(wall=np.array(wall), acc=d_acc.cpu().numpy(), leaf=leaf, stage=stage,
+                         table={f"leaf{k[0]}_stage{k[1]}": v for k, v in table.items()}, stats=ctx.stats())
+        ctx.close()
+    e = np.linalg.norm(res["fp32"]["acc"] - res["fp64"]["acc"], axis=1) / np.linalg.norm(res["fp64"]["acc"], axis=1)
+    ms = res["fp32"]["wall"] * 1e3
+    return {
+        "metric": METRIC, "value": n / float(np.mean(res["fp32"]["wall"])), "unit": "particles/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": float(np.mean(ms)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed",
+        "data": "synthetic",
+        "config": {"workload": "ensemble of 64 periodic clustered boxes, 32^3..128^3 particles (log-uniform, "
+                               "seed 5), density of 256^3 in 500 Mpc/h, one batched evaluation, autotuned",
+                   "particles": n, "boxes": 64, "order": prm.order, "theta": prm.theta,
+                   "leaf_fp32": res["fp32"]["leaf"], "p2p_staging_fp32": res["fp32"]["stage"],
+                   "leaf_fp64": res["fp64"]["leaf"]},
+        "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
+        "autotune_ms": {"fp32": res["fp32"]["table"], "fp64": res["fp64"]["table"]},
+        "fp64": {"value": n / float(np.mean(res["fp64"]["wall"])),
+                 "ms_per_step": float(np.mean(res["fp64"]["wall"]) * 1e3)},
+        "fp32_over_fp64": float(np.mean(res["fp64"]["wall"]) / np.mean(res["fp32"]["wall"])),
+        "error_fp32_vs_fp64": {"p50": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max())},
+        "p2p_interactions": res["fp32"]["stats"]["p2p_interactions"],
+    }
+
+
+def run_serving(args, torch):
+    """Resident boxes, Poisson arrivals of single-step requests at swept rates."""
+    nbox = args.serve_boxes
+    log(f"generating {nbox} boxes")
+    counts, lengths, pos, prm = ic.workload_ensemble(nbox, 16 ** 3, 192 ** 3, seed=6, order=args.order)
+    off = np.concatenate([[0], np.cumsum(counts)])
+    vel = ic.normal(7, pos.size).reshape(pos.shape) * 300.0
+    dkh = 0.5 * pg.kick_factor(np.log(0.30), np.log(0.31)) * ic.GRAV_CONST
+    dd = pg.drift_factor(np.log(0.30), np.log(0.31))
+    out = {}
+    for name, prec, rates in (("fp32", pg.FP32_MIXED, args.rates32), ("fp64", pg.FP64, args.rates64)):
+        srv = pg.Server(prm, prec, nlanes=2, hbm_budget=200e9, max_batch_particles=64 << 20)
+        t0 = time.perf_counter()
+        ids = [srv.add_box(pos[off[b]:off[b + 1]], vel[off[b]:off[b + 1]], lengths[b]) for b in range(nbox)]
+        srv.drain()  # the first force of every box
+        log(f"[{name}] {nbox} boxes resident ({int(counts.sum())} particles), first forces in "
+            f"{time.perf_counter() - t0:.1f} s")
+        rows = []
+        rng = np.random.Generator(np.random.PCG64(8))
+        for rate in rates:
+            nreq = max(32, int(rate * args.serve_seconds))
+            boxes = rng.integers(0, nbox, nreq)
+            gaps = rng.exponential(1.0 / rate, nreq)
+            lat, done = [None] * nreq, threading.Event()
+            rids = [None] * nreq
+            submitted = threading.Semaphore(0)
+
+            def waiter():
+                for k in range(nreq):
+                    submitted.acquire()
+                    lat[k] = srv.wait(rids[k])
+                done.set()
+
+            th = threading.Thread(target=waiter)
+            th.start()
+            s0 = srv.stats()
+            t_start = time.perf_counter()
+            t_next = t_start
+            for k in range(nreq):
+                t_next += gaps[k]
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                rids[k] = srv.submit(ids[boxes[k]], dkh, dd)
+                submitted.release()
+            done.wait()
+            th.join()
+            span = time.perf_counter() - t_start
+            s1 = srv.stats()
+            stepped = s1["particles_stepped"] - s0["particles_stepped"]
+            lat_a = np.array(lat)
+            row = {"rate_req_per_s": rate, "requests": nreq, "particles_per_s": stepped / span,
+                   "achieved_req_per_s": nreq / span, "latency_ms": {"p50": float(np.percentile(lat_a, 50)),
+                                                                     "p99": float(np.percentile(lat_a, 99))},
+                   "mean_boxes_per_batch": ((s1["mean_boxes_per_batch"] * s1["batches"] -
+                                             s0["mean_boxes_per_batch"] * s0["batches"]) /
+                                            max(1, s1["batches"] - s0["batches"]))}
+            rows.append(row)
+            log(f"[{name}] rate {rate}/s: {row['particles_per_s'] / 1e6:.1f} M particles/s, "
+                f"latency p50 {row['latency_ms']['p50']:.0f} ms p99 {row['latency_ms']['p99']:.0f} ms, "
+                f"{row['mean_boxes_per_batch']:.1f} boxes per batch")
+        out[name] = rows
+        srv.close()
+    best32 = max(r["particles_per_s"] for r in out["fp32"])
+    best64 = max(r["particles_per_s"] for r in out["fp64"])
+    sat = max(out["fp32"], key=lambda r: r["particles_per_s"])
+    return {
+        "metric": METRIC, "value": best32, "unit": "particles/s", "n_gpus": 1, "steps": None, "warmup": None,
+        "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32-mixed", "data": "synthetic",
+        "config": {"workload": f"serving: {nbox} resident clustered boxes of 16^3..192^3 particles (log-uniform, "
+                               "seed 6), Poisson single-step requests (force + FP64 kick/drift), 2 lanes, "
+                               "HBM budget 200 GB", "particles": int(counts.sum()), "boxes": nbox},
+        "latency_ms": sat["latency_ms"], "sweep": out, "fp64": {"value": best64},
+        "fp32_over_fp64": best32 / best64,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +215,11 @@ def main():
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
+    ap.add_argument("--workload", choices=["clustered", "ensemble", "serving"], default="clustered")
+    ap.add_argument("--serve-boxes", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival span per rate")
+    ap.add_argument("--rates32", type=float, nargs="+", default=[4, 8, 16, 32, 64, 128])
+    ap.add_argument("--rates64", type=float, nargs="+", default=[2, 4, 8, 16])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="first copy MiB mebibytes device to device (a plain copy kernel of known size, "
                          "to scale hardware byte counters)")
@@ -101,6 +239,10 @@ def main():
         torch.cuda.synchronize()
         log(f"copy calibration: {nbytes} bytes read and written once")
         del src, dst
+    if args.workload != "clustered":
+        fn = run_ensemble if args.workload == "ensemble" else run_serving
+        print(json.dumps(fn(args, torch)), flush=True)
+        return
     log("generating particles")
     pos, prm = ic.workload_clustered(nside=args.nside, order=args.order)
     n = len(pos)

@@ -89,6 +89,32 @@ int photons_force_host(photons_ctx *ctx, int64_t n, const double *pos, double *a
 int photons_force_device(photons_ctx *ctx, int64_t n, const double *d_pos, double *d_acc);
 int photons_sync(photons_ctx *ctx);
 
+/* Many independent periodic boxes in one evaluation (no counterpart in the
+ * reference, whose force call covers one domain: src/fmm.c:909-1075 per rank).
+ * Box b holds counts[b] >= 1 particles (more than maxleaf when periodic) in
+ * [0, lengths[b])^3; the boxes' particles are concatenated in pos/acc, box 0
+ * first.  All other parameters (mass, soft, rs, rc, theta, order, leaf size,
+ * precision) are the context's and shared by the boxes.  Each box's result is
+ * that of the box alone: trees, lists and shifts never cross boxes.  One tree
+ * build, walk and P2P/M2L launch covers the whole batch.                      */
+int photons_force_boxes_host(photons_ctx *ctx, int nbox, const int64_t *counts, const double *lengths,
+                             const double *pos, double *acc);
+int photons_force_boxes_device(photons_ctx *ctx, int nbox, const int64_t *counts, const double *lengths,
+                               const double *d_pos, double *d_acc);
+
+/* Tuning (no counterpart in the reference, whose MAXLEAF is a compile-time
+ * constant, inc/photoNs.h).  The FP32-mixed P2P kernel stages 256 or 512 source
+ * particles per batch in LDS (default 512).  photons_autotune_boxes times leaf
+ * sizes 16/32/64 x staging 256/512 (staging only for FP32-mixed) on the
+ * caller's batch, keeps the fastest in the context and leaves its result in
+ * d_acc; ms[6] (or NULL) gets the mean ms per evaluation of each candidate,
+ * leaf-major, -1 where a candidate was skipped (boxes of <= leaf particles). */
+int photons_set_p2p_stage(photons_ctx *ctx, int stage);
+int photons_get_p2p_stage(const photons_ctx *ctx);
+int photons_autotune_boxes(photons_ctx *ctx, int nbox, const int64_t *counts, const double *lengths,
+                           const double *d_pos, double *d_acc, int reps, int *best_leaf, int *best_stage,
+                           double *ms);
+
 /* The same force call in two steps, as the reference splits it:
  *   photons_prepare_host   upload, tree, P2M and M2M   (fmm_prepare)
  *   photons_evaluate_host  walk, M2L, L2L, P2P and L2P (fmm_task + fmm_ext);
@@ -125,6 +151,44 @@ int photons_m2l_batch(photons_ctx *ctx, int order, int count, double rs, const d
 int photons_l2l_batch(photons_ctx *ctx, int order, int count, const double *d_d, const double *d_L, double *d_tL);
 int photons_l2p_batch(photons_ctx *ctx, int order, int n, const double *d_pos, const int32_t *d_owner,
                       const double *d_center, const double *d_L, double *d_acc);
+
+/* ---- serving: resident boxes and continuous batching of step requests ----
+ * No counterpart in the reference (its driver steps one domain in a loop,
+ * src/photoNs.c:136-313).  A step request advances one box by the reference's
+ * loop body without PM (src/photoNs.c:208-268), in FP64 on the GPU:
+ *   v += acc*dkh; x += v*dd; wrap into [0, L); acc = force(x); v += acc*dkh
+ * with dkh = G*kick/2 and dd = drift (photons_kick_factor/_drift_factor over
+ * the step's ln a interval, src/initial.c:650-683).  Boxes stay in HBM between
+ * steps and are admitted while pos/vel/acc (72 B per particle) fit the HBM
+ * budget minus what the lanes reserve for evaluation buffers (about 1 KB per
+ * particle of max_batch_particles, per lane).  Each of nlanes lanes (own
+ * context and stream) takes, whenever it is free, every queued request whose
+ * box is idle (arrival order, one request per box at a time, up to
+ * max_batch_particles) and evaluates them as one batch of boxes; the lanes'
+ * work overlaps on the device.  Thread safe; requests of one box run in order. */
+typedef struct photons_server photons_server;
+typedef struct {
+    int64_t requests_done, particles_stepped, batches, queued;
+    double mean_boxes_per_batch, span_s, resident_bytes, reserved_bytes;
+    int32_t nboxes, pad;
+} photons_server_stats;
+
+const char *photons_server_last_error(void);
+int photons_server_create(photons_server **srv, const photons_params *prm, int device, int nlanes,
+                          double hbm_budget_bytes, int64_t max_batch_particles);
+int photons_server_destroy(photons_server *srv);
+/* pos, vel [n][3] in [0, boxsize); acc the current force or NULL (then the box's
+ * first request computes it, as the reference does before its first step)  */
+int photons_server_add_box(photons_server *srv, int64_t n, double boxsize, const double *pos, const double *vel,
+                           const double *acc, int *box_id);
+int photons_server_remove_box(photons_server *srv, int box_id);
+int photons_server_submit(photons_server *srv, int box_id, double dkh, double dd, int64_t *request_id);
+int photons_server_wait(photons_server *srv, int64_t request_id, double *latency_ms);
+int photons_server_drain(photons_server *srv);
+int photons_server_read_box(photons_server *srv, int box_id, double *pos, double *vel, double *acc);
+int photons_server_get_stats(photons_server *srv, photons_server_stats *st);
+double photons_kick_factor(double loga_i, double loga_f, double omega_m, double omega_x);
+double photons_drift_factor(double loga_i, double loga_f, double omega_m, double omega_x);
 
 /* ======================================================================= */
 /* 2. drop-in layer: the reference's entry points                            */

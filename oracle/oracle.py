@@ -74,6 +74,12 @@ def lib():
         L.orc_acceptance.argtypes = [dp, dp, dp, C.c_double, C.c_double]
         L.orc_p2p_pair.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, dp]
         L.orc_direct.restype = C.c_int
+        L.orc_kick_loga.restype = C.c_double
+        L.orc_kick_loga.argtypes = [C.c_double] * 4
+        L.orc_drift_loga.restype = C.c_double
+        L.orc_drift_loga.argtypes = [C.c_double] * 4
+        L.orc_step.restype = C.c_int
+        L.orc_step.argtypes = [C.POINTER(OrcParams), C.c_int, dp, dp, dp, C.c_double, C.c_double]
         L.orc_direct.argtypes = [C.POINTER(OrcParams), C.c_int, dp, C.c_int, ip, C.c_double, dp, C.c_int]
         _lib = L
     return _lib
@@ -110,6 +116,27 @@ def force(pos, prm):
     if rc != 0:
         raise RuntimeError(f"orc_force failed ({rc})")
     return acc, st.as_dict()
+
+
+def kick_loga(loga_i, loga_f, omega_m=0.25, omega_x=0.75):
+    """int dln a / (a H) by 128-block Simpson (src/initial.c:650-666)"""
+    return lib().orc_kick_loga(loga_i, loga_f, omega_m, omega_x)
+
+
+def drift_loga(loga_i, loga_f, omega_m=0.25, omega_x=0.75):
+    """int dln a / (a^2 H) by 128-block Simpson (src/initial.c:668-683)"""
+    return lib().orc_drift_loga(loga_i, loga_f, omega_m, omega_x)
+
+
+def step(pos, vel, acc, prm, dkh, dd):
+    """one reference step (src/photoNs.c:208-268, PM omitted): returns new (pos, vel, acc)"""
+    pos = np.array(pos, dtype=np.float64, order="C")
+    vel = np.array(vel, dtype=np.float64, order="C")
+    acc = np.array(acc, dtype=np.float64, order="C")
+    rc = lib().orc_step(C.byref(params(prm)), pos.shape[0], _dp(pos), _dp(vel), _dp(acc), dkh, dd)
+    if rc != 0:
+        raise RuntimeError(f"orc_step failed ({rc})")
+    return pos, vel, acc
 
 
 class Tree:

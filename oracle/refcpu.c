@@ -979,3 +979,56 @@ int orc_direct(const orc_params *prm, int n, const double *pos, int nt, const in
     free(cnt); free(cidx); free(cof); free(fill);
     return 0;
 }
+
+/* ---------------------------------------------------------------------------- time integration */
+
+/* 128-block composite Simpson in ln a of a^-k / H(a), H(a) = 0.1 sqrt(Om a^-3 + OL),
+ * k = 1 for the kick and 2 for the drift (src/initial.c:650-683: weights 1, 4, 2,
+ * ..., 4, 1 over 3, the integrand at ln a_i + n dloga, the end points from exp). */
+static double simpson_loga(double loga_i, double loga_f, double om, double ox, int k)
+{
+    const int nblock = 128;
+    const double dloga = (loga_f - loga_i) / nblock;
+    double z = 1.0 / exp(loga_i);
+    double h = 0.1 * sqrt(om * z * z * z + ox);
+    double zk = k == 1 ? z : z * z;
+    double sum = dloga * zk / h;
+    for (int j = 1; j < nblock; j++) {
+        z = 1.0 / exp(loga_i + dloga * j);
+        h = 0.1 * sqrt(om * z * z * z + ox);
+        zk = k == 1 ? z : z * z;
+        sum += 2.0 * (1 + j % 2) * dloga * zk / h;
+    }
+    z = 1.0 / exp(loga_f);
+    h = 0.1 * sqrt(om * z * z * z + ox);
+    zk = k == 1 ? z : z * z;
+    sum += dloga * zk / h;
+    return sum / 3.0;
+}
+
+double orc_kick_loga(double loga_i, double loga_f, double omega_m, double omega_x)
+{
+    return simpson_loga(loga_i, loga_f, omega_m, omega_x, 1);
+}
+
+double orc_drift_loga(double loga_i, double loga_f, double omega_m, double omega_x)
+{
+    return simpson_loga(loga_i, loga_f, omega_m, omega_x, 2);
+}
+
+/* src/photoNs.c:208-268 without the PM terms: half kick with the previous force,
+ * drift, wrap (the reference's while loops), new force, half kick */
+int orc_step(const orc_params *prm, int n, double *pos, double *vel, double *acc, double dkh, double dd)
+{
+    const double L = prm->boxsize;
+    for (int i = 0; i < 3 * n; i++) vel[i] += acc[i] * dkh;
+    for (int i = 0; i < 3 * n; i++) pos[i] += vel[i] * dd;
+    for (int i = 0; i < 3 * n; i++) {
+        while (pos[i] < 0.0) pos[i] += L;
+        while (pos[i] >= L) pos[i] -= L;
+    }
+    int rc = orc_force(prm, n, pos, acc, NULL);
+    if (rc) return rc;
+    for (int i = 0; i < 3 * n; i++) vel[i] += acc[i] * dkh;
+    return 0;
+}

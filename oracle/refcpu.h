@@ -103,6 +103,18 @@ void orc_p2p_pair(const double *xt, const double *xs, double mass, double soft, 
 int orc_direct(const orc_params *prm, int n, const double *pos, int nt, const int32_t *targets,
                double rmax, double *acc, int nthreads);
 
+/* ---- time integration (the reference's step, src/photoNs.c:136-268) ----- */
+/* Kick and drift factors over [ln a_i, ln a_f] by 128-block Simpson in ln a,
+ * H(a) = 0.1 sqrt(Om a^-3 + OL) (src/initial.c:650-683):
+ *   kick  = int dln a / (a H),   drift = int dln a / (a^2 H).             */
+double orc_kick_loga(double loga_i, double loga_f, double omega_m, double omega_x);
+double orc_drift_loga(double loga_i, double loga_f, double omega_m, double omega_x);
+/* One step of n particles of one periodic box, in the reference's order:
+ * v += acc*dkh; x += v*dd; wrap into [0, L); acc = short-range force(x);
+ * v += acc*dkh (dkh = kick/2 * G; the PM force is not part of this path).
+ * acc holds the previous force on entry and the new one on return.        */
+int orc_step(const orc_params *prm, int n, double *pos, double *vel, double *acc, double dkh, double dd);
+
 #ifdef __cplusplus
 }
 #endif

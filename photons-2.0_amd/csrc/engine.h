@@ -9,6 +9,10 @@
 //              rs (FP32-mixed path), one row per cell
 //   lists      P2P: CSR over target leaves; M2L: CSR over target cells; an entry is
 //              source cell | shift id << 27 (shift id 0 = home box, 1..26 images)
+//   boxes      a context holds a forest: nbox independent periodic boxes, each with
+//              its own length, whose particles are contiguous (box b owns caller
+//              slots box_off[b] .. box_off[b+1]); every cell belongs to one box
+//              (cell_box) and pairs never cross boxes
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,6 +54,13 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// per-box data the kernels read: box length by cell
+struct BoxView {
+    const double *len;    // [nbox]
+    const int *cell_box;  // [ncell]
+    __device__ __forceinline__ double len_of(int cell) const { return len[cell_box[cell]]; }
+};
+
 struct Timing {
     bool enabled = false;
     hipEvent_t ev[16];
@@ -64,6 +75,13 @@ struct Ctx {
 
     int64_t n = 0;
     int nleaf = 0, nnode = 0, ncell = 0, nlevel = 0, maxdepth = 0;
+    int nbox = 1;
+    int p2p_stage = 512;           // FP32 P2P staging slots (256 or 512; tuned)
+    std::vector<int64_t> box_off;  // particle offsets of the boxes (nbox + 1)
+    std::vector<double> box_len;   // box lengths
+    DevBuf d_box_off, d_box_len;   // device copies
+    DevBuf d_box_root;             // root cell id of each box (int)
+    DevBuf cell_box;               // box of each cell (int)
     std::vector<int> level_off;  // nodes of depth d: level_nodes[level_off[d] .. level_off[d+1])
 
     // persistent device buffers, grown on demand
@@ -117,6 +135,9 @@ T *dptr(DevBuf &b)
 {
     return reinterpret_cast<T *>(b.p);
 }
+inline BoxView box_view(Ctx &c) { return BoxView{dptr<double>(c.d_box_len), dptr<int>(c.cell_box)}; }
+// sets the boxes of the next evaluation (counts and lengths; uploads them)
+void set_boxes(Ctx &c, int nbox, const int64_t *counts, const double *lengths);
 
 // phases (tree.hip, walk.hip, kernels.hip)
 void build_tree(Ctx &c);                 // px.. sorted, cells and levels built

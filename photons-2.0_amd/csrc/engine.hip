@@ -10,6 +10,7 @@
 //   p2p       P2P + L2P, one wavefront per target leaf, result in caller order
 // The tree build and the walk read a few counts back to size the next step.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
@@ -51,6 +52,18 @@ static void release(DevBuf &b)
     if (b.p) (void)hipFree(b.p);
     b.p = nullptr;
     b.cap = 0;
+}
+
+void set_boxes(Ctx &c, int nbox, const int64_t *counts, const double *lengths)
+{
+    c.nbox = nbox;
+    c.box_off.assign(nbox + 1, 0);
+    c.box_len.assign(lengths, lengths + nbox);
+    for (int b = 0; b < nbox; b++) c.box_off[b + 1] = c.box_off[b] + counts[b];
+    void *o = grow(c.d_box_off, sizeof(int64_t) * (size_t)(nbox + 1));
+    void *l = grow(c.d_box_len, sizeof(double) * (size_t)nbox);
+    PH_CHECK(hipMemcpyAsync(o, c.box_off.data(), sizeof(int64_t) * (nbox + 1), hipMemcpyHostToDevice, c.stream));
+    PH_CHECK(hipMemcpyAsync(l, c.box_len.data(), sizeof(double) * nbox, hipMemcpyHostToDevice, c.stream));
 }
 
 void phase_mark(Ctx &c)
@@ -143,7 +156,8 @@ int photons_destroy(photons_ctx *x)
                           &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
                           &c.seg_flag, &c.seg_nn, &c.seg_nl, &c.seg_non, &c.seg_lon, &c.t_size, &c.t_lsize,
                           &c.t_dfs, &c.t_lfirst, &c.t_final, &c.l_final, &c.err, &c.stats, &c.bfs_a, &c.bfs_b,
-                          &c.p2p_keys, &c.m2l_keys, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part,
+                          &c.p2p_keys, &c.m2l_keys, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off,
+                          &c.d_box_len, &c.d_box_root, &c.cell_box,
                           &x->io_pos, &x->io_acc, &x->op_a, &x->op_b, &x->op_c, &x->op_d, &x->op_e};
         for (DevBuf *b : bufs) release(*b);
         if (c.pinned.p) (void)hipHostFree(c.pinned.p);
@@ -245,6 +259,23 @@ static std::string check_n(const Ctx &c, int64_t n)
     return "";
 }
 
+static std::string check_boxes(const Ctx &c, int nbox, const int64_t *counts, const double *lengths)
+{
+    if (nbox < 1 || !counts || !lengths) return "need at least one box";
+    int64_t total = 0;
+    for (int b = 0; b < nbox; b++) {
+        if (counts[b] < 1) return "box " + std::to_string(b) + " holds no particles";
+        if (!(lengths[b] > 0.0)) return "box " + std::to_string(b) + " has a non-positive length";
+        std::string m = check_n(c, counts[b]);
+        if (!m.empty()) return "box " + std::to_string(b) + ": " + m;
+        total += counts[b];
+    }
+    return check_n(c, total);
+}
+
+// one box of the context's own length
+static void single_box(Ctx &c, int64_t n) { set_boxes(c, 1, &n, &c.prm.boxsize); }
+
 }  // namespace photons
 
 extern "C" {
@@ -261,12 +292,122 @@ int photons_force_device(photons_ctx *x, int64_t n, const double *d_pos, double 
             c.nleaf = c.nnode = c.ncell = 0;
             return 0;
         }
+        single_box(c, n);
         prepare(c, d_pos);
         evaluate(c, d_acc, true);
         collect_timing(c, false);
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_force_device", e);
+    }
+}
+
+int photons_force_boxes_device(photons_ctx *x, int nbox, const int64_t *counts, const double *lengths,
+                               const double *d_pos, double *d_acc)
+{
+    try {
+        Ctx &c = x->c;
+        std::string m = check_boxes(c, nbox, counts, lengths);
+        if (!m.empty()) return fail_msg(m);
+        PH_CHECK(hipSetDevice(c.device));
+        set_boxes(c, nbox, counts, lengths);
+        c.n = c.box_off[nbox];
+        prepare(c, d_pos);
+        evaluate(c, d_acc, true);
+        collect_timing(c, false);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_force_boxes_device", e);
+    }
+}
+
+int photons_set_p2p_stage(photons_ctx *x, int stage)
+{
+    if (stage != 256 && stage != 512) return fail_msg("the P2P stage must hold 256 or 512 particles");
+    x->c.p2p_stage = stage;
+    return 0;
+}
+
+int photons_get_p2p_stage(const photons_ctx *x) { return x->c.p2p_stage; }
+
+// Leaf size (16, 32, 64) and, for FP32-mixed, the P2P staging size (256, 512) timed
+// on the caller's own batch: each candidate runs reps + 1 evaluations (the first
+// untimed), wall clock from queueing to the end of the last one; the fastest is
+// kept in the context and d_acc holds its result.
+int photons_autotune_boxes(photons_ctx *x, int nbox, const int64_t *counts, const double *lengths,
+                           const double *d_pos, double *d_acc, int reps, int *best_leaf, int *best_stage,
+                           double *ms /* [6] or NULL: leaf-major, stage 256 then 512 */)
+{
+    try {
+        Ctx &c = x->c;
+        if (reps < 1) return fail_msg("reps must be positive");
+        const photons_params keep = c.prm;
+        const int leaves[3] = {16, 32, 64}, stages[2] = {256, 512};
+        const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+        double best = 1e300;
+        int bl = keep.maxleaf, bs = c.p2p_stage;
+        for (int i = 0; i < 3; i++)
+            for (int j = 0; j < 2; j++) {
+                if (ms) ms[2 * i + j] = -1.0;
+                if (!f32 && j == 0) continue;  // the staging size is an FP32 choice
+                c.prm.maxleaf = leaves[i];
+                c.p2p_stage = stages[j];
+                if (!check_boxes(c, nbox, counts, lengths).empty()) continue;  // boxes too small for this leaf
+                int rc = photons_force_boxes_device(x, nbox, counts, lengths, d_pos, d_acc);
+                if (rc == 0) rc = photons_sync(x);
+                if (rc != 0) {
+                    c.prm = keep;
+                    return rc;
+                }
+                const auto t0 = std::chrono::steady_clock::now();
+                for (int r = 0; r < reps && rc == 0; r++) rc = photons_force_boxes_device(x, nbox, counts, lengths, d_pos, d_acc);
+                if (rc == 0) rc = photons_sync(x);
+                if (rc != 0) {
+                    c.prm = keep;
+                    return rc;
+                }
+                const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;
+                if (ms) ms[2 * i + j] = t;
+                if (t < best) {
+                    best = t;
+                    bl = leaves[i];
+                    bs = stages[j];
+                }
+            }
+        c.prm = keep;
+        c.prm.maxleaf = bl;
+        c.p2p_stage = bs;
+        int rc = photons_force_boxes_device(x, nbox, counts, lengths, d_pos, d_acc);
+        if (rc == 0) rc = photons_sync(x);
+        if (best_leaf) *best_leaf = bl;
+        if (best_stage) *best_stage = bs;
+        return rc;
+    } catch (const std::exception &e) {
+        return fail("photons_autotune_boxes", e);
+    }
+}
+
+int photons_force_boxes_host(photons_ctx *x, int nbox, const int64_t *counts, const double *lengths,
+                             const double *pos, double *acc)
+{
+    try {
+        Ctx &c = x->c;
+        std::string m = check_boxes(c, nbox, counts, lengths);
+        if (!m.empty()) return fail_msg(m);
+        PH_CHECK(hipSetDevice(c.device));
+        set_boxes(c, nbox, counts, lengths);
+        const int64_t n = c.n = c.box_off[nbox];
+        double *dp = reinterpret_cast<double *>(grow(x->io_pos, sizeof(double) * 3 * (size_t)n));
+        double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)n));
+        PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
+        prepare(c, dp);
+        evaluate(c, da, true);
+        PH_CHECK(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        collect_timing(c, true);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_force_boxes_host", e);
     }
 }
 
@@ -279,6 +420,7 @@ int photons_force_host(photons_ctx *x, int64_t n, const double *pos, double *acc
         PH_CHECK(hipSetDevice(c.device));
         c.n = n;
         if (n == 0) return 0;
+        single_box(c, n);
         double *dp = reinterpret_cast<double *>(grow(x->io_pos, sizeof(double) * 3 * (size_t)n));
         double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)n));
         PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
@@ -304,6 +446,7 @@ int photons_prepare_host(photons_ctx *x, int64_t n, const double *pos)
         PH_CHECK(hipSetDevice(c.device));
         c.n = n;
         if (n == 0) return 0;
+        single_box(c, n);
         double *dp = reinterpret_cast<double *>(grow(x->io_pos, sizeof(double) * 3 * (size_t)n));
         PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
         prepare(c, dp);

@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(128) k_m2l_items(int nitems, const int *__rest
                                                    const int *__restrict__ item_off, const CellGeom *__restrict__ geom,
                                                    const int64_t *__restrict__ off, const int32_t *__restrict__ src,
                                                    const T *__restrict__ M, double scale, double inv2rs,
-                                                   double boxsize, T *__restrict__ part)
+                                                   BoxView bx, T *__restrict__ part)
 {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
     if (it >= nitems) return;
@@ -133,11 +133,12 @@ __global__ void __launch_bounds__(128) k_m2l_items(int nitems, const int *__rest
     T L[NT];
     static_for<0, NT>([&](auto k) { L[k.value] = T(0); });
     const CellGeom gX = geom[X];
+    const double boxlen = bx.len_of(X);
     for (int64_t e = e0; e < e1; e++) {
         const int32_t s = src[e];
         const int B = entry_cell(s), sid = entry_shift(s);
         double sx, sy, sz;
-        shift_xyz(sid, boxsize, sx, sy, sz);
+        shift_xyz(sid, boxlen, sx, sy, sz);
         const CellGeom gB = geom[B];
         // relative coordinates formed in double (then scaled to units of rs for float)
         const T rx = (T)((gX.c[0] - (gB.c[0] + sx)) * scale);
@@ -231,7 +232,8 @@ struct P2PArgs {
     const float *ox, *oy, *oz;
     const int32_t *pidx;
     const void *L;  // leaf expansions (double or float)
-    double mass, soft, inv2rs, inv_rs, boxsize;
+    double mass, soft, inv2rs, inv_rs;
+    BoxView bx;
     int nleaf;
     int with_l2p;
     int accumulate;       // out += instead of out =
@@ -279,13 +281,14 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     const int it = IT.ipart + t;
     const double xt = a.px[it], yt = a.py[it], zt = a.pz[it];
     const double m = a.mass, eps = a.soft, meps3 = m / (eps * eps * eps);
+    const double boxlen = a.bx.len_of(T);
     double ax = 0.0, ay = 0.0, az = 0.0;
     for (int64_t e = a.off[T]; e < a.off[T + 1]; e++) {
         const int32_t s = a.src[e];
         const int S = entry_cell(s), sid = entry_shift(s);
         const CellInfo IS = a.info[S];
         double shx, shy, shz;
-        shift_vec(sid, a.boxsize, shx, shy, shz);
+        shift_vec(sid, boxlen, shx, shy, shz);
         if (lane < IS.npart) {
             const int j = IS.ipart + lane;
             sx_[lane] = a.px[j] + shx;  // image body = pos + shift (src/remotes.c:84-86)
@@ -361,8 +364,8 @@ __device__ __forceinline__ void pair_f32(const PairConsts &k, float dx, float dy
 }
 
 // Source particles staged in LDS per batch, as float offsets from the target leaf's
-// centre (3 x 4 bytes each).
-constexpr int kP2PStage = 512;
+// centre (3 x 4 bytes each): STAGE = 256 or 512 slots (a tuning choice; 512 is the
+// default), at least 128 so the final 6 x 64 partial sums fit the same array.
 
 // Lane layout for a target leaf of n particles (1 <= n <= 64); a = 64 / n, e = 64 - a n.
 //   pass A: lane l serves target l % n and the sources q = l / n + i sA of the staged
@@ -398,14 +401,16 @@ __device__ __forceinline__ LaneMap lane_map(int n, int lane)
 // One wavefront per target leaf.  The source list is consumed in batches: 64 list
 // entries are read at once, one per lane (entry -> cell -> centre offset, formed in
 // double), a wave prefix sum of their particle counts picks the leading entries that
-// fit kP2PStage particles, and their particles are staged in LDS as float offsets
+// fit STAGE particles, and their particles are staged in LDS as float offsets
 // from the target centre, all loads of a batch issued together.  The pairs of the
 // batch are then spread over the lanes by the two-pass layout above.  Partial sums
 // are combined per target in a fixed lane order, so results are deterministic.
-template <int P>
+template <int P, int STAGE>
 __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
 {
-    __shared__ float sx[kP2PStage], sy[kP2PStage], sz[kP2PStage];
+    static_assert(STAGE >= 128 && STAGE % 64 == 0, "stage size");
+    __shared__ float stage[3 * STAGE];
+    float *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ float ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
     const int T = blockIdx.x;
@@ -417,6 +422,7 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
     const float txA = a.ox[IT.ipart + L.tA], tyA = a.oy[IT.ipart + L.tA], tzA = a.oz[IT.ipart + L.tA];
     const float txB = a.ox[IT.ipart + L.tB], tyB = a.oy[IT.ipart + L.tB], tzB = a.oz[IT.ipart + L.tB];
     const CellGeom gT = a.geom[T];
+    const double boxlen = a.bx.len_of(T);
     PairConsts pc;
     pc.ieps = (float)(1.0 / a.soft);
     pc.inv4rs = (float)(0.5 * a.inv2rs);
@@ -433,7 +439,7 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
             const CellInfo IS = a.info[S];
             const CellGeom gS = a.geom[S];
             double shx, shy, shz;
-            shift_vec(sid, a.boxsize, shx, shy, shz);
+            shift_vec(sid, boxlen, shx, shy, shz);
             ecx[lane] = (float)((gS.c[0] + shx) - gT.c[0]);
             ecy[lane] = (float)((gS.c[1] + shy) - gT.c[1]);
             ecz[lane] = (float)((gS.c[2] + shz) - gT.c[2]);
@@ -446,14 +452,14 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
             const int v = __shfl_up(incl, d, 64);
             if (lane >= d) incl += v;
         }
-        // the counts are at most 64 <= kP2PStage, so at least one entry fits
-        const int ne = __popcll(__ballot(lane < nb && incl <= kP2PStage));
+        // the counts are at most 64 <= STAGE, so at least one entry fits
+        const int ne = __popcll(__ballot(lane < nb && incl <= STAGE));
         const int tot = __shfl(incl, ne - 1, 64);
         ecnt[lane] = cnt;
         eincl[lane] = lane < ne ? incl : 0x7fffffff;
         __syncthreads();
         // stage: slot p belongs to the first entry whose inclusive count exceeds p
-        constexpr int R = kP2PStage / 64;
+        constexpr int R = STAGE / 64;
         float vx[R], vy[R], vz[R];
         int js[R];
 #pragma unroll
@@ -504,7 +510,7 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
         __syncthreads();  // the staging arrays are rewritten by the next batch
     }
     // per-target sums in a fixed order: pass A lanes t + n k, then pass B lanes
-    float *red = sx;  // 6 x 64 floats, the staging array is free now
+    float *red = stage;  // 6 x 64 floats, the staging array is free now
     red[lane] = axA;
     red[64 + lane] = ayA;
     red[128 + lane] = azA;
@@ -621,6 +627,8 @@ void m2l_pass(Ctx &c, bool accumulate)
     need(c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_m2l), "M2L list");
     void *L = c.L.p;
     hipStream_t st = c.stream;
+    need(c.cell_box, sizeof(int) * (size_t)c.ncell, "cell boxes");
+    const BoxView bx = box_view(c);
     const CellGeom *geom = dptr<CellGeom>(c.geom);
     const int64_t *off = dptr<int64_t>(c.m2l_off);
     const int32_t *src = dptr<int32_t>(c.m2l_src);
@@ -642,14 +650,14 @@ void m2l_pass(Ctx &c, bool accumulate)
         if (n_items)                                                                                               \
             k_m2l_items<PP, float><<<nblk(n_items, 128), 128, 0, st>>>(                                            \
                 n_items, item_tgt, item_off, geom, off, src, reinterpret_cast<const float *>(M + (size_t)c.ncell * NT), \
-                1.0 / c.prm.rs, 0.5, c.prm.boxsize, reinterpret_cast<float *>(part));                              \
+                1.0 / c.prm.rs, 0.5, bx, reinterpret_cast<float *>(part));                                \
         k_m2l_reduce<PP, float><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, item_off,                           \
                                                                     reinterpret_cast<const float *>(part),         \
                                                                     reinterpret_cast<float *>(L), accumulate);     \
     } else {                                                                                                       \
         if (n_items)                                                                                               \
             k_m2l_items<PP, double><<<nblk(n_items, 128), 128, 0, st>>>(n_items, item_tgt, item_off, geom, off, src, \
-                                                                        M, 1.0, 0.5 / c.prm.rs, c.prm.boxsize,     \
+                                                                        M, 1.0, 0.5 / c.prm.rs, bx,                \
                                                                         reinterpret_cast<double *>(part));         \
         k_m2l_reduce<PP, double><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, item_off,                          \
                                                                      reinterpret_cast<const double *>(part),       \
@@ -690,6 +698,7 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     need(c.p2p_off, sizeof(int64_t) * (size_t)(c.nleaf + 1), "P2P list");
     need(c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_p2p), "P2P list");
     need(c.geom, sizeof(CellGeom) * (size_t)c.ncell, "cells");
+    need(c.cell_box, sizeof(int) * (size_t)c.ncell, "cell boxes");
     need(c.px, sizeof(double) * (size_t)c.n, "positions");
     if (f32p) need(c.ox, sizeof(float) * (size_t)c.n, "float offsets");
     if (with_l2p) need(c.L, (f32p ? sizeof(float) : sizeof(double)) * (size_t)c.ncell * nterms(c.prm.order),
@@ -711,7 +720,7 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     a.soft = c.prm.soft;
     a.inv2rs = 0.5 / c.prm.rs;
     a.inv_rs = 1.0 / c.prm.rs;
-    a.boxsize = c.prm.boxsize;
+    a.bx = box_view(c);
     a.nleaf = c.nleaf;
     a.with_l2p = with_l2p;
     a.accumulate = accumulate;
@@ -720,7 +729,8 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
     hipStream_t st = c.stream;
 #define P2P_L(PP)                                              \
-    if (f32) k_p2p_f32<PP><<<c.nleaf, 64, 0, st>>>(a);         \
+    if (f32 && c.p2p_stage == 256) k_p2p_f32<PP, 256><<<c.nleaf, 64, 0, st>>>(a); \
+    else if (f32) k_p2p_f32<PP, 512><<<c.nleaf, 64, 0, st>>>(a);          \
     else k_p2p_f64<PP><<<c.nleaf, 64, 0, st>>>(a);
     if (c.nleaf > 0) PH_DISPATCH(c.prm.order, P2P_L)
 #undef P2P_L

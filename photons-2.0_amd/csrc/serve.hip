@@ -1,0 +1,511 @@
+// serve.hip -- a request loop over resident boxes: continuous batching of
+// single-step (force + kick/drift) requests on one GPU.
+//
+// Each box keeps pos, vel and the last force (acc) in HBM, caller order, FP64.  A
+// step is the reference's loop body without PM (src/photoNs.c:208-268): half kick
+// with the previous force, drift, wrap into [0, L), new short-range force, half
+// kick.  Requests queue in arrival order.  Each of nlanes worker lanes (its own
+// force context and HIP stream) takes, whenever it is free, every queued request
+// whose box is idle, up to its particle capacity, and evaluates them as one forest
+// (photons_force_boxes_device), so requests that arrive while others run are taken
+// as soon as a lane frees up, and the lanes' work overlaps on the device.  Boxes
+// are admitted while the resident bytes stay within the HBM budget minus what the
+// lanes reserve for their evaluation buffers.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/photons_fmm.h"
+
+namespace {
+
+#define SV_CHECK(x)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (x);                                                          \
+        if (e_ != hipSuccess) throw std::runtime_error(std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// evaluation buffers per particle of a batch: tree, expansions, lists (~37 entries
+// per particle at the 256^3 density, 8 bytes each plus sort buffers) and the
+// batch's own pos/acc copies, rounded up
+constexpr double kTransientBytesPerParticle = 1024.0;
+constexpr double kResidentBytesPerParticle = 72.0;  // pos, vel, acc (3 doubles each)
+
+double now_s()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+// reference wrap (src/photoNs.c:244-260); far outliers are folded first so the
+// loops stay short
+__device__ __forceinline__ double wrap(double x, double L)
+{
+    if (!(fabs(x) < 4.0 * L)) x -= floor(x / L) * L;
+    while (x < 0.0) x += L;
+    while (x >= L) x -= L;
+    return x;
+}
+
+// first half kick with the previous force, drift, wrap; writes the new position to
+// the box and to the batch input
+__global__ void k_kick_drift(int64_t n3, double *__restrict__ pos, double *__restrict__ vel,
+                             const double *__restrict__ acc, double dkh, double dd, double L, int step,
+                             double *__restrict__ out)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    double x = pos[i];
+    if (step) {
+        const double v = vel[i] + acc[i] * dkh;
+        vel[i] = v;
+        x = wrap(x + v * dd, L);
+        pos[i] = x;
+    }
+    out[i] = x;
+}
+
+// the new force becomes the box's acc; second half kick
+__global__ void k_kick(int64_t n3, double *__restrict__ vel, double *__restrict__ acc, const double *__restrict__ in,
+                       double dkh, int step)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n3) return;
+    const double a = in[i];
+    acc[i] = a;
+    if (step) vel[i] += a * dkh;
+}
+
+struct Box {
+    int64_t n = 0;
+    double L = 0.0;
+    double *pos = nullptr, *vel = nullptr, *acc = nullptr;
+    bool busy = false;      // a request of this box is being evaluated
+    bool removed = false;
+};
+
+enum Kind { kStep = 0, kForce = 1 };
+
+struct Request {
+    int box = -1;
+    int kind = kStep;
+    double dkh = 0.0, dd = 0.0;
+    double t_submit = 0.0, t_done = 0.0;
+    bool done = false;
+    std::string error;
+};
+
+struct Lane {
+    photons_ctx *ctx = nullptr;
+    hipStream_t stream = nullptr;
+    double *in = nullptr, *out = nullptr;  // batch pos / acc, [cap][3]
+    int64_t cap = 0;
+    std::thread th;
+};
+
+}  // namespace
+
+struct photons_server {
+    photons_params prm;
+    int device = 0;
+    double budget = 0.0, reserve = 0.0, resident = 0.0, peak = 0.0;
+    int64_t lane_cap = 0;  // particles per batch
+    int max_boxes_per_batch = 256;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<int64_t> queue;  // request ids, arrival order
+    std::map<int64_t, Request> reqs;
+    std::map<int, Box> boxes;
+    int next_box = 0;
+    int64_t next_req = 0;
+    bool stop = false;
+    std::vector<Lane> lanes;
+    // statistics
+    int64_t n_done = 0, particles_stepped = 0, batches = 0, batch_boxes = 0;
+    double t_first = -1.0, t_last = 0.0;
+    std::string fatal;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(const std::string &m)
+{
+    g_err = m;
+    return -1;
+}
+
+// one batch on one lane: requests `ids` (their boxes marked busy by the caller)
+void run_batch(photons_server *s, Lane &ln, const std::vector<int64_t> &ids)
+{
+    std::vector<int64_t> counts;
+    std::vector<double> lengths;
+    std::vector<Request> rq;
+    std::vector<Box> bx;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        for (int64_t id : ids) {
+            rq.push_back(s->reqs[id]);
+            bx.push_back(s->boxes[rq.back().box]);
+        }
+    }
+    int64_t off = 0;
+    for (size_t k = 0; k < ids.size(); k++) {
+        const int64_t n3 = 3 * bx[k].n;
+        const int step = rq[k].kind == kStep;
+        k_kick_drift<<<(int)((n3 + 255) / 256), 256, 0, ln.stream>>>(n3, bx[k].pos, bx[k].vel, bx[k].acc, rq[k].dkh,
+                                                                    rq[k].dd, bx[k].L, step, ln.in + 3 * off);
+        counts.push_back(bx[k].n);
+        lengths.push_back(bx[k].L);
+        off += bx[k].n;
+    }
+    SV_CHECK(hipGetLastError());
+    if (photons_force_boxes_device(ln.ctx, (int)counts.size(), counts.data(), lengths.data(), ln.in, ln.out) != 0)
+        throw std::runtime_error(std::string("force: ") + photons_last_error());
+    off = 0;
+    for (size_t k = 0; k < ids.size(); k++) {
+        const int64_t n3 = 3 * bx[k].n;
+        k_kick<<<(int)((n3 + 255) / 256), 256, 0, ln.stream>>>(n3, bx[k].vel, bx[k].acc, ln.out + 3 * off,
+                                                              rq[k].dkh, rq[k].kind == kStep);
+        off += bx[k].n;
+    }
+    SV_CHECK(hipGetLastError());
+    SV_CHECK(hipStreamSynchronize(ln.stream));
+}
+
+void lane_loop(photons_server *s, int lane_id)
+{
+    Lane &ln = s->lanes[lane_id];
+    (void)hipSetDevice(s->device);
+    for (;;) {
+        std::vector<int64_t> ids;
+        {
+            std::unique_lock<std::mutex> g(s->mu);
+            for (;;) {
+                if (s->stop) return;
+                // every queued request whose box is idle, in arrival order, within capacity
+                int64_t total = 0;
+                std::vector<int> taken;
+                for (auto it = s->queue.begin(); it != s->queue.end();) {
+                    Request &r = s->reqs[*it];
+                    Box &b = s->boxes[r.box];
+                    bool box_taken = false;
+                    for (int t : taken) box_taken |= (t == r.box);
+                    if (b.busy || box_taken || (int)ids.size() >= s->max_boxes_per_batch ||
+                        (total > 0 && total + b.n > ln.cap)) {
+                        ++it;
+                        continue;
+                    }
+                    ids.push_back(*it);
+                    taken.push_back(r.box);
+                    total += b.n;
+                    it = s->queue.erase(it);
+                }
+                if (!ids.empty()) {
+                    for (int t : taken) s->boxes[t].busy = true;
+                    break;
+                }
+                s->cv_work.wait(g);
+            }
+        }
+        std::string err;
+        try {
+            run_batch(s, ln, ids);
+        } catch (const std::exception &e) {
+            err = e.what();
+        }
+        const double t = now_s();
+        {
+            std::lock_guard<std::mutex> g(s->mu);
+            for (int64_t id : ids) {
+                Request &r = s->reqs[id];
+                Box &b = s->boxes[r.box];
+                b.busy = false;
+                r.done = true;
+                r.t_done = t;
+                r.error = err;
+                if (err.empty()) {
+                    s->n_done++;
+                    if (r.kind == kStep) s->particles_stepped += b.n;
+                }
+            }
+            s->batches++;
+            s->batch_boxes += (int64_t)ids.size();
+            s->t_last = t;
+            if (!err.empty() && s->fatal.empty()) s->fatal = err;
+        }
+        s->cv_done.notify_all();
+        s->cv_work.notify_all();  // boxes freed: other lanes may take their next requests
+    }
+}
+
+int64_t enqueue(photons_server *s, int box, int kind, double dkh, double dd)
+{
+    Request r;
+    r.box = box;
+    r.kind = kind;
+    r.dkh = dkh;
+    r.dd = dd;
+    r.t_submit = now_s();
+    const int64_t id = s->next_req++;
+    s->reqs[id] = r;
+    s->queue.push_back(id);
+    if (s->t_first < 0.0) s->t_first = r.t_submit;
+    return id;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *photons_server_last_error(void) { return g_err.c_str(); }
+
+int photons_server_create(photons_server **out, const photons_params *prm, int device, int nlanes,
+                          double hbm_budget_bytes, int64_t max_batch_particles)
+{
+    try {
+        if (!out || !prm) return fail("null argument");
+        if (nlanes < 1 || nlanes > 8) return fail("nlanes must be in [1, 8]");
+        if (!(hbm_budget_bytes > 0.0)) return fail("the HBM budget must be positive");
+        if (max_batch_particles < 1) return fail("max_batch_particles must be positive");
+        SV_CHECK(hipSetDevice(device));
+        size_t free_b = 0, total_b = 0;
+        SV_CHECK(hipMemGetInfo(&free_b, &total_b));
+        if (hbm_budget_bytes > (double)free_b) return fail("the HBM budget exceeds the free device memory");
+        photons_server *s = new photons_server();
+        s->prm = *prm;
+        s->device = device;
+        s->budget = hbm_budget_bytes;
+        s->lane_cap = max_batch_particles;
+        s->reserve = nlanes * (double)max_batch_particles * kTransientBytesPerParticle;
+        if (s->reserve >= s->budget) {
+            delete s;
+            return fail("the lanes' evaluation buffers alone exceed the HBM budget");
+        }
+        s->lanes.resize(nlanes);
+        for (int l = 0; l < nlanes; l++) {
+            Lane &ln = s->lanes[l];
+            SV_CHECK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+            if (photons_create(&ln.ctx, prm, device, ln.stream) != 0)
+                throw std::runtime_error(std::string("lane context: ") + photons_last_error());
+            ln.cap = max_batch_particles;
+            SV_CHECK(hipMalloc(&ln.in, sizeof(double) * 3 * (size_t)ln.cap));
+            SV_CHECK(hipMalloc(&ln.out, sizeof(double) * 3 * (size_t)ln.cap));
+        }
+        for (int l = 0; l < nlanes; l++) s->lanes[l].th = std::thread(lane_loop, s, l);
+        *out = s;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(std::string("photons_server_create: ") + e.what());
+    }
+}
+
+int photons_server_destroy(photons_server *s)
+{
+    if (!s) return 0;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    for (Lane &ln : s->lanes)
+        if (ln.th.joinable()) ln.th.join();
+    (void)hipSetDevice(s->device);
+    for (Lane &ln : s->lanes) {
+        if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+        if (ln.ctx) photons_destroy(ln.ctx);
+        if (ln.in) (void)hipFree(ln.in);
+        if (ln.out) (void)hipFree(ln.out);
+        if (ln.stream) (void)hipStreamDestroy(ln.stream);
+    }
+    for (auto &kv : s->boxes) {
+        (void)hipFree(kv.second.pos);
+        (void)hipFree(kv.second.vel);
+        (void)hipFree(kv.second.acc);
+    }
+    delete s;
+    return 0;
+}
+
+int photons_server_add_box(photons_server *s, int64_t n, double boxsize, const double *pos, const double *vel,
+                           const double *acc, int *box_id)
+{
+    try {
+        if (!s || !pos || !vel || !box_id) return fail("null argument");
+        if (!(boxsize > 0.0)) return fail("boxsize must be positive");
+        if (n <= (int64_t)s->prm.maxleaf) return fail("a periodic box needs more than maxleaf particles");
+        if (n > s->lane_cap) return fail("the box is larger than a lane's batch capacity");
+        const double bytes = kResidentBytesPerParticle * (double)n;
+        std::unique_lock<std::mutex> g(s->mu);
+        if (s->resident + bytes > s->budget - s->reserve) return fail("HBM budget exhausted: box not admitted");
+        s->resident += bytes;
+        s->peak = std::max(s->peak, s->resident + s->reserve);
+        g.unlock();
+        SV_CHECK(hipSetDevice(s->device));
+        Box b;
+        b.n = n;
+        b.L = boxsize;
+        const size_t nb = sizeof(double) * 3 * (size_t)n;
+        SV_CHECK(hipMalloc(&b.pos, nb));
+        SV_CHECK(hipMalloc(&b.vel, nb));
+        SV_CHECK(hipMalloc(&b.acc, nb));
+        SV_CHECK(hipMemcpy(b.pos, pos, nb, hipMemcpyHostToDevice));
+        SV_CHECK(hipMemcpy(b.vel, vel, nb, hipMemcpyHostToDevice));
+        if (acc) SV_CHECK(hipMemcpy(b.acc, acc, nb, hipMemcpyHostToDevice));
+        g.lock();
+        const int id = s->next_box++;
+        s->boxes[id] = b;
+        // without a given force the box's first request computes it (the force before the
+        // reference's first step, src/photoNs.c:97-116)
+        if (!acc) enqueue(s, id, kForce, 0.0, 0.0);
+        *box_id = id;
+        g.unlock();
+        s->cv_work.notify_all();
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(std::string("photons_server_add_box: ") + e.what());
+    }
+}
+
+int photons_server_remove_box(photons_server *s, int box_id)
+{
+    std::unique_lock<std::mutex> g(s->mu);
+    auto it = s->boxes.find(box_id);
+    if (it == s->boxes.end()) return fail("no such box");
+    // wait until no request of the box is queued or running
+    s->cv_done.wait(g, [&] {
+        if (it->second.busy) return false;
+        for (int64_t q : s->queue)
+            if (s->reqs[q].box == box_id) return false;
+        return true;
+    });
+    Box b = it->second;
+    s->boxes.erase(it);
+    s->resident -= kResidentBytesPerParticle * (double)b.n;
+    g.unlock();
+    (void)hipSetDevice(s->device);
+    (void)hipFree(b.pos);
+    (void)hipFree(b.vel);
+    (void)hipFree(b.acc);
+    return 0;
+}
+
+int photons_server_submit(photons_server *s, int box_id, double dkh, double dd, int64_t *request_id)
+{
+    std::unique_lock<std::mutex> g(s->mu);
+    if (!s->fatal.empty()) return fail("server stopped after an error: " + s->fatal);
+    if (s->boxes.find(box_id) == s->boxes.end()) return fail("no such box");
+    const int64_t id = enqueue(s, box_id, kStep, dkh, dd);
+    if (request_id) *request_id = id;
+    g.unlock();
+    s->cv_work.notify_all();
+    return 0;
+}
+
+int photons_server_wait(photons_server *s, int64_t request_id, double *latency_ms)
+{
+    std::unique_lock<std::mutex> g(s->mu);
+    auto it = s->reqs.find(request_id);
+    if (it == s->reqs.end()) return fail("no such request");
+    s->cv_done.wait(g, [&] { return it->second.done; });
+    if (latency_ms) *latency_ms = 1e3 * (it->second.t_done - it->second.t_submit);
+    if (!it->second.error.empty()) return fail(it->second.error);
+    return 0;
+}
+
+int photons_server_drain(photons_server *s)
+{
+    std::unique_lock<std::mutex> g(s->mu);
+    s->cv_done.wait(g, [&] {
+        if (!s->queue.empty()) return false;
+        for (auto &kv : s->boxes)
+            if (kv.second.busy) return false;
+        return true;
+    });
+    if (!s->fatal.empty()) return fail(s->fatal);
+    return 0;
+}
+
+int photons_server_read_box(photons_server *s, int box_id, double *pos, double *vel, double *acc)
+{
+    try {
+        std::unique_lock<std::mutex> g(s->mu);
+        auto it = s->boxes.find(box_id);
+        if (it == s->boxes.end()) return fail("no such box");
+        s->cv_done.wait(g, [&] { return !it->second.busy; });
+        const Box b = it->second;
+        it->second.busy = true;  // no lane takes the box while it is read
+        g.unlock();
+        SV_CHECK(hipSetDevice(s->device));
+        const size_t nb = sizeof(double) * 3 * (size_t)b.n;
+        std::string err;
+        try {
+            if (pos) SV_CHECK(hipMemcpy(pos, b.pos, nb, hipMemcpyDeviceToHost));
+            if (vel) SV_CHECK(hipMemcpy(vel, b.vel, nb, hipMemcpyDeviceToHost));
+            if (acc) SV_CHECK(hipMemcpy(acc, b.acc, nb, hipMemcpyDeviceToHost));
+        } catch (const std::exception &e) {
+            err = e.what();
+        }
+        g.lock();
+        s->boxes[box_id].busy = false;
+        g.unlock();
+        s->cv_work.notify_all();
+        s->cv_done.notify_all();
+        if (!err.empty()) return fail(err);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(std::string("photons_server_read_box: ") + e.what());
+    }
+}
+
+int photons_server_get_stats(photons_server *s, photons_server_stats *st)
+{
+    std::lock_guard<std::mutex> g(s->mu);
+    st->requests_done = s->n_done;
+    st->particles_stepped = s->particles_stepped;
+    st->batches = s->batches;
+    st->mean_boxes_per_batch = s->batches ? (double)s->batch_boxes / s->batches : 0.0;
+    st->span_s = s->t_first >= 0.0 ? s->t_last - s->t_first : 0.0;
+    st->nboxes = (int)s->boxes.size();
+    st->resident_bytes = s->resident;
+    st->reserved_bytes = s->reserve;
+    st->queued = (int64_t)s->queue.size();
+    return 0;
+}
+
+// kick and drift factors of the reference (src/initial.c:650-683): 128-block
+// composite Simpson in ln a of a^-1/H and a^-2/H, H(a) = 0.1 sqrt(Om a^-3 + OL)
+static double simpson_loga(double li, double lf, double om, double ox, int k)
+{
+    const int nblock = 128;
+    const double h = (lf - li) / nblock;
+    auto f = [&](double z) {
+        const double H = 0.1 * std::sqrt(om * z * z * z + ox);
+        return (k == 1 ? z : z * z) / H;
+    };
+    double s = h * f(1.0 / std::exp(li));
+    for (int j = 1; j < nblock; j++) s += 2.0 * (1 + j % 2) * h * f(1.0 / std::exp(li + h * j));
+    s += h * f(1.0 / std::exp(lf));
+    return s / 3.0;
+}
+
+double photons_kick_factor(double loga_i, double loga_f, double omega_m, double omega_x)
+{
+    return simpson_loga(loga_i, loga_f, omega_m, omega_x, 1);
+}
+
+double photons_drift_factor(double loga_i, double loga_f, double omega_m, double omega_x)
+{
+    return simpson_loga(loga_i, loga_f, omega_m, omega_x, 2);
+}
+
+}  // extern "C"

@@ -306,9 +306,38 @@ __global__ void k_number(const int *__restrict__ t_son, int b, int e, const int 
     }
 }
 
+// the roots (temporary ids 0 .. nbox-1, one per box, in box order) take consecutive
+// depth-first ranges: dfs and lfirst of root b = cells and leaves of the earlier roots
+__global__ void k_root_starts(int nbox, int nleaf, const int *__restrict__ size, const int *__restrict__ lsize,
+                              int *__restrict__ dfs, int *__restrict__ lfirst, int *__restrict__ box_root)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int d = 0, lf = 0;
+    for (int b = 0; b < nbox; b++) {
+        dfs[b] = d;
+        lfirst[b] = lf;
+        box_root[b] = nleaf + d - lf;
+        d += size[b];
+        lf += lsize[b];
+    }
+}
+
+// box owning particle slot i: the last b with box_off[b] <= i
+__device__ __forceinline__ int box_of_slot(const int64_t *__restrict__ box_off, int nbox, int64_t i)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (box_off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __global__ void k_emit_nodes(int ntmp, int nleaf, const int *__restrict__ dfs, const int *__restrict__ lfirst,
                              const int *__restrict__ size, TreeTmp T, const int *__restrict__ l_final,
-                             CellGeom *__restrict__ geom, CellInfo *__restrict__ info, int *__restrict__ level_nodes)
+                             CellGeom *__restrict__ geom, CellInfo *__restrict__ info, int *__restrict__ level_nodes,
+                             const int64_t *__restrict__ box_off, int nbox, int *__restrict__ cell_box)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= ntmp) return;
@@ -337,11 +366,14 @@ __global__ void k_emit_nodes(int ntmp, int nleaf, const int *__restrict__ dfs, c
     I.depth = T.t_depth[v];
     info[f] = I;
     level_nodes[v] = f;
+    // nodes hold more than maxleaf >= 2 particles, so their first slot names the box
+    cell_box[f] = box_of_slot(box_off, nbox, I.ipart);
 }
 
 __global__ void k_emit_leaves(int nl, int nleaf, const int *__restrict__ l_final, const int *__restrict__ l_dfs,
                               const int *__restrict__ dfs, const int *__restrict__ lfirst, TreeTmp T,
-                              CellGeom *__restrict__ geom, CellInfo *__restrict__ info)
+                              CellGeom *__restrict__ geom, CellInfo *__restrict__ info,
+                              const int64_t *__restrict__ box_off, int nbox, int *__restrict__ cell_box)
 {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= nl) return;
@@ -363,6 +395,8 @@ __global__ void k_emit_leaves(int nl, int nleaf, const int *__restrict__ l_final
     I.parent = nleaf + dfs[p] - lfirst[p];
     I.depth = T.t_depth[p] + 1;
     info[f] = I;
+    // a leaf may be empty: its parent's first slot names the box (every box holds a particle)
+    cell_box[f] = box_of_slot(box_off, nbox, T.t_start[p]);
 }
 
 __global__ void k_init_particles(int64_t n, const double *__restrict__ pos, Particles p)
@@ -447,7 +481,7 @@ void build_tree(Ctx &c)
         T.lcap = c.lcap = newcap;
     };
     // level-order temporaries: keep the capacity of earlier builds, start at ~4N/maxleaf
-    const int want = (int)std::max<int64_t>(1024, 4 * n / std::max(1, maxleaf) + 16);
+    const int want = (int)std::max<int64_t>(std::max(1024, 2 * c.nbox + 16), 4 * n / std::max(1, maxleaf) + 16);
     if (c.tcap < want) {
         alloc_nodes(want, 0);
     } else {
@@ -462,24 +496,31 @@ void build_tree(Ctx &c)
         T.l_lo = dptr<double>(c.l_lo); T.l_hi = dptr<double>(c.l_hi); T.lcap = c.lcap;
     }
 
-    // root
+    // roots: one per box, temporary ids 0 .. nbox-1, boxes [0, L_b]^3
+    const int nbox = c.nbox;
     {
-        int zero = 0, cnt = (int)n, parent = -1;
-        double lo = 0.0, hi = c.prm.boxsize;
-        PH_CHECK(hipMemcpyAsync(T.t_start, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-        PH_CHECK(hipMemcpyAsync(T.t_count, &cnt, sizeof(int), hipMemcpyHostToDevice, st));
-        PH_CHECK(hipMemcpyAsync(T.t_depth, &zero, sizeof(int), hipMemcpyHostToDevice, st));
-        PH_CHECK(hipMemcpyAsync(T.t_parent, &parent, sizeof(int), hipMemcpyHostToDevice, st));
+        std::vector<int> start(nbox), cnt(nbox), zero(nbox, 0), parent(nbox, -1);
+        std::vector<double> lo(3 * (size_t)nbox, 0.0), hi(3 * (size_t)nbox);
+        for (int b = 0; b < nbox; b++) {
+            start[b] = (int)c.box_off[b];
+            cnt[b] = (int)(c.box_off[b + 1] - c.box_off[b]);
+            for (int d = 0; d < 3; d++) hi[d * (size_t)nbox + b] = c.box_len[b];
+        }
+        const size_t ib = sizeof(int) * nbox, db = sizeof(double) * nbox;
+        PH_CHECK(hipMemcpyAsync(T.t_start, start.data(), ib, hipMemcpyHostToDevice, st));
+        PH_CHECK(hipMemcpyAsync(T.t_count, cnt.data(), ib, hipMemcpyHostToDevice, st));
+        PH_CHECK(hipMemcpyAsync(T.t_depth, zero.data(), ib, hipMemcpyHostToDevice, st));
+        PH_CHECK(hipMemcpyAsync(T.t_parent, parent.data(), ib, hipMemcpyHostToDevice, st));
         for (int d = 0; d < 3; d++) {
-            PH_CHECK(hipMemcpyAsync(T.t_lo + d * T.tcap, &lo, sizeof(double), hipMemcpyHostToDevice, st));
-            PH_CHECK(hipMemcpyAsync(T.t_hi + d * T.tcap, &hi, sizeof(double), hipMemcpyHostToDevice, st));
+            PH_CHECK(hipMemcpyAsync(T.t_lo + d * T.tcap, lo.data() + d * (size_t)nbox, db, hipMemcpyHostToDevice, st));
+            PH_CHECK(hipMemcpyAsync(T.t_hi + d * T.tcap, hi.data() + d * (size_t)nbox, db, hipMemcpyHostToDevice, st));
         }
         PH_CHECK(hipStreamSynchronize(st));
     }
 
     int *pin = reinterpret_cast<int *>(c.pinned.p);
-    std::vector<int> lvl = {0, 1};  // temp node ids of each level
-    int ntmp = 1, nleaf = 0;
+    std::vector<int> lvl = {0, nbox};  // temp node ids of each level
+    int ntmp = nbox, nleaf = 0;
     int depth = 0;
     std::vector<int> seg_chunk_h;
     while (lvl[depth + 1] > lvl[depth]) {
@@ -553,8 +594,8 @@ void build_tree(Ctx &c)
         int b = lvl[d], e = lvl[d + 1];
         k_sizes<<<blocks(e - b, 256), 256, 0, st>>>(T.t_son, b, e, size, lsize);
     }
-    PH_CHECK(hipMemsetAsync(dfs, 0, sizeof(int), st));
-    PH_CHECK(hipMemsetAsync(lfirst, 0, sizeof(int), st));
+    int *box_root = ensure<int>(c.d_box_root, nbox);
+    k_root_starts<<<1, 1, 0, st>>>(nbox, nleaf, size, lsize, dfs, lfirst, box_root);
     for (int d = 0; d < nlevels; d++) {
         int b = lvl[d], e = lvl[d + 1];
         k_number<<<blocks(e - b, 256), 256, 0, st>>>(T.t_son, b, e, size, lsize, dfs, lfirst, l_final, l_dfs);
@@ -567,9 +608,12 @@ void build_tree(Ctx &c)
     CellGeom *geom = ensure<CellGeom>(c.geom, c.ncell);
     CellInfo *info = ensure<CellInfo>(c.info, c.ncell);
     int *level_nodes = ensure<int>(c.level_nodes, ntmp);
+    int *cell_box = ensure<int>(c.cell_box, c.ncell);
+    const int64_t *box_off = dptr<int64_t>(c.d_box_off);
     k_emit_nodes<<<blocks(ntmp, 256), 256, 0, st>>>(ntmp, nleaf, dfs, lfirst, size, T, l_final, geom, info,
-                                                     level_nodes);
-    k_emit_leaves<<<blocks(nleaf, 256), 256, 0, st>>>(nleaf, nleaf, l_final, l_dfs, dfs, lfirst, T, geom, info);
+                                                     level_nodes, box_off, nbox, cell_box);
+    k_emit_leaves<<<blocks(nleaf, 256), 256, 0, st>>>(nleaf, nleaf, l_final, l_dfs, dfs, lfirst, T, geom, info,
+                                                      box_off, nbox, cell_box);
     c.level_off.assign(lvl.begin(), lvl.end());
     PH_CHECK(hipGetLastError());
 }

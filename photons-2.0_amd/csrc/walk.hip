@@ -47,8 +47,8 @@ struct BfsArgs {
     const CellGeom *geom;
     const CellInfo *info;
     int nleaf;
-    double theta, theta2, rc, rc2, boxsize;
-    CellGeom root;  // domain box
+    double theta, theta2, rc, rc2;
+    BoxView bx;  // box lengths: shifts and the domain box [0, L]^3 of the image prune
     const uint64_t *in;
     int64_t n_in;
     uint64_t *out;
@@ -140,15 +140,22 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, RuleOut 
         o.p2p = 1;
         return;
     }
-    double sx, sy, sz;
-    shift_of(sid, a.boxsize, sx, sy, sz);
+    double sx = 0.0, sy = 0.0, sz = 0.0, L = 0.0;
+    if (sid != 0) {
+        L = a.bx.len_of(A);
+        shift_of(sid, L, sx, sy, sz);
+    }
     const CellGeom gA = a.geom[A], gB = a.geom[B];
     const double d0 = gA.c[0] - (gB.c[0] + sx);
     const double d1 = gA.c[1] - (gB.c[1] + sy);
     const double d2 = gA.c[2] - (gB.c[2] + sz);
     const int flag = mac(gA.w, gB.w, d0, d1, d2, a.theta2, a.rc2);
     if (flag < 0) return;
-    const bool pruned = sid != 0 && !bleaf && let_pruned(a.root, gB, sx, sy, sz, a.rc, a.theta);
+    bool pruned = false;
+    if (sid != 0 && !bleaf) {
+        const CellGeom box = {{0.5 * L, 0.5 * L, 0.5 * L}, {L, L, L}};
+        pruned = let_pruned(box, gB, sx, sy, sz, a.rc, a.theta);
+    }
     int open;  // 1 target, 2 source
     if (flag == 1 || (aleaf && pruned)) {
         // accepted; in the image walk a target leaf against a pruned node takes M2L
@@ -277,10 +284,21 @@ static void scan64(Ctx &c, const int64_t *in, int64_t *out, int n)
     PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, c.stream));
 }
 
-__global__ void k_init_frontier(uint64_t *f, int A, int B, int shift_lo, int shift_hi)
+// first frontier: (root, root, shift) of every box for the shifts [lo, hi), or one
+// given (A, B) pair per shift when box_root is null
+__global__ void k_init_frontier(uint64_t *f, const int *__restrict__ box_root, int nbox, int A, int B, int shift_lo,
+                                int shift_hi)
 {
-    int s = shift_lo + threadIdx.x;
-    if (s < shift_hi) f[threadIdx.x] = pair_of(A, B, s);
+    const int nshift = shift_hi - shift_lo;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (box_root ? nbox : 1) * nshift) return;
+    const int s = shift_lo + i % nshift;
+    if (box_root) {
+        const int r = box_root[i / nshift];
+        f[i] = pair_of(r, r, s);
+    } else {
+        f[i] = pair_of(A, B, s);
+    }
 }
 
 // CSR offsets of the target-sorted tasks: off[t] = first task with target >= t
@@ -363,18 +381,20 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     a.theta2 = c.prm.theta * c.prm.theta;
     a.rc = c.prm.rc;
     a.rc2 = c.prm.rc * c.prm.rc;
-    a.boxsize = c.prm.boxsize;
-    PH_CHECK(hipMemcpyAsync(&a.root, a.geom + nleaf, sizeof(CellGeom), hipMemcpyDeviceToHost, st));
-    PH_CHECK(hipStreamSynchronize(st));
+    a.bx = box_view(c);
+    need(c.cell_box, sizeof(int) * (size_t)ncell, "cell boxes");
 
-    // frontier ping-pong and task buffers, kept from the last call and grown exactly
-    grow_keep(c.bfs_a, 64 * sizeof(uint64_t), 0, st);
-    int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 16;
-
+    // first frontier: the roots of all boxes, or the given pair (start_target >= 0)
+    const bool forest = start_target < 0;
     const int nshift = shift_hi - shift_lo;
-    k_init_frontier<<<1, 32, 0, st>>>(reinterpret_cast<uint64_t *>(c.bfs_a.p), start_target, start_source, shift_lo,
-                                      shift_hi);
-    int64_t n_in = nshift, n_p2p = 0, n_m2l = 0;
+    const int64_t n0 = (int64_t)(forest ? c.nbox : 1) * nshift;
+    if (forest) need(c.d_box_root, sizeof(int) * (size_t)c.nbox, "box roots");
+    grow_keep(c.bfs_a, sizeof(uint64_t) * std::max<int64_t>(n0, 64), 0, st);
+    int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 16;
+    k_init_frontier<<<(int)((n0 + 255) / 256), 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.bfs_a.p),
+                                                             forest ? dptr<int>(c.d_box_root) : nullptr, c.nbox,
+                                                             start_target, start_source, shift_lo, shift_hi);
+    int64_t n_in = n0, n_p2p = 0, n_m2l = 0;
     int levels = 0;
     while (n_in > 0) {
         const int ntile = (int)((n_in + kBfsTile - 1) / kBfsTile);
@@ -426,6 +446,6 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     PH_CHECK(hipGetLastError());
 }
 
-void run_walk(Ctx &c, int shift_lo, int shift_hi) { run_walk_from(c, shift_lo, shift_hi, c.nleaf, c.nleaf); }
+void run_walk(Ctx &c, int shift_lo, int shift_hi) { run_walk_from(c, shift_lo, shift_hi, -1, -1); }
 
 }  // namespace photons

@@ -174,3 +174,17 @@ def workload_clustered(nside: int = 256, box_at_256: float = 500000.0, rc_factor
 def ensemble_sizes(nbox: int, nmin: int, nmax: int, seed: int) -> np.ndarray:
     u = uniform(seed, nbox)
     return np.floor(np.exp(np.log(nmin) + u * (np.log(nmax) - np.log(nmin)))).astype(np.int64)
+
+
+def workload_ensemble(nbox: int = 64, nmin: int = 32 ** 3, nmax: int = 128 ** 3, seed: int = 5,
+                      order: int = 4, theta: float = 0.4, rc_factor: float = 6.0, box_at_256: float = 500000.0):
+    """Independent periodic boxes of log-uniform sizes in [nmin, nmax] (seeded), each
+    clustered like the 256^3 workload at its density: L_b = 500 Mpc/h (N_b/256^3)^(1/3).
+    At equal density rs (1.25 mean spacings), the softening and the particle mass are
+    the same for every box, so one parameter set serves the batch.
+    Returns (counts, lengths, concatenated positions, shared ForceParams)."""
+    counts = ensemble_sizes(nbox, nmin, nmax, seed)
+    lengths = box_at_256 * (counts / 256.0 ** 3) ** (1.0 / 3.0)
+    pos = [clustered(int(n), float(L), seed=1000 + 10 * b)[0] for b, (n, L) in enumerate(zip(counts, lengths))]
+    prm = force_params(256 ** 3, box_at_256, 256, rc_factor=rc_factor, order=order, theta=theta)
+    return counts, lengths, np.concatenate(pos, axis=0), prm

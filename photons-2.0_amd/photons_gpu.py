@@ -60,6 +60,12 @@ def lib():
         L.photons_force_host.argtypes = [vp, C.c_int64, dp, dp]
         L.photons_force_device.argtypes = [vp, C.c_int64, vp, vp]
         L.photons_sync.argtypes = [vp]
+        L.photons_force_boxes_host.argtypes = [vp, C.c_int, i64p, dp, dp, dp]
+        L.photons_force_boxes_device.argtypes = [vp, C.c_int, i64p, dp, vp, vp]
+        L.photons_set_p2p_stage.argtypes = [vp, C.c_int]
+        L.photons_get_p2p_stage.argtypes = [vp]
+        L.photons_autotune_boxes.argtypes = [vp, C.c_int, i64p, dp, vp, vp, C.c_int, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_int), dp]
         L.photons_prepare_host.argtypes = [vp, C.c_int64, dp]
         L.photons_evaluate_host.argtypes = [vp, dp]
         L.photons_tree_perm.argtypes = [vp, ip]
@@ -149,8 +155,50 @@ class Context:
         _check(lib().photons_force_device(self.h, int(n), C.c_void_p(d_pos), C.c_void_p(d_acc)),
                "photons_force_device")
 
+    def force_boxes(self, counts, lengths, pos):
+        """independent periodic boxes in one evaluation: counts[b] particles of box b
+        (concatenated in pos, box 0 first) in [0, lengths[b])^3 -> accelerations"""
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        if counts.shape != lengths.shape or counts.sum() != pos.shape[0]:
+            raise PhotonsError("counts, lengths and positions do not match")
+        acc = np.zeros_like(pos)
+        _check(lib().photons_force_boxes_host(self.h, len(counts), counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                              _dp(lengths), _dp(pos), _dp(acc)), "photons_force_boxes_host")
+        return acc
+
+    def force_boxes_device(self, counts, lengths, d_pos, d_acc):
+        """as force_boxes with device pointers (ints); queued on the context stream"""
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+        self._keep = (counts, lengths)
+        _check(lib().photons_force_boxes_device(self.h, len(counts), counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                                _dp(lengths), C.c_void_p(d_pos), C.c_void_p(d_acc)),
+               "photons_force_boxes_device")
+
     def sync(self):
         _check(lib().photons_sync(self.h), "photons_sync")
+
+    def set_p2p_stage(self, stage):
+        _check(lib().photons_set_p2p_stage(self.h, int(stage)), "photons_set_p2p_stage")
+
+    def p2p_stage(self):
+        return lib().photons_get_p2p_stage(self.h)
+
+    def autotune_boxes(self, counts, lengths, d_pos, d_acc, reps=1):
+        """time leaf 16/32/64 x P2P staging 256/512 on this batch; keeps the fastest.
+        Returns (leaf, stage, {(leaf, stage): ms})"""
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+        bl, bs, ms = C.c_int(), C.c_int(), np.zeros(6)
+        _check(lib().photons_autotune_boxes(self.h, len(counts), counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                            _dp(lengths), C.c_void_p(d_pos), C.c_void_p(d_acc), int(reps),
+                                            C.byref(bl), C.byref(bs), _dp(ms)), "photons_autotune_boxes")
+        self.params.maxleaf = bl.value
+        table = {(leaf, st): float(ms[2 * i + j]) for i, leaf in enumerate((16, 32, 64))
+                 for j, st in enumerate((256, 512)) if ms[2 * i + j] >= 0}
+        return bl.value, bs.value, table
 
     def prepare(self, pos):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
@@ -206,3 +254,112 @@ class Context:
 
 def nterms(order):
     return (order + 1) * (order + 2) * (order + 3) // 6
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("requests_done", C.c_int64), ("particles_stepped", C.c_int64), ("batches", C.c_int64),
+                ("queued", C.c_int64), ("mean_boxes_per_batch", C.c_double), ("span_s", C.c_double),
+                ("resident_bytes", C.c_double), ("reserved_bytes", C.c_double), ("nboxes", C.c_int32),
+                ("pad", C.c_int32)]
+
+    def as_dict(self):
+        return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
+
+
+def _server_lib():
+    L = lib()
+    if not getattr(L, "_server_types", False):
+        vp, dp, i64p = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int64)
+        L.photons_server_last_error.restype = C.c_char_p
+        L.photons_server_create.argtypes = [C.POINTER(vp), C.POINTER(Params), C.c_int, C.c_int, C.c_double, C.c_int64]
+        L.photons_server_destroy.argtypes = [vp]
+        L.photons_server_add_box.argtypes = [vp, C.c_int64, C.c_double, dp, dp, dp, C.POINTER(C.c_int)]
+        L.photons_server_remove_box.argtypes = [vp, C.c_int]
+        L.photons_server_submit.argtypes = [vp, C.c_int, C.c_double, C.c_double, i64p]
+        L.photons_server_wait.argtypes = [vp, C.c_int64, dp]
+        L.photons_server_drain.argtypes = [vp]
+        L.photons_server_read_box.argtypes = [vp, C.c_int, dp, dp, dp]
+        L.photons_server_get_stats.argtypes = [vp, C.POINTER(ServerStats)]
+        L.photons_kick_factor.restype = C.c_double
+        L.photons_kick_factor.argtypes = [C.c_double] * 4
+        L.photons_drift_factor.restype = C.c_double
+        L.photons_drift_factor.argtypes = [C.c_double] * 4
+        L._server_types = True
+    return L
+
+
+def _scheck(rc, what):
+    if rc != 0:
+        raise PhotonsError(f"{what}: {_server_lib().photons_server_last_error().decode()}")
+
+
+def kick_factor(loga_i, loga_f, omega_m=0.25, omega_x=0.75):
+    return _server_lib().photons_kick_factor(loga_i, loga_f, omega_m, omega_x)
+
+
+def drift_factor(loga_i, loga_f, omega_m=0.25, omega_x=0.75):
+    return _server_lib().photons_drift_factor(loga_i, loga_f, omega_m, omega_x)
+
+
+class Server:
+    """Resident boxes and a continuous-batching request loop (photons_server_*)."""
+
+    def __init__(self, prm, precision=FP32_MIXED, device=0, nlanes=2, hbm_budget=200e9,
+                 max_batch_particles=32 << 20):
+        L = _server_lib()
+        self.h = C.c_void_p()
+        self.params = make_params(prm, precision)
+        _scheck(L.photons_server_create(C.byref(self.h), C.byref(self.params), device, nlanes, float(hbm_budget),
+                                        int(max_batch_particles)), "photons_server_create")
+        self._n = {}
+
+    def close(self):
+        if self.h:
+            _server_lib().photons_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_box(self, pos, vel, boxsize, acc=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        vel = np.ascontiguousarray(vel, dtype=np.float64)
+        a = None if acc is None else _dp(np.ascontiguousarray(acc, dtype=np.float64))
+        bid = C.c_int()
+        _scheck(_server_lib().photons_server_add_box(self.h, pos.shape[0], float(boxsize), _dp(pos), _dp(vel), a,
+                                                     C.byref(bid)), "photons_server_add_box")
+        self._n[bid.value] = pos.shape[0]
+        return bid.value
+
+    def remove_box(self, bid):
+        _scheck(_server_lib().photons_server_remove_box(self.h, bid), "photons_server_remove_box")
+        del self._n[bid]
+
+    def submit(self, bid, dkh, dd):
+        rid = C.c_int64()
+        _scheck(_server_lib().photons_server_submit(self.h, bid, float(dkh), float(dd), C.byref(rid)),
+                "photons_server_submit")
+        return rid.value
+
+    def wait(self, rid):
+        ms = C.c_double()
+        _scheck(_server_lib().photons_server_wait(self.h, rid, C.byref(ms)), "photons_server_wait")
+        return ms.value
+
+    def drain(self):
+        _scheck(_server_lib().photons_server_drain(self.h), "photons_server_drain")
+
+    def read_box(self, bid):
+        n = self._n[bid]
+        pos, vel, acc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        _scheck(_server_lib().photons_server_read_box(self.h, bid, _dp(pos), _dp(vel), _dp(acc)),
+                "photons_server_read_box")
+        return pos, vel, acc
+
+    def stats(self):
+        st = ServerStats()
+        _scheck(_server_lib().photons_server_get_stats(self.h, C.byref(st)), "photons_server_get_stats")
+        return st.as_dict()

@@ -59,3 +59,13 @@ def test_no_gpu_means_an_error_not_a_fallback(lib):
     _, prm = ic.workload_lattice32()
     with pytest.raises(pg.PhotonsError):
         pg.Context(prm)
+
+
+def test_kick_drift_factors_match_the_oracle():
+    # host-side arithmetic of the library (no GPU needed)
+    import numpy as np
+    import oracle
+    for ai, af in ((0.02, 0.025), (0.3, 0.31), (0.5, 1.0)):
+        li, lf = np.log(ai), np.log(af)
+        assert abs(pg.kick_factor(li, lf) / oracle.kick_loga(li, lf) - 1) < 1e-14
+        assert abs(pg.drift_factor(li, lf) / oracle.drift_loga(li, lf) - 1) < 1e-14
