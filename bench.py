@@ -67,7 +67,7 @@ def run_precision(torch, pos_dev, n, prm, precision, steps, warmup):
     return dict(wall=np.array(wall), phases=ph, stats=st, acc=acc)
 
 
-def cpu_baseline(nside=40):
+def cpu_baseline(nside=48):
     """The FP64 CPU oracle (a restatement of the reference force path, one core) on a
     clustered sample with the 256^3 workload's density, rs and rc (nside^3 particles)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

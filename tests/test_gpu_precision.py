@@ -1,8 +1,9 @@
 """FP32-mixed vs FP64 accuracy over the expansion order and opening angle.
 
-Clustered particles at the density, rs and rc of the 256^3 workload (64^3 of them in a
-125 Mpc/h box), p in {3, 4, 5} x theta in {0.3, 0.4, 0.5}.  Errors are per-particle
-relative acceleration errors on a seeded subsample against
+Clustered particles at the density, rs and rc of the 256^3 workload: 64^3 of them in a
+125 Mpc/h box with a 4096-particle subsample by default, the 256^3 workload itself with
+a 65536-particle subsample under --full; p in {3, 4, 5} x theta in {0.3, 0.4, 0.5}.
+Errors are per-particle relative acceleration errors on the seeded subsample against
   - the FP64 path on the GPU (the gate: p99 <= 1e-4 for FP32-mixed),
   - the FP64 direct-sum oracle, untruncated (all sources within 10 rs, where the split
     kernel is below 1e-10) and truncated at rc.
@@ -32,9 +33,10 @@ def rel(a, b):
 
 
 @pytest.fixture(scope="module")
-def box():
-    pos, prm = ic.workload_clustered(nside=64)
-    sub = np.sort(np.random.Generator(np.random.PCG64(4)).choice(len(pos), 4096, replace=False)).astype(np.int32)
+def box(full_sizes):
+    nside, nsub = (256, 65536) if full_sizes else (64, 4096)
+    pos, prm = ic.workload_clustered(nside=nside)
+    sub = np.sort(np.random.Generator(np.random.PCG64(4)).choice(len(pos), nsub, replace=False)).astype(np.int32)
     full = oracle.direct(pos, prm, sub, 10.0 * prm.rs, nthreads=16)
     trunc = oracle.direct(pos, prm, sub, prm.rc, nthreads=16)
     return pos, prm, sub, full, trunc
@@ -45,7 +47,7 @@ def test_precision_sweep(box):
     rows = []
     for order in (3, 4, 5):
         for theta in (0.3, 0.4, 0.5):
-            p = ic.force_params(len(pos), prm.boxsize, 64, order=order, theta=theta)
+            p = ic.force_params(len(pos), prm.boxsize, round(len(pos) ** (1 / 3)), order=order, theta=theta)
             a64 = pg.Context(p, pg.FP64).force(pos)
             a32 = pg.Context(p, pg.FP32_MIXED).force(pos)
             r = {"order": order, "theta": theta,
@@ -60,7 +62,7 @@ def test_precision_sweep(box):
                   flush=True)
             assert r["fp32_vs_fp64"]["p99"] <= 1e-4
     os.makedirs(OUT, exist_ok=True)
-    with open(os.path.join(OUT, "precision_sweep.json"), "w") as f:
+    with open(os.path.join(OUT, f"precision_sweep_{round(len(pos) ** (1 / 3))}.json"), "w") as f:
         json.dump(rows, f, indent=1)
     # the FP32-mixed error stays below the FMM's own error against the direct sum
     for r in rows:
