@@ -228,9 +228,33 @@ __device__ __forceinline__ float split_factor_f32(float u)
     return gauss_f32(u) * __builtin_fmaf(1.1283791670955126f, u, erfcx_tq(t));
 }
 
+// erfcx(u) = t Q20(t), t = 1/(1+u/2), u >= 0: a least-squares fit of the relative
+// error done here in 50-digit arithmetic on 1200 Chebyshev nodes in t (relative error
+// 2.6e-14 evaluated in double over u >= 0)
+__device__ __forceinline__ double erfcx_tq20(double t)
+{
+    constexpr double c[21] = {
+        2.82094791773874920e-01,  2.82094791776525355e-01,  2.46832942452028026e-01,  1.76309262185868731e-01,
+        8.37464818453457355e-02,  -4.40302425728239695e-03, -5.67611432847028585e-02, -5.38614146130594953e-02,
+        3.03490794480198913e-03,  -7.70416507730420971e-03, 3.74060151838098820e-01,  -1.28536904212529279e+00,
+        3.66391328683279927e+00,  -8.19524649206594979e+00, 1.29420464606019987e+01,  -1.42835478387177517e+01,
+        1.10704731945007335e+01,  -5.95597239466930795e+00, 2.13201704144582260e+00,  -4.58832900301043400e-01,
+        4.50751019138224407e-02};
+    double q = c[20];
+    static_for<0, 20>([&](auto k) { q = fma(q, t, c[19 - k.value]); });
+    return q;
+}
+
+// G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2) = exp(-u^2) (erfcx(u) + 2u/sqrt(pi)): one exp
+// instead of the two inside erfc and beside it; t = 1/(1+u/2) by v_rcp_f64 and two
+// Newton steps.  Relative error ~3e-14 against the erfc/exp form.
 __device__ __forceinline__ double split_factor_f64(double u)
 {
-    return erfc(u) + (2.0 / 1.7724538509055160273) * u * exp(-u * u);
+    const double x = fma(0.5, u, 1.0);
+    double t = __builtin_amdgcn_rcp(x);
+    t = fma(t, fma(-x, t, 1.0), t);
+    t = fma(t, fma(-x, t, 1.0), t);
+    return exp(-u * u) * fma(1.1283791670955126, u, t * erfcx_tq20(t));
 }
 
 // radial factors f_n = (r^-1 d/dr)^n [erfc(r/2rs)/r], n = 0..P, by the upward

@@ -265,78 +265,6 @@ __device__ __forceinline__ void store_acc(const P2PArgs &a, int i, double ax, do
     a.out[o + 2] = az;
 }
 
-template <int P>
-__global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
-{
-    __shared__ double sx_[kP2PMaxLeaf], sy_[kP2PMaxLeaf], sz_[kP2PMaxLeaf];
-    __shared__ double red[3][64];
-    const int T = blockIdx.x;
-    const int lane = threadIdx.x;
-    const CellInfo IT = a.info[T];
-    const int nT = IT.npart;
-    if (nT == 0) return;
-    const int nsub = nT >= 32 ? (nT > 64 ? 1 : 64 / nT) : min(64 / nT, 32);
-    const int t = lane % nT, k = lane / nT;
-    const bool active = k < nsub && t < nT;
-    const int it = IT.ipart + t;
-    const double xt = a.px[it], yt = a.py[it], zt = a.pz[it];
-    const double m = a.mass, eps = a.soft, meps3 = m / (eps * eps * eps);
-    const double boxlen = a.bx.len_of(T);
-    double ax = 0.0, ay = 0.0, az = 0.0;
-    for (int64_t e = a.off[T]; e < a.off[T + 1]; e++) {
-        const int32_t s = a.src[e];
-        const int S = entry_cell(s), sid = entry_shift(s);
-        const CellInfo IS = a.info[S];
-        double shx, shy, shz;
-        shift_vec(sid, boxlen, shx, shy, shz);
-        if (lane < IS.npart) {
-            const int j = IS.ipart + lane;
-            sx_[lane] = a.px[j] + shx;  // image body = pos + shift (src/remotes.c:84-86)
-            sy_[lane] = a.py[j] + shy;
-            sz_[lane] = a.pz[j] + shz;
-        }
-        __syncthreads();
-        if (active) {
-            for (int q = k; q < IS.npart; q += nsub) {
-                const double dx = sx_[q] - xt, dy = sy_[q] - yt, dz = sz_[q] - zt;
-                const double dr = sqrt(dx * dx + dy * dy + dz * dz);
-                double f = dr < eps ? meps3 : m / (dr * dr * dr);
-                f *= split_factor_f64(dr * a.inv2rs);
-                ax += dx * f;
-                ay += dy * f;
-                az += dz * f;
-            }
-        }
-        __syncthreads();
-    }
-    red[0][lane] = ax;
-    red[1][lane] = ay;
-    red[2][lane] = az;
-    __syncthreads();
-    if (lane < nT) {
-        double bx = 0.0, by = 0.0, bz = 0.0;
-        for (int kk = 0; kk < nsub; kk++) {
-            bx += red[0][lane + kk * nT];
-            by += red[1][lane + kk * nT];
-            bz += red[2][lane + kk * nT];
-        }
-        if (a.with_l2p) {
-            constexpr int NT = nterms(P);
-            const double *Lp = reinterpret_cast<const double *>(a.L) + (size_t)T * NT;
-            double Lr[NT];
-#pragma unroll
-            for (int q = 0; q < NT; q++) Lr[q] = Lp[q];
-            const CellGeom g = a.geom[T];
-            double lx, ly, lz;
-            l2p_eval<P, double>(Lr, xt - g.c[0], yt - g.c[1], zt - g.c[2], lx, ly, lz);
-            bx += lx;
-            by += ly;
-            bz += lz;
-        }
-        store_acc(a, it, bx, by, bz);
-    }
-}
-
 // FP32 pair force per interaction, about 30 VALU instructions:
 //   r2 = |d|^2 + 1e-30 (keeps rsqrt finite for coincident particles, whose d = 0)
 //   1/max(r, eps)^3 = min(rsqrt(r2), 1/eps)^3   (the reference's softening branch)
@@ -352,7 +280,9 @@ __device__ __forceinline__ void pair_f32(const PairConsts &k, float dx, float dy
 {
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, 1e-30f)));
     const float ri = __builtin_amdgcn_rsqf(r2);
-    const float rc = fminf(ri, k.ieps);
+    // min(ri, 1/eps) on the bit patterns: both are positive floats, whose order is
+    // that of their integer images (one v_min_i32, no NaN canonicalisation)
+    const float rc = __int_as_float(min(__float_as_int(ri), __float_as_int(k.ieps)));
     const float r = r2 * ri;
     const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(r, k.inv4rs, 1.0f));
     const float R = __builtin_fmaf(r, k.c2u, erfcx_tq7(t));
@@ -555,6 +485,132 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
             bz += lz * is2;
         }
         store_acc(a, it, (double)bx, (double)by, (double)bz);
+    }
+}
+
+// FP64 pair force in the reference's arithmetic (src/fmm.c:834-852): d = x_s - x_t,
+// r = |d|, m/eps^3 when r < eps else m/r^3, times G(r/2rs).
+__device__ __forceinline__ void pair_f64(double dx, double dy, double dz, double m, double eps, double meps3,
+                                         double inv2rs, double &ax, double &ay, double &az)
+{
+    const double dr = sqrt(dx * dx + dy * dy + dz * dz);
+    double f = dr < eps ? meps3 : m / (dr * dr * dr);
+    f *= split_factor_f64(dr * inv2rs);
+    ax += dx * f;
+    ay += dy * f;
+    az += dz * f;
+}
+
+// FP64 P2P + L2P: the batching and lane layout of k_p2p_f32, with source positions
+// staged in double as the reference forms them (image body = pos + shift,
+// src/remotes.c:84-86) and the reference's pair arithmetic.
+template <int P>
+__global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
+{
+    constexpr int STAGE = 256;
+    __shared__ double stage[3 * STAGE];
+    double *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
+    __shared__ double ecx[64], ecy[64], ecz[64];
+    __shared__ int eincl[64], ecnt[64], eip[64];
+    const int T = blockIdx.x;
+    const int lane = threadIdx.x;
+    const CellInfo IT = a.info[T];
+    const int n = IT.npart;
+    if (n == 0) return;
+    const LaneMap L = lane_map(n, lane);
+    const double xA = a.px[IT.ipart + L.tA], yA = a.py[IT.ipart + L.tA], zA = a.pz[IT.ipart + L.tA];
+    const double xB = a.px[IT.ipart + L.tB], yB = a.py[IT.ipart + L.tB], zB = a.pz[IT.ipart + L.tB];
+    const double m = a.mass, eps = a.soft, meps3 = m / (eps * eps * eps), inv2rs = a.inv2rs;
+    const double boxlen = a.bx.len_of(T);
+    double axA = 0.0, ayA = 0.0, azA = 0.0, axB = 0.0, ayB = 0.0, azB = 0.0;
+    const int64_t e1 = a.off[T + 1];
+    for (int64_t base = a.off[T]; base < e1;) {
+        const int nb = (int)min((int64_t)64, e1 - base);
+        int cnt = 0;
+        if (lane < nb) {
+            const int32_t s = a.src[base + lane];
+            const int S = entry_cell(s), sid = entry_shift(s);
+            const CellInfo IS = a.info[S];
+            double shx, shy, shz;
+            shift_vec(sid, boxlen, shx, shy, shz);
+            ecx[lane] = shx;
+            ecy[lane] = shy;
+            ecz[lane] = shz;
+            eip[lane] = IS.ipart;
+            cnt = IS.npart;
+        }
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int v = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += v;
+        }
+        const int ne = __popcll(__ballot(lane < nb && incl <= STAGE));
+        const int tot = __shfl(incl, ne - 1, 64);
+        ecnt[lane] = cnt;
+        eincl[lane] = lane < ne ? incl : 0x7fffffff;
+        __syncthreads();
+        // one staging round at a time (FP64 pairs cost enough that the load latency
+        // is hidden by the other wavefronts; fewer live registers)
+        for (int p = lane; p < tot; p += 64) {
+            int j = 0;
+#pragma unroll
+            for (int s = 32; s >= 1; s >>= 1)
+                if (eincl[j + s - 1] <= p) j += s;
+            const int gi = eip[j] + p - (eincl[j] - ecnt[j]);
+            sx[p] = a.px[gi] + ecx[j];
+            sy[p] = a.py[gi] + ecy[j];
+            sz[p] = a.pz[gi] + ecz[j];
+        }
+        __syncthreads();
+        for (int q = L.kA; q < tot; q += L.sA)
+            pair_f64(sx[q] - xA, sy[q] - yA, sz[q] - zA, m, eps, meps3, inv2rs, axA, ayA, azA);
+        if (L.e)
+            for (int q = L.kB; q < tot; q += L.sB)
+                pair_f64(sx[q] - xB, sy[q] - yB, sz[q] - zB, m, eps, meps3, inv2rs, axB, ayB, azB);
+        base += ne;
+        __syncthreads();  // the staging and entry arrays are rewritten by the next batch
+    }
+    double *red = stage;  // 6 x 64 doubles, the staging array is free now
+    red[lane] = axA;
+    red[64 + lane] = ayA;
+    red[128 + lane] = azA;
+    red[192 + lane] = axB;
+    red[256 + lane] = ayB;
+    red[320 + lane] = azB;
+    __syncthreads();
+    if (lane < n) {
+        const int t = lane;
+        double bx = 0.0, by = 0.0, bz = 0.0;
+        const int kmax = t < L.e ? L.a + 1 : L.a;
+        for (int k = 0; k < kmax; k++) {
+            bx += red[t + n * k];
+            by += red[64 + t + n * k];
+            bz += red[128 + t + n * k];
+        }
+        if (t >= L.e) {
+            for (int j = 0; j < L.b; j++) {
+                const int l = (t - L.e) + L.m * j;
+                bx += red[192 + l];
+                by += red[256 + l];
+                bz += red[320 + l];
+            }
+        }
+        const int it = IT.ipart + t;
+        if (a.with_l2p) {
+            constexpr int NT = nterms(P);
+            const double *Lp = reinterpret_cast<const double *>(a.L) + (size_t)T * NT;
+            double Lr[NT];
+#pragma unroll
+            for (int q = 0; q < NT; q++) Lr[q] = Lp[q];
+            const CellGeom g = a.geom[T];
+            double lx, ly, lz;
+            l2p_eval<P, double>(Lr, a.px[it] - g.c[0], a.py[it] - g.c[1], a.pz[it] - g.c[2], lx, ly, lz);
+            bx += lx;
+            by += ly;
+            bz += lz;
+        }
+        store_acc(a, it, bx, by, bz);
     }
 }
 
