@@ -35,6 +35,10 @@ METRIC = "particles/s per force eval, FP32-mixed vs FP64 @1e-4 err; p50/p99 late
 PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32
 PEAK_FP64_VALU = 78.6
 P2P_FLOPS = 27  # per particle-particle interaction
+# HBM bytes per FP32 P2P launch at 256^3 clustered, p = 4, from rocprofv3 --pmc
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy in the same run);
+# the counts and commands are kept with the P2P counter summary under profiles/
+P2P_FP32_TRAFFIC_256 = (2.0 * 15100000 + 784000) * 1024
 
 
 def log(msg):
@@ -106,7 +110,14 @@ def run_ensemble(args, torch):
         res[name] = dict(wall=np.array(wall), acc=d_acc.cpu().numpy(), leaf=leaf, stage=stage,
                          table={f"leaf{k[0]}_stage{k[1]}": v for k, v in table.items()}, stats=ctx.stats())
         ctx.close()
-    e = np.linalg.norm(res["fp32"]["acc"] - res["fp64"]["acc"], axis=1) / np.linalg.norm(res["fp64"]["acc"], axis=1)
+    # the precision error at equal tree parameters: FP64 with the FP32 leaf size
+    p64 = ic.ForceParams(**{**prm.as_dict(), "maxleaf": res["fp32"]["leaf"]})
+    ref = pg.Context(p64, pg.FP64)
+    ref.force_boxes_device(counts, lengths, d_pos.data_ptr(), d_acc.data_ptr())
+    ref.sync()
+    a64 = d_acc.cpu().numpy()
+    ref.close()
+    e = np.linalg.norm(res["fp32"]["acc"] - a64, axis=1) / np.linalg.norm(a64, axis=1)
     ms = res["fp32"]["wall"] * 1e3
     return {
         "metric": METRIC, "value": n / float(np.mean(res["fp32"]["wall"])), "unit": "particles/s",
@@ -219,7 +230,7 @@ def main():
     ap.add_argument("--serve-boxes", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival span per rate")
     ap.add_argument("--rates32", type=float, nargs="+", default=[4, 8, 16, 32, 64, 128])
-    ap.add_argument("--rates64", type=float, nargs="+", default=[2, 4, 8, 16])
+    ap.add_argument("--rates64", type=float, nargs="+", default=[2, 4, 8, 16, 32])
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="first copy MiB mebibytes device to device (a plain copy kernel of known size, "
                          "to scale hardware byte counters)")
@@ -286,8 +297,12 @@ def main():
         "tree": {"nleaf": st["nleaf"], "nnode": st["nnode"], "depth": st["depth"],
                  "p2p_pairs": st["p2p_pairs"], "m2l_pairs": st["m2l_pairs"]},
         "roofline": {"bound": "valu", "kernel": "k_p2p_f32" if fp32 else "k_p2p_f64", "achieved": achieved,
-                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "note": "27 flops per interaction x interactions / mean P2P kernel time (HIP events)"},
+                     "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": (P2P_FP32_TRAFFIC_256 if fp32 and args.nside == 256 and prm.order == 4 else None),
+                     "note": "27 flops per interaction x interactions / mean P2P kernel time (HIP events on the "
+                             "library's stream); compute bound on the vector ALU (3 of ~31 instructions per "
+                             "interaction are quarter-rate transcendentals); traffic = HBM bytes per launch "
+                             "from rocprofv3 PMC"},
     }
     if "fp32" in res and "fp64" in res:
         a32, a64 = res["fp32"]["acc"], res["fp64"]["acc"]
