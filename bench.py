@@ -71,18 +71,39 @@ def run_precision(torch, pos_dev, n, prm, precision, steps, warmup):
     return dict(wall=np.array(wall), phases=ph, stats=st, acc=acc)
 
 
-def cpu_baseline(nside=48):
-    """The FP64 CPU oracle (a restatement of the reference force path, one core) on a
-    clustered sample with the 256^3 workload's density, rs and rc (nside^3 particles)."""
+def _cpu_rank(args):
+    """one 'rank': the FP64 oracle on its own clustered box (separate process)"""
+    nside, seed = args
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    pos, prm = ic.workload_clustered(nside=nside, order=4)
+    pos, prm = ic.workload_clustered(nside=nside, order=4, seed=seed)
     t0 = time.perf_counter()
     oracle.force(pos, prm)
-    dt = time.perf_counter() - t0
-    return {"value": len(pos) / dt, "unit": "particles/s", "cores": 1, "kind": "port",
-            "sample": f"{nside}^3 clustered particles at the same density, rs, rc, theta, p=4 and leaf size "
-                      f"as the 256^3 workload (box {prm.boxsize:.0f} kpc/h), one force evaluation, {dt:.1f} s"}
+    return len(pos), time.perf_counter() - t0
+
+
+def cpu_baseline(nside=48, cores=None):
+    """The FP64 CPU oracle (a restatement of the reference force path, single threaded
+    like one reference rank) on clustered boxes with the 256^3 workload's density, rs,
+    rc, theta, p = 4 and leaf size (nside^3 particles each): first one rank alone, then
+    one rank per host core at once, each on its own box (the reference's one rank per
+    core, without its domain exchange)."""
+    import multiprocessing as mp
+    if cores is None:
+        cores = min(16, len(os.sched_getaffinity(0)))
+    n1, t1 = _cpu_rank((nside, 101))
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores) as pool:
+        t0 = time.perf_counter()
+        res = pool.map(_cpu_rank, [(nside, 101 + k) for k in range(cores)])
+        wall = time.perf_counter() - t0
+    ntot = sum(r[0] for r in res)
+    _, prm = ic.workload_clustered(nside=nside, order=4)
+    return {"value": ntot / wall, "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent single-threaded ranks, each one force evaluation of its own {nside}^3 "
+                      f"clustered box at the density, rs, rc, theta, p=4 and leaf size of the 256^3 workload "
+                      f"(box {prm.boxsize:.0f} kpc/h), {wall:.1f} s wall",
+            "one_core": {"value": n1 / t1, "seconds": t1}}
 
 
 def run_ensemble(args, torch):
@@ -316,7 +337,7 @@ def main():
         out["error_fp32_vs_fp64"] = {"p50": float(np.median(e)), "p99": float(np.percentile(e, 99)),
                                      "max": float(e.max())}
     if not args.no_cpu_baseline:
-        log("cpu baseline (oracle, one core)")
+        log("cpu baseline (oracle: one rank, then one rank per core)")
         out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 
