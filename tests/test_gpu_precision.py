@@ -12,6 +12,7 @@ truncation, which the printed table shows.
 """
 import json
 import os
+import tempfile
 
 import numpy as np
 import pytest
@@ -21,7 +22,8 @@ import oracle
 import photons_gpu as pg
 
 pytestmark = pytest.mark.gpu
-OUT = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gpurun_out")
+# the table is written where PHOTONS_RESULTS_DIR points (default: the system temp dir)
+OUT = os.environ.get("PHOTONS_RESULTS_DIR", tempfile.gettempdir())
 
 
 def pct(e):
