@@ -488,13 +488,23 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
     }
 }
 
-// FP64 pair force in the reference's arithmetic (src/fmm.c:834-852): d = x_s - x_t,
-// r = |d|, m/eps^3 when r < eps else m/r^3, times G(r/2rs).
+// FP64 pair force as the reference forms it (src/fmm.c:834-852): d = x_s - x_t,
+// r = |d|, m/eps^3 when r < eps else m/r^3, times G(r/2rs).  1/r comes from
+// v_rsq_f64 and two Newton steps (r = r^2 / r, m/r^3 = m (1/r)^3): the same values
+// to an ulp, without the IEEE square root and divide sequences.
 __device__ __forceinline__ void pair_f64(double dx, double dy, double dz, double m, double eps, double meps3,
                                          double inv2rs, double &ax, double &ay, double &az)
 {
-    const double dr = sqrt(dx * dx + dy * dy + dz * dz);
-    double f = dr < eps ? meps3 : m / (dr * dr * dr);
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    // floored so that coincident particles (d = 0) keep a finite 1/r and take the
+    // eps branch with r = 0
+    const double r2s = fmax(r2, 1e-280);
+    double ri = __builtin_amdgcn_rsq(r2s);
+    const double h = 0.5 * r2s;
+    ri = ri * fma(-h * ri, ri, 1.5);
+    ri = ri * fma(-h * ri, ri, 1.5);
+    const double dr = r2 * ri;
+    double f = dr < eps ? meps3 : m * (ri * ri * ri);
     f *= split_factor_f64(dr * inv2rs);
     ax += dx * f;
     ay += dy * f;
