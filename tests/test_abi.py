@@ -69,3 +69,26 @@ def test_kick_drift_factors_match_the_oracle():
         li, lf = np.log(ai), np.log(af)
         assert abs(pg.kick_factor(li, lf) / oracle.kick_loga(li, lf) - 1) < 1e-14
         assert abs(pg.drift_factor(li, lf) / oracle.drift_loga(li, lf) - 1) < 1e-14
+
+
+def test_c_driver_fails_loudly_without_a_gpu(tmp_path):
+    """the compiled reference-style driver links the library; with no GPU the force
+    call stops with a message instead of computing anything on the CPU"""
+    import numpy as np
+    exe = os.path.join(ROOT, "photons-2.0_amd", "examples", "fmm_driver")
+    if not os.path.exists(exe):
+        pytest.skip("driver not built")
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
+    with open(fin, "wb") as f:
+        f.write(np.array([100, 32, 0, 0], np.int32).tobytes())
+        f.write(np.array([1000.0, 1.0, 1.0, 10.0, 60.0, 0.4]).tobytes())
+        f.write(np.random.default_rng(0).random((100, 3)).tobytes())
+    r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "GPU" in (r.stdout + r.stderr)

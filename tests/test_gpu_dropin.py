@@ -204,3 +204,37 @@ def test_pair_kernels_of_kernels_h():
         tol = 1e-5 if kern.endswith("rinvsqrt") else 1e-12
         assert np.abs(got - want).max() <= tol * np.abs(want).max(), kern
         assert np.abs(body[:a, 3:6]).max(initial=0.0) == 0.0
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "photons-2.0_amd", "examples",
+                      "fmm_driver")
+
+
+def run_driver(pos, prm, tmp, fp32=False):
+    """the C program photons-2.0_amd/examples/fmm_driver.c: the reference driver's
+    force call (src/photoNs.c:222-252) linked against libphotons_fmm.so"""
+    import subprocess
+    fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+    d = prm.as_dict()
+    with open(fin, "wb") as f:
+        f.write(np.array([len(pos), d["maxleaf"], 0, 0], np.int32).tobytes())
+        f.write(np.array([d["boxsize"], d["mass"], d["soft"], d["rs"], d["rc"], d["theta"]]).tobytes())
+        f.write(np.ascontiguousarray(pos, dtype=np.float64).tobytes())
+    subprocess.run([DRIVER, fin, fout] + (["fp32"] if fp32 else []), check=True, timeout=120)
+    raw = open(fout, "rb").read()
+    n = len(pos)
+    acc = np.frombuffer(raw[: 24 * n], dtype=np.float64).reshape(n, 3)
+    cnt = np.frombuffer(raw[24 * n:], dtype=np.int64)
+    return acc, cnt
+
+
+@pytest.mark.parametrize("name", ["clus16_pm32", "lattice32_rc6"])
+def test_compiled_c_driver_matches_reference_golden(name, tmp_path):
+    pos, prm = case_inputs(name)
+    acc, cnt = run_driver(pos, prm, str(tmp_path))
+    g = check_golden(name, acc)
+    assert list(cnt[:4]) == list(g["counters"][:4])
+    assert cnt[4] == g["counters"][4] and cnt[5] == g["counters"][5]
+    a32, _ = run_driver(pos, prm, str(tmp_path), fp32=True)
+    e = np.linalg.norm(a32[g["idx"]] - g["acc"], axis=1) / np.linalg.norm(g["acc"], axis=1)
+    assert np.percentile(e, 99) <= 1e-4
