@@ -190,6 +190,24 @@ int photons_server_get_stats(photons_server *srv, photons_server_stats *st);
 double photons_kick_factor(double loga_i, double loga_f, double omega_m, double omega_x);
 double photons_drift_factor(double loga_i, double loga_f, double omega_m, double omega_x);
 
+/* ---- Gadget-2 snapshot/IC files (host only) -------------------------------
+ * The reference's format and conventions (src/snapshot.c:5-22, 64-120, 211-293,
+ * 397-503): 256-byte header, float32 positions, float32 velocities stored as
+ * v / a^1.5 (a = 1/(1+z)); the reader returns v.  The writer writes one file of
+ * type-1 particles with the record lengths set (the reference leaves them unset;
+ * neither reader looks at them).                                              */
+typedef struct {
+    int64_t n;              /* particles in this file (sum of npart) */
+    int32_t npart[6], npart_total[6], num_files, pad;
+    double mass[6], time, redshift, boxsize, omega_m, omega_l, hubble;
+} photons_snapshot_info;
+
+const char *photons_snapshot_last_error(void);
+int photons_snapshot_header(const char *path, photons_snapshot_info *info);
+int photons_snapshot_read(const char *path, int64_t n_start, int64_t n_count, double *pos, double *vel);
+int photons_snapshot_write(const char *path, const photons_snapshot_info *info, int64_t n, int64_t npart_total_all,
+                           const double *pos, const double *vel);
+
 /* ======================================================================= */
 /* 2. drop-in layer: the reference's entry points                            */
 /* ======================================================================= */

@@ -363,3 +363,67 @@ class Server:
         st = ServerStats()
         _scheck(_server_lib().photons_server_get_stats(self.h, C.byref(st)), "photons_server_get_stats")
         return st.as_dict()
+
+
+# --------------------------------------------------------------------------- Gadget-2 files (host only)
+
+class SnapshotInfo(C.Structure):
+    _fields_ = [("n", C.c_int64), ("npart", C.c_int32 * 6), ("npart_total", C.c_int32 * 6), ("num_files", C.c_int32),
+                ("pad", C.c_int32), ("mass", C.c_double * 6), ("time", C.c_double), ("redshift", C.c_double),
+                ("boxsize", C.c_double), ("omega_m", C.c_double), ("omega_l", C.c_double), ("hubble", C.c_double)]
+
+    def as_dict(self):
+        d = {}
+        for f in self._fields_:
+            if f[0] == "pad":
+                continue
+            v = getattr(self, f[0])
+            d[f[0]] = list(v) if hasattr(v, "__len__") else v
+        return d
+
+
+def _snap_lib():
+    L = lib()
+    if not getattr(L, "_snap_types", False):
+        L.photons_snapshot_last_error.restype = C.c_char_p
+        L.photons_snapshot_header.argtypes = [C.c_char_p, C.POINTER(SnapshotInfo)]
+        L.photons_snapshot_read.argtypes = [C.c_char_p, C.c_int64, C.c_int64, C.c_void_p, C.c_void_p]
+        L.photons_snapshot_write.argtypes = [C.c_char_p, C.POINTER(SnapshotInfo), C.c_int64, C.c_int64,
+                                             C.c_void_p, C.c_void_p]
+        L._snap_types = True
+    return L
+
+
+def _snap_check(rc, what):
+    if rc != 0:
+        raise PhotonsError(f"{what}: {_snap_lib().photons_snapshot_last_error().decode()}")
+
+
+def snapshot_header(path):
+    """header of a Gadget-2 file as a dict (src/snapshot.c:64-120)"""
+    info = SnapshotInfo()
+    _snap_check(_snap_lib().photons_snapshot_header(str(path).encode(), C.byref(info)), "photons_snapshot_header")
+    return info.as_dict()
+
+
+def snapshot_read(path, start=0, count=None):
+    """positions and velocities [count][3] (velocities times a^1.5, src/snapshot.c:211-293)"""
+    if count is None:
+        count = snapshot_header(path)["n"] - start
+    pos, vel = np.zeros((count, 3)), np.zeros((count, 3))
+    _snap_check(_snap_lib().photons_snapshot_read(str(path).encode(), int(start), int(count), pos.ctypes.data,
+                                                  vel.ctypes.data), "photons_snapshot_read")
+    return pos, vel
+
+
+def snapshot_write(path, pos, vel, boxsize, redshift, mass, omega_m=0.25, omega_l=0.75, hubble=0.7, n_total=None):
+    """one file of type-1 particles (src/snapshot.c:397-503)"""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    vel = np.ascontiguousarray(vel, dtype=np.float64)
+    info = SnapshotInfo()
+    info.boxsize, info.redshift, info.omega_m, info.omega_l, info.hubble = boxsize, redshift, omega_m, omega_l, hubble
+    info.mass[1] = mass
+    n = pos.shape[0]
+    _snap_check(_snap_lib().photons_snapshot_write(str(path).encode(), C.byref(info), n,
+                                                   n if n_total is None else int(n_total), pos.ctypes.data,
+                                                   vel.ctypes.data), "photons_snapshot_write")
