@@ -190,6 +190,22 @@ int photons_server_get_stats(photons_server *srv, photons_server_stats *st);
 double photons_kick_factor(double loga_i, double loga_f, double omega_m, double omega_x);
 double photons_drift_factor(double loga_i, double loga_f, double omega_m, double omega_x);
 
+/* ---- PM long-range force (one rank's whole periodic box, FP64) -------------
+ * The reference's partmesh() and its convolution on one rank (src/partmesh.c:
+ * 800-1550, src/conv.f90:171-229): CIC deposit with mesh points at cell centres,
+ * FFT, Green's function pref exp(-k^2 (2 pi rs/L)^2) sinc^-4 / k^2 with
+ * pref = L^2/(pi N^3), gf(0) = pref and pi = 3.1415926 as the reference
+ * truncates it, inverse FFT, 4th-order central-difference gradient, CIC
+ * interpolation.  acc_pm (no G) in the caller's order.  No counterpart of the
+ * reference's multi-rank slab exchange: the whole mesh lives on the GPU.      */
+typedef struct photons_pm photons_pm;
+const char *photons_pm_last_error(void);
+int photons_pm_create(photons_pm **pm, int nside, double boxsize, double rs, double mass, int device, void *stream);
+int photons_pm_destroy(photons_pm *pm);
+int photons_pm_force_device(photons_pm *pm, int64_t n, const double *d_pos, double *d_acc);
+int photons_pm_force_host(photons_pm *pm, int64_t n, const double *pos, double *acc);
+int photons_pm_sync(photons_pm *pm);
+
 /* ---- Gadget-2 snapshot/IC files (host only) -------------------------------
  * The reference's format and conventions (src/snapshot.c:5-22, 64-120, 211-293,
  * 397-503): 256-byte header, float32 positions, float32 velocities stored as

@@ -427,3 +427,57 @@ def snapshot_write(path, pos, vel, boxsize, redshift, mass, omega_m=0.25, omega_
     _snap_check(_snap_lib().photons_snapshot_write(str(path).encode(), C.byref(info), n,
                                                    n if n_total is None else int(n_total), pos.ctypes.data,
                                                    vel.ctypes.data), "photons_snapshot_write")
+
+
+# --------------------------------------------------------------------------- PM long-range force
+
+def _pm_lib():
+    L = lib()
+    if not getattr(L, "_pm_types", False):
+        vp, dp = C.c_void_p, C.POINTER(C.c_double)
+        L.photons_pm_last_error.restype = C.c_char_p
+        L.photons_pm_create.argtypes = [C.POINTER(vp), C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, vp]
+        L.photons_pm_destroy.argtypes = [vp]
+        L.photons_pm_force_device.argtypes = [vp, C.c_int64, vp, vp]
+        L.photons_pm_force_host.argtypes = [vp, C.c_int64, dp, dp]
+        L.photons_pm_sync.argtypes = [vp]
+        L._pm_types = True
+    return L
+
+
+class PM:
+    """PM long-range force on an nside^3 mesh (photons_pm_*): acc_pm without G."""
+
+    def __init__(self, nside, boxsize, rs, mass, device=0, stream=None):
+        L = _pm_lib()
+        self.h = C.c_void_p()
+        rc = L.photons_pm_create(C.byref(self.h), int(nside), float(boxsize), float(rs), float(mass), device,
+                                 C.c_void_p(stream) if stream else None)
+        if rc != 0:
+            raise PhotonsError(f"photons_pm_create: {L.photons_pm_last_error().decode()}")
+
+    def close(self):
+        if self.h:
+            _pm_lib().photons_pm_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def force(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        acc = np.zeros_like(pos)
+        if _pm_lib().photons_pm_force_host(self.h, pos.shape[0], _dp(pos), _dp(acc)) != 0:
+            raise PhotonsError(f"photons_pm_force_host: {_pm_lib().photons_pm_last_error().decode()}")
+        return acc
+
+    def force_device(self, n, d_pos, d_acc):
+        if _pm_lib().photons_pm_force_device(self.h, int(n), C.c_void_p(d_pos), C.c_void_p(d_acc)) != 0:
+            raise PhotonsError(f"photons_pm_force_device: {_pm_lib().photons_pm_last_error().decode()}")
+
+    def sync(self):
+        if _pm_lib().photons_pm_sync(self.h) != 0:
+            raise PhotonsError(f"photons_pm_sync: {_pm_lib().photons_pm_last_error().decode()}")
