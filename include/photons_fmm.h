@@ -287,6 +287,15 @@ void walk_task_m2l_ext(int im, int jm);
 /* image exchange, src/remotes.c:684-751 (one rank: idx = 0 only) */
 void fmm_remote(int idx, double shift[3]);
 
+/* PM, src/partmesh.c:11-16, 800-1550 (inc/partmesh.h:16-22), on the GPU through
+ * photons_pm_*: part[].acc_pm = long-range force of part[] (no G) on an
+ * NSIDE^3 mesh with splitRadius, BOXSIZE and MASSPART.  pm_thread may run
+ * beside fmm_task, as the reference's PMTHREAD build does.  A program that
+ * still links the reference's partmesh.c keeps its own definitions.         */
+void partmesh(void);
+void partmesh_thread(void);
+void *pm_thread(void *arg);
+
 #ifdef __cplusplus
 }
 #endif

@@ -477,3 +477,57 @@ void fmm_remote(int idx, double shift[3])
     /* fmm_remote_task: the image P2P walk, then the image M2L walk (src/remotes.c:631-682) */
     walk_eval(first_node, first_node, g_shift_id, g_shift_id + 1, 1, 1);
 }
+
+/* ---- PM: src/partmesh.c:800-1550 (partmesh), :11-16 (pm_thread) on one rank ------ */
+int NSIDE; /* inc/photoNs.h:58 */
+
+static photons_pm *g_pm;
+static int g_pm_nside;
+static double g_pm_box, g_pm_rs, g_pm_mass;
+static double *g_pm_io;
+static int64_t g_pm_cap;
+
+/* part[].acc_pm = long-range force of part[] (overwritten, no G), on the GPU */
+void partmesh(void)
+{
+    if (!g_pm || g_pm_nside != NSIDE || g_pm_box != BOXSIZE || g_pm_rs != splitRadius || g_pm_mass != MASSPART) {
+        if (g_pm) photons_pm_destroy(g_pm);
+        g_pm = NULL;
+        if (photons_pm_create(&g_pm, NSIDE, BOXSIZE, splitRadius, MASSPART, 0, NULL) != 0) {
+            fprintf(stderr, "photons: partmesh: %s\n", photons_pm_last_error());
+            exit(1);
+        }
+        g_pm_nside = NSIDE;
+        g_pm_box = BOXSIZE;
+        g_pm_rs = splitRadius;
+        g_pm_mass = MASSPART;
+    }
+    if (NPART <= 0) return;
+    if (NPART > g_pm_cap) {
+        free(g_pm_io);
+        g_pm_io = (double *)malloc(sizeof(double) * 6 * (size_t)NPART);
+        if (!g_pm_io) {
+            fprintf(stderr, "photons: partmesh: out of host memory\n");
+            exit(1);
+        }
+        g_pm_cap = NPART;
+    }
+    double *pos = g_pm_io, *acc = g_pm_io + 3 * (size_t)NPART;
+    for (int i = 0; i < NPART; i++)
+        for (int k = 0; k < 3; k++) pos[3 * i + k] = part[i].pos[k];
+    if (photons_pm_force_host(g_pm, NPART, pos, acc) != 0) {
+        fprintf(stderr, "photons: partmesh: %s\n", photons_pm_last_error());
+        exit(1);
+    }
+    for (int i = 0; i < NPART; i++)
+        for (int k = 0; k < 3; k++) part[i].acc_pm[k] = acc[3 * i + k];
+}
+
+void partmesh_thread(void) { partmesh(); }
+
+void *pm_thread(void *arg)
+{
+    (void)arg;
+    partmesh();
+    return NULL;
+}

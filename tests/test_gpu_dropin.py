@@ -238,3 +238,22 @@ def test_compiled_c_driver_matches_reference_golden(name, tmp_path):
     a32, _ = run_driver(pos, prm, str(tmp_path), fp32=True)
     e = np.linalg.norm(a32[g["idx"]] - g["acc"], axis=1) / np.linalg.norm(g["acc"], axis=1)
     assert np.percentile(e, 99) <= 1e-4
+
+
+def test_partmesh_drop_in_matches_the_pm_oracle():
+    """partmesh() with the reference's globals (src/partmesh.c:800-1550) fills
+    part[].acc_pm; checked against the numpy restatement oracle/pm.py"""
+    import pm
+    r = Ref()
+    pos, prm = case_inputs("lattice32_rc6")
+    r.set_params(prm, len(pos))
+    r.i("NSIDE").value = 64
+    body = bodies(pos)
+    r.set_part(body)
+    r.lib.partmesh.restype = None
+    r.lib.partmesh()
+    want = pm.pm_force(pos, prm.mass, 64, prm.boxsize, prm.rs)
+    got = body[:, 9:12]
+    assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
+    # the FMM globals and part[].pos are untouched
+    np.testing.assert_array_equal(body[:, 0:3], pos)
