@@ -342,6 +342,7 @@ int photons_autotune_boxes(photons_ctx *x, int nbox, const int64_t *counts, cons
         Ctx &c = x->c;
         if (reps < 1) return fail_msg("reps must be positive");
         const photons_params keep = c.prm;
+        const int keep_stage = c.p2p_stage;
         const int leaves[3] = {16, 32, 64}, stages[2] = {256, 512};
         const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
         double best = 1e300;
@@ -357,6 +358,7 @@ int photons_autotune_boxes(photons_ctx *x, int nbox, const int64_t *counts, cons
                 if (rc == 0) rc = photons_sync(x);
                 if (rc != 0) {
                     c.prm = keep;
+                    c.p2p_stage = keep_stage;
                     return rc;
                 }
                 const auto t0 = std::chrono::steady_clock::now();
@@ -364,6 +366,7 @@ int photons_autotune_boxes(photons_ctx *x, int nbox, const int64_t *counts, cons
                 if (rc == 0) rc = photons_sync(x);
                 if (rc != 0) {
                     c.prm = keep;
+                    c.p2p_stage = keep_stage;
                     return rc;
                 }
                 const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / reps;

@@ -89,7 +89,6 @@ struct Box {
     double L = 0.0;
     double *pos = nullptr, *vel = nullptr, *acc = nullptr;
     bool busy = false;      // a request of this box is being evaluated
-    bool removed = false;
 };
 
 enum Kind { kStep = 0, kForce = 1 };
@@ -309,11 +308,20 @@ int photons_server_create(photons_server **out, const photons_params *prm, int d
     }
 }
 
+// Waits for the queued and running requests, then stops the lanes.  No other call
+// on the server may run concurrently with or after this one.
 int photons_server_destroy(photons_server *s)
 {
     if (!s) return 0;
     {
-        std::lock_guard<std::mutex> g(s->mu);
+        std::unique_lock<std::mutex> g(s->mu);
+        s->cv_done.wait(g, [&] {
+            if (!s->fatal.empty()) return true;  // the lanes take no more work
+            if (!s->queue.empty()) return false;
+            for (auto &kv : s->boxes)
+                if (kv.second.busy) return false;
+            return true;
+        });
         s->stop = true;
     }
     s->cv_work.notify_all();
@@ -379,15 +387,19 @@ int photons_server_add_box(photons_server *s, int64_t n, double boxsize, const d
 int photons_server_remove_box(photons_server *s, int box_id)
 {
     std::unique_lock<std::mutex> g(s->mu);
-    auto it = s->boxes.find(box_id);
-    if (it == s->boxes.end()) return fail("no such box");
-    // wait until no request of the box is queued or running
+    if (s->boxes.find(box_id) == s->boxes.end()) return fail("no such box");
+    // wait until no request of the box is queued or running (looked up again after
+    // every wake-up: another thread may remove the box meanwhile)
     s->cv_done.wait(g, [&] {
+        auto it = s->boxes.find(box_id);
+        if (it == s->boxes.end()) return true;
         if (it->second.busy) return false;
         for (int64_t q : s->queue)
             if (s->reqs[q].box == box_id) return false;
         return true;
     });
+    auto it = s->boxes.find(box_id);
+    if (it == s->boxes.end()) return fail("no such box");
     Box b = it->second;
     s->boxes.erase(it);
     s->resident -= kResidentBytesPerParticle * (double)b.n;
