@@ -2,9 +2,8 @@
 //
 // The reference's partmesh() with its convolution (src/partmesh.c:800-1550,
 // src/conv.f90:171-229) on one rank, in FP64:
-//   1. CIC deposit, mesh points at cell centres (src/partmesh.c:877-944): one thread
-//      per particle, eight atomic adds into the periodic N^3 mesh, then a scale by
-//      (N/L)^3 (:948-959);
+//   1. CIC deposit, mesh points at cell centres (src/partmesh.c:877-944), into the
+//      periodic N^3 mesh, then a scale by (N/L)^3 (:948-959);
 //   2. hipFFT D2Z, multiply by the Green's function with sinc^-4 deconvolution and
 //      the Gaussian long-range filter, pref = L^2/(pi N^3), gf(0) = pref, and pi as
 //      the reference truncates it (src/conv.f90:142), hipFFT Z2D (unnormalised, as
@@ -13,12 +12,21 @@
 //      reference forms at each particle's 8 CIC points, src/partmesh.c:1328-1550),
 //      then one thread per particle interpolates them with the CIC weights in the
 //      reference's term order.
-// acc_pm is written in the caller's order, without G (the kick applies it).  The
-// deposit's atomic adds make the density's last bits depend on their order.
+// acc_pm is written in the caller's order, without G (the kick applies it).
+// Clustered particles pile onto few cells, so the deposit first sorts the particles
+// by the lower corner of their CIC cloud: a wavefront then sums the contributions of
+// each run of equal corners (segmented scan over shuffles) and one lane per run adds
+// the eight sums atomically.  The interpolation walks the same order, so neighbouring
+// lanes read neighbouring cells.  The atomic adds make the density's last bits depend
+// on their order.
 #include <hip/hip_runtime.h>
 #include <hipfft/hipfft.h>
 
 #include <cmath>
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
 #include <stdexcept>
 #include <string>
 
@@ -85,20 +93,81 @@ __global__ void k_fill(double *p, size_t n, double v)
     if (t < n) p[t] = v;
 }
 
-__global__ void k_deposit(int64_t n, const double *__restrict__ pos, double mass, int N, double norm, double delta,
-                          double *__restrict__ mesh)
+// lower corner of each particle's CIC cloud (x slowest), with the particle index
+__global__ void k_corner_keys(int64_t n, const double *__restrict__ pos, int N, double norm, double delta,
+                              uint32_t *__restrict__ key, int32_t *__restrict__ idx)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const Cic c = cic(pos + 3 * t, N, norm, delta);
-    atomicAdd(&mesh[at(c.i, c.j, c.k, N)], mass * c.win * c.wjn * c.wkn);
-    atomicAdd(&mesh[at(c.ii, c.j, c.k, N)], mass * c.wi * c.wjn * c.wkn);
-    atomicAdd(&mesh[at(c.i, c.jj, c.k, N)], mass * c.win * c.wj * c.wkn);
-    atomicAdd(&mesh[at(c.i, c.j, c.kk, N)], mass * c.win * c.wjn * c.wk);
-    atomicAdd(&mesh[at(c.ii, c.jj, c.k, N)], mass * c.wi * c.wj * c.wkn);
-    atomicAdd(&mesh[at(c.ii, c.j, c.kk, N)], mass * c.wi * c.wjn * c.wk);
-    atomicAdd(&mesh[at(c.i, c.jj, c.kk, N)], mass * c.win * c.wj * c.wk);
-    atomicAdd(&mesh[at(c.ii, c.jj, c.kk, N)], mass * c.wi * c.wj * c.wk);
+    // the cloud covers {lo, lo+1} per axis, lo = i or the neighbour below (wrapped)
+    const int lx = c.ii == (c.i + 1) % N ? c.i : c.ii;
+    const int ly = c.jj == (c.j + 1) % N ? c.j : c.jj;
+    const int lz = c.kk == (c.k + 1) % N ? c.k : c.kk;
+    key[t] = (uint32_t)at(lx, ly, lz, N);
+    idx[t] = (int32_t)t;
+}
+
+// particles in corner order: each wavefront sums runs of equal corners and the run's
+// last lane adds the eight sums (corner terms formed as the reference forms them,
+// mass * wx * wy * wz)
+__global__ void __launch_bounds__(256) k_deposit_runs(int64_t n, const double *__restrict__ pos,
+                                                      const uint32_t *__restrict__ key,
+                                                      const int32_t *__restrict__ idx, double mass, int N,
+                                                      double norm, double delta, double *__restrict__ mesh)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x % 64;
+    const bool live = t < n;
+    const uint32_t k = live ? key[t] : 0xffffffffu;
+    double v[8];
+    int cx = 0, cy = 0, cz = 0;
+    if (live) {
+        const Cic c = cic(pos + 3 * (int64_t)idx[t], N, norm, delta);
+        // weights at lo and lo + 1 per axis
+        const bool ux = c.ii == (c.i + 1) % N, uy = c.jj == (c.j + 1) % N, uz = c.kk == (c.k + 1) % N;
+        const double x0 = ux ? c.win : c.wi, x1 = ux ? c.wi : c.win;
+        const double y0 = uy ? c.wjn : c.wj, y1 = uy ? c.wj : c.wjn;
+        const double z0 = uz ? c.wkn : c.wk, z1 = uz ? c.wk : c.wkn;
+        cx = ux ? c.i : c.ii;
+        cy = uy ? c.j : c.jj;
+        cz = uz ? c.k : c.kk;
+        v[0] = mass * x0 * y0 * z0;
+        v[1] = mass * x1 * y0 * z0;
+        v[2] = mass * x0 * y1 * z0;
+        v[3] = mass * x1 * y1 * z0;
+        v[4] = mass * x0 * y0 * z1;
+        v[5] = mass * x1 * y0 * z1;
+        v[6] = mass * x0 * y1 * z1;
+        v[7] = mass * x1 * y1 * z1;
+    } else {
+        for (int q = 0; q < 8; q++) v[q] = 0.0;
+    }
+    // segmented inclusive scan: keys are sorted, so equal keys l-d and l mean the
+    // whole stretch between them is one run
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t kd = __shfl_up(k, d, 64);
+        const bool same = lane >= d && kd == k;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const double u = __shfl_up(v[q], d, 64);
+            if (same) v[q] += u;
+        }
+    }
+    const uint32_t knext = __shfl_down(k, 1, 64);
+    const bool tail = live && (lane == 63 || knext != k);
+    if (tail) {
+        const int x1 = (cx + 1) % N, y1 = (cy + 1) % N, z1 = (cz + 1) % N;
+        atomicAdd(&mesh[at(cx, cy, cz, N)], v[0]);
+        atomicAdd(&mesh[at(x1, cy, cz, N)], v[1]);
+        atomicAdd(&mesh[at(cx, y1, cz, N)], v[2]);
+        atomicAdd(&mesh[at(x1, y1, cz, N)], v[3]);
+        atomicAdd(&mesh[at(cx, cy, z1, N)], v[4]);
+        atomicAdd(&mesh[at(x1, cy, z1, N)], v[5]);
+        atomicAdd(&mesh[at(cx, y1, z1, N)], v[6]);
+        atomicAdd(&mesh[at(x1, y1, z1, N)], v[7]);
+    }
 }
 
 __global__ void k_scale(double *p, size_t n, double s)
@@ -165,16 +234,19 @@ __device__ __forceinline__ double interp(const double *g, const Cic &c, int N)
            c.win * c.wj * c.wk * g[at(c.i, c.jj, c.kk, N)] + c.wi * c.wj * c.wk * g[at(c.ii, c.jj, c.kk, N)];
 }
 
-__global__ void k_interpolate(int64_t n, const double *__restrict__ pos, int N, double norm, double delta,
-                              const double *__restrict__ gx, const double *__restrict__ gy,
-                              const double *__restrict__ gz, double *__restrict__ acc)
+// particles visited in corner order (idx), results written in the caller's order
+__global__ void k_interpolate(int64_t n, const double *__restrict__ pos, const int32_t *__restrict__ idx, int N,
+                              double norm, double delta, const double *__restrict__ gx,
+                              const double *__restrict__ gy, const double *__restrict__ gz,
+                              double *__restrict__ acc)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const Cic c = cic(pos + 3 * t, N, norm, delta);
-    acc[3 * t] = interp(gx, c, N);
-    acc[3 * t + 1] = interp(gy, c, N);
-    acc[3 * t + 2] = interp(gz, c, N);
+    const int64_t p = idx[t];
+    const Cic c = cic(pos + 3 * p, N, norm, delta);
+    acc[3 * p] = interp(gx, c, N);
+    acc[3 * p + 1] = interp(gy, c, N);
+    acc[3 * p + 2] = interp(gz, c, N);
 }
 
 inline int blocks(size_t n, int b) { return (int)((n + b - 1) / b); }
@@ -190,6 +262,12 @@ struct photons_pm {
     double *mesh = nullptr, *grad = nullptr, *io_pos = nullptr, *io_acc = nullptr;
     hipfftDoubleComplex *spec = nullptr;
     int64_t io_cap = 0;
+    // corner sort: keys and particle indices (two halves each), rocPRIM temporary
+    uint32_t *key = nullptr;
+    int32_t *idx = nullptr;
+    void *sort_tmp = nullptr;
+    size_t sort_tmp_bytes = 0;
+    int64_t sort_cap = 0;
 };
 
 extern "C" {
@@ -242,7 +320,8 @@ int photons_pm_destroy(photons_pm *p)
     if (p->stream) (void)hipStreamSynchronize(p->stream);
     if (p->fwd) hipfftDestroy(p->fwd);
     if (p->inv) hipfftDestroy(p->inv);
-    for (void *b : {(void *)p->mesh, (void *)p->grad, (void *)p->spec, (void *)p->io_pos, (void *)p->io_acc})
+    for (void *b : {(void *)p->mesh, (void *)p->grad, (void *)p->spec, (void *)p->io_pos, (void *)p->io_acc,
+                    (void *)p->key, (void *)p->idx, p->sort_tmp})
         if (b) (void)hipFree(b);
     if (p->own_stream && p->stream) (void)hipStreamDestroy(p->stream);
     delete p;
@@ -255,14 +334,41 @@ int photons_pm_force_device(photons_pm *p, int64_t n, const double *d_pos, doubl
 {
     try {
         if (n < 0) return fail("negative particle count");
+        if (n > INT32_MAX) return fail("more than 2^31 particles");
         PM_CHECK(hipSetDevice(p->device));
         if (n == 0) return 0;
         const int N = p->N;
         const size_t n3 = (size_t)N * N * N, nc = (size_t)N * N * (N / 2 + 1);
         const double norm = N / p->L, delta = 1.0 / norm;
         hipStream_t st = p->stream;
+        if (n > p->sort_cap) {
+            for (void *b : {(void *)p->key, (void *)p->idx}) if (b) PM_CHECK(hipFree(b));
+            p->key = nullptr;
+            p->idx = nullptr;
+            p->sort_cap = 0;
+            PM_CHECK(hipMalloc(&p->key, sizeof(uint32_t) * 2 * (size_t)n));
+            PM_CHECK(hipMalloc(&p->idx, sizeof(int32_t) * 2 * (size_t)n));
+            p->sort_cap = n;
+        }
+        k_corner_keys<<<blocks(n, 256), 256, 0, st>>>(n, d_pos, N, norm, delta, p->key, p->idx);
+        rocprim::double_buffer<uint32_t> kb(p->key, p->key + p->sort_cap);
+        rocprim::double_buffer<int32_t> vb(p->idx, p->idx + p->sort_cap);
+        unsigned bits = 1;
+        while (bits < 32 && ((uint64_t)1 << bits) < (uint64_t)n3) bits++;
+        size_t tmp_bytes = 0;
+        PM_CHECK(rocprim::radix_sort_pairs(nullptr, tmp_bytes, kb, vb, (size_t)n, 0, bits, st));
+        if (tmp_bytes > p->sort_tmp_bytes) {
+            if (p->sort_tmp) PM_CHECK(hipFree(p->sort_tmp));
+            p->sort_tmp = nullptr;
+            p->sort_tmp_bytes = 0;
+            PM_CHECK(hipMalloc(&p->sort_tmp, tmp_bytes));
+            p->sort_tmp_bytes = tmp_bytes;
+        }
+        PM_CHECK(rocprim::radix_sort_pairs(p->sort_tmp, tmp_bytes, kb, vb, (size_t)n, 0, bits, st));
+        const uint32_t *skey = kb.current();
+        const int32_t *sidx = vb.current();
         k_fill<<<blocks(n3, 256), 256, 0, st>>>(p->mesh, n3, 0.0);
-        k_deposit<<<blocks(n, 256), 256, 0, st>>>(n, d_pos, p->mass, N, norm, delta, p->mesh);
+        k_deposit_runs<<<blocks(n, 256), 256, 0, st>>>(n, d_pos, skey, sidx, p->mass, N, norm, delta, p->mesh);
         k_scale<<<blocks(n3, 256), 256, 0, st>>>(p->mesh, n3, norm * norm * norm);
         FFT_CHECK(hipfftExecD2Z(p->fwd, p->mesh, p->spec));
         const double ismth2 = (2 * kPiTrunc * p->rs / p->L) * (2 * kPiTrunc * p->rs / p->L);
@@ -271,7 +377,7 @@ int photons_pm_force_device(photons_pm *p, int64_t n, const double *d_pos, doubl
         FFT_CHECK(hipfftExecZ2D(p->inv, p->spec, p->mesh));
         k_gradient<<<blocks(n3, 256), 256, 0, st>>>(p->mesh, N, 0.5 * N / p->L, p->grad, p->grad + n3,
                                                     p->grad + 2 * n3);
-        k_interpolate<<<blocks(n, 256), 256, 0, st>>>(n, d_pos, N, norm, delta, p->grad, p->grad + n3,
+        k_interpolate<<<blocks(n, 256), 256, 0, st>>>(n, d_pos, sidx, N, norm, delta, p->grad, p->grad + n3,
                                                       p->grad + 2 * n3, d_acc);
         PM_CHECK(hipGetLastError());
         return 0;
