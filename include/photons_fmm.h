@@ -206,6 +206,22 @@ int photons_pm_force_device(photons_pm *pm, int64_t n, const double *d_pos, doub
 int photons_pm_force_host(photons_pm *pm, int64_t n, const double *pos, double *acc);
 int photons_pm_sync(photons_pm *pm);
 
+/* ---- one TreePM step on device-resident particles --------------------------
+ * Replaces one pass of the reference's time loop (src/photoNs.c:136-254) with
+ * the particles kept in HBM: v += acc_pm dkh; v += acc dkh; x += v dd; wrap into
+ * [0, boxsize); acc = short-range force (fmm_construct .. fmm_ext, the tree
+ * rebuilt as the reference does every step); acc_pm = PM force (partmesh, run on
+ * the PM handle's stream alongside); v += acc dkh; v += acc_pm dkh.
+ * dkh = G kick/2 and dd = drift over the step's ln a interval
+ * (photons_kick_factor / photons_drift_factor).  pm NULL: no long-range part
+ * (d_acc_pm is then unused).  The PM handle must be for the context's boxsize.
+ * Before the first step acc (and acc_pm) hold the forces of the initial
+ * positions, as the reference computes them before its loop
+ * (src/photoNs.c:97-116).  Returns with the last kick queued on the context's
+ * stream (photons_sync waits).                                               */
+int photons_step_device(photons_ctx *ctx, photons_pm *pm, int64_t n, double *d_pos, double *d_vel, double *d_acc,
+                        double *d_acc_pm, double dkh, double dd);
+
 /* ---- Gadget-2 snapshot/IC files (host only) -------------------------------
  * The reference's format and conventions (src/snapshot.c:5-22, 64-120, 211-293,
  * 397-503): 256-byte header, float32 positions, float32 velocities stored as

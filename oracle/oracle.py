@@ -139,6 +139,32 @@ def step(pos, vel, acc, prm, dkh, dd):
     return pos, vel, acc
 
 
+def wrap(x, boxsize):
+    """src/photoNs.c:178-196: add or subtract the box length until x is in [0, L)"""
+    x = np.array(x, dtype=np.float64)
+    while True:
+        lo, hi = x < 0.0, x >= boxsize
+        if not (lo.any() or hi.any()):
+            return x
+        x = np.where(lo, x + boxsize, np.where(hi, x - boxsize, x))
+
+
+def treepm_step(pos, vel, acc, acc_pm, prm, pm_force, dkh, dd):
+    """one pass of the reference's time loop with both force parts
+    (src/photoNs.c:158-254): v += acc_pm dkh; v += acc dkh; x += v dd; wrap; new acc
+    (the FMM force of this module) and acc_pm (pm_force(pos), e.g. oracle.pm);
+    v += acc dkh; v += acc_pm dkh.  Returns new (pos, vel, acc, acc_pm)."""
+    vel = np.array(vel, dtype=np.float64)
+    vel = vel + np.asarray(acc_pm) * dkh
+    vel = vel + np.asarray(acc) * dkh
+    pos = wrap(np.asarray(pos, dtype=np.float64) + vel * dd, params(prm).boxsize)
+    acc, _ = force(pos, prm)
+    acc_pm = np.asarray(pm_force(pos), dtype=np.float64)
+    vel = vel + acc * dkh
+    vel = vel + acc_pm * dkh
+    return pos, vel, acc, acc_pm
+
+
 class Tree:
     def __init__(self, pos, prm):
         self.pos = np.ascontiguousarray(pos, dtype=np.float64)

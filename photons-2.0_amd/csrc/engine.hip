@@ -18,6 +18,7 @@
 
 #include "engine.h"
 #include "fmm_math.h"
+#include "kick.h"
 
 namespace photons {
 void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, double *out);
@@ -474,6 +475,45 @@ int photons_evaluate_host(photons_ctx *x, double *acc_tree_order)
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_evaluate_host", e);
+    }
+}
+
+// One step of the reference's time loop (src/photoNs.c:136-254) on device-resident
+// particles: half kicks with the previous forces (acc_pm, then acc), drift, wrap, a new
+// tree and short-range force (the reference rebuilds every step), the PM force on the
+// PM handle's stream at the same time, half kicks with the new forces (acc, then acc_pm).
+int photons_step_device(photons_ctx *x, photons_pm *pm, int64_t n, double *d_pos, double *d_vel, double *d_acc,
+                        double *d_acc_pm, double dkh, double dd)
+{
+    try {
+        if (!x) return fail_msg("photons_step_device: null context");
+        Ctx &c = x->c;
+        std::string m = check_n(c, n);
+        if (!m.empty()) return fail_msg(m);
+        if (n == 0) return 0;
+        if (!d_pos || !d_vel || !d_acc) return fail_msg("photons_step_device: pos, vel and acc are required");
+        if (pm && !d_acc_pm) return fail_msg("photons_step_device: a PM handle needs the acc_pm array");
+        PH_CHECK(hipSetDevice(c.device));
+        const int64_t n3 = 3 * n;
+        const double *apm = pm ? d_acc_pm : nullptr;
+        k_kick_drift<<<kick_blocks(n3), 256, 0, c.stream>>>(n3, d_pos, d_vel, d_acc, apm, dkh, dd, c.prm.boxsize, 1,
+                                                           nullptr);
+        PH_CHECK(hipGetLastError());
+        if (pm) {
+            // the PM reads the new positions on its own stream
+            PH_CHECK(hipStreamSynchronize(c.stream));
+            if (photons_pm_force_device(pm, n, d_pos, d_acc_pm) != 0)
+                return fail_msg(std::string("photons_step_device: PM: ") + photons_pm_last_error());
+        }
+        const int rc = photons_force_device(x, n, d_pos, d_acc);
+        if (pm && photons_pm_sync(pm) != 0 && rc == 0)
+            return fail_msg(std::string("photons_step_device: PM: ") + photons_pm_last_error());
+        if (rc != 0) return rc;
+        k_kick<<<kick_blocks(n3), 256, 0, c.stream>>>(n3, d_vel, d_acc, nullptr, apm, dkh, 1);
+        PH_CHECK(hipGetLastError());
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_step_device", e);
     }
 }
 

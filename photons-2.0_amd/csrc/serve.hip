@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "../../include/photons_fmm.h"
+#include "kick.h"
 
 namespace {
 
@@ -43,45 +44,6 @@ constexpr double kResidentBytesPerParticle = 72.0;  // pos, vel, acc (3 doubles 
 double now_s()
 {
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-// reference wrap (src/photoNs.c:244-260); far outliers are folded first so the
-// loops stay short
-__device__ __forceinline__ double wrap(double x, double L)
-{
-    if (!(fabs(x) < 4.0 * L)) x -= floor(x / L) * L;
-    while (x < 0.0) x += L;
-    while (x >= L) x -= L;
-    return x;
-}
-
-// first half kick with the previous force, drift, wrap; writes the new position to
-// the box and to the batch input
-__global__ void k_kick_drift(int64_t n3, double *__restrict__ pos, double *__restrict__ vel,
-                             const double *__restrict__ acc, double dkh, double dd, double L, int step,
-                             double *__restrict__ out)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n3) return;
-    double x = pos[i];
-    if (step) {
-        const double v = vel[i] + acc[i] * dkh;
-        vel[i] = v;
-        x = wrap(x + v * dd, L);
-        pos[i] = x;
-    }
-    out[i] = x;
-}
-
-// the new force becomes the box's acc; second half kick
-__global__ void k_kick(int64_t n3, double *__restrict__ vel, double *__restrict__ acc, const double *__restrict__ in,
-                       double dkh, int step)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n3) return;
-    const double a = in[i];
-    acc[i] = a;
-    if (step) vel[i] += a * dkh;
 }
 
 struct Box {
@@ -161,8 +123,8 @@ void run_batch(photons_server *s, Lane &ln, const std::vector<int64_t> &ids)
     for (size_t k = 0; k < ids.size(); k++) {
         const int64_t n3 = 3 * bx[k].n;
         const int step = rq[k].kind == kStep;
-        k_kick_drift<<<(int)((n3 + 255) / 256), 256, 0, ln.stream>>>(n3, bx[k].pos, bx[k].vel, bx[k].acc, rq[k].dkh,
-                                                                    rq[k].dd, bx[k].L, step, ln.in + 3 * off);
+        k_kick_drift<<<kick_blocks(n3), 256, 0, ln.stream>>>(n3, bx[k].pos, bx[k].vel, bx[k].acc, nullptr, rq[k].dkh,
+                                                            rq[k].dd, bx[k].L, step, ln.in + 3 * off);
         counts.push_back(bx[k].n);
         lengths.push_back(bx[k].L);
         off += bx[k].n;
@@ -173,8 +135,8 @@ void run_batch(photons_server *s, Lane &ln, const std::vector<int64_t> &ids)
     off = 0;
     for (size_t k = 0; k < ids.size(); k++) {
         const int64_t n3 = 3 * bx[k].n;
-        k_kick<<<(int)((n3 + 255) / 256), 256, 0, ln.stream>>>(n3, bx[k].vel, bx[k].acc, ln.out + 3 * off,
-                                                              rq[k].dkh, rq[k].kind == kStep);
+        k_kick<<<kick_blocks(n3), 256, 0, ln.stream>>>(n3, bx[k].vel, bx[k].acc, ln.out + 3 * off, nullptr,
+                                                      rq[k].dkh, rq[k].kind == kStep);
         off += bx[k].n;
     }
     SV_CHECK(hipGetLastError());

@@ -177,6 +177,16 @@ class Context:
                                                 _dp(lengths), C.c_void_p(d_pos), C.c_void_p(d_acc)),
                "photons_force_boxes_device")
 
+    def step_device(self, pm, n, d_pos, d_vel, d_acc, d_acc_pm, dkh, dd):
+        """one TreePM step of [n][3] device arrays (pointers as ints): half kicks with
+        the previous acc_pm and acc, drift, wrap, new acc and acc_pm, half kicks; pm may
+        be None (no long-range part, d_acc_pm unused).  Queued on the context stream."""
+        L = _pm_lib()
+        _check(L.photons_step_device(self.h, pm.h if pm is not None else None, int(n), C.c_void_p(d_pos),
+                                     C.c_void_p(d_vel), C.c_void_p(d_acc),
+                                     C.c_void_p(d_acc_pm) if d_acc_pm else None, float(dkh), float(dd)),
+               "photons_step_device")
+
     def sync(self):
         _check(lib().photons_sync(self.h), "photons_sync")
 
@@ -441,6 +451,7 @@ def _pm_lib():
         L.photons_pm_force_device.argtypes = [vp, C.c_int64, vp, vp]
         L.photons_pm_force_host.argtypes = [vp, C.c_int64, dp, dp]
         L.photons_pm_sync.argtypes = [vp]
+        L.photons_step_device.argtypes = [vp, vp, C.c_int64, vp, vp, vp, vp, C.c_double, C.c_double]
         L._pm_types = True
     return L
 

@@ -92,3 +92,31 @@ def test_c_driver_fails_loudly_without_a_gpu(tmp_path):
     r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "GPU" in (r.stdout + r.stderr)
+
+
+def test_run_program_fails_loudly_without_a_gpu(tmp_path):
+    """the reference-style run program reads the parameter file and the IC, then stops
+    with a message when there is no GPU; a bad parameter file stops it earlier"""
+    import numpy as np
+    import photons_gpu as pg
+    exe = os.path.join(ROOT, "photons-2.0_amd", "examples", "photons_run")
+    if not os.path.exists(exe):
+        pytest.skip("run program not built")
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    pos = np.random.default_rng(0).random((4096, 3)) * 1000.0
+    ic_path = tmp_path / "ic.gdt2"
+    pg.snapshot_write(ic_path, pos, np.zeros_like(pos), boxsize=1000.0, redshift=9.0, mass=1.0)
+    prm = tmp_path / "run.param"
+    prm.write_text(f"InputPath {ic_path}\nOutputPath {tmp_path}\nOutputName out\nNumMeshSide 16\n")
+    r = subprocess.run([exe, str(prm), "--nstep", "1"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
+    assert "GPU" in (r.stdout + r.stderr)
+    assert not (tmp_path / "out_3.0").exists()
+    prm.write_text(f"InputPath {ic_path}\nSnapFormat -2\nNumMeshSide 16\n")
+    r = subprocess.run([exe, str(prm)], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0 and "SnapFormat" in r.stderr

@@ -70,3 +70,43 @@ def test_step_uses_the_old_force_then_the_new_one(box):
     assert np.allclose(p1, np.mod(pos + vh * dd, prm.boxsize), rtol=0, atol=1e-9 * prm.boxsize)
     a_new, _ = oracle.force(p1, prm)
     assert np.allclose(v1, vh + a_new * dkh, rtol=1e-14, atol=0)
+
+
+def test_treepm_step_without_pm_equals_the_c_step(box):
+    # the numpy TreePM step with no long-range part is the C restatement's step, bit for bit
+    pos, vel, acc0, prm = box
+    dkh, dd = 1e-3, 2.0
+    want = oracle.step(pos, vel, acc0, prm, dkh, dd)
+    zero = np.zeros_like(pos)
+    got = oracle.treepm_step(pos, vel, acc0, zero, prm, lambda p: np.zeros_like(p), dkh, dd)
+    for g, w in zip(got[:3], want):
+        assert np.array_equal(g, w)
+    assert not got[3].any()
+
+
+def test_treepm_step_puts_the_long_range_kicks_around_the_short_range_ones(box):
+    # a uniform long-range field that changes between steps: the old one kicks before
+    # the drift, the new one after the new short-range force
+    pos, vel, acc0, prm = box
+    amax = np.abs(acc0).max()
+    g_old = np.array([0.5, -1.0, 0.25]) * amax
+    g_new = np.array([3.0, -2.0, 1.0]) * amax
+    apm0 = np.broadcast_to(g_old, pos.shape)
+    dkh, dd = 1e-3, 2.0
+    p1, v1, a1, apm1 = oracle.treepm_step(pos, vel, acc0, apm0, prm, lambda p: np.broadcast_to(g_new, p.shape),
+                                          dkh, dd)
+    vh = vel + g_old * dkh + acc0 * dkh
+    assert np.allclose(p1, np.mod(pos + vh * dd, prm.boxsize), rtol=0, atol=1e-9 * prm.boxsize)
+    a_new, _ = oracle.force(p1, prm)
+    assert np.array_equal(a1, a_new)
+    assert np.array_equal(apm1, np.broadcast_to(g_new, pos.shape))
+    assert np.allclose(v1, vh + (a_new + g_new) * dkh, rtol=1e-13, atol=1e-13 * np.abs(v1).max())
+
+
+def test_wrap_folds_every_coordinate_into_the_box():
+    L = 7.0
+    x = np.array([-20.5, -7.0, -1e-17, 0.0, 6.999999, 7.0, 13.9, 30.0])
+    w = oracle.wrap(x, L)
+    assert ((w >= 0) & (w < L)).all()
+    d = np.abs(w - np.mod(x, L))
+    assert (np.minimum(d, L - d) <= 1e-12).all()  # -1e-17 + L rounds to L, which wraps to 0
