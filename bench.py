@@ -38,7 +38,7 @@ P2P_FLOPS = 27  # per particle-particle interaction
 # HBM bytes per FP32 P2P launch at 256^3 clustered, p = 4, from rocprofv3 --pmc
 # FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy in the same run);
 # the counts and commands are kept with the P2P counter summary under profiles/
-P2P_FP32_TRAFFIC_256 = (2.0 * 15100000 + 784000) * 1024
+P2P_FP32_TRAFFIC_256 = (2.0 * 21180026 + 1174366) * 1024
 
 
 def log(msg):
@@ -321,8 +321,10 @@ def main():
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": (P2P_FP32_TRAFFIC_256 if fp32 and args.nside == 256 and prm.order == 4 else None),
                      "note": "27 flops per interaction x interactions / mean P2P kernel time (HIP events on the "
-                             "library's stream); compute bound on the vector ALU (3 of ~31 instructions per "
-                             "interaction are quarter-rate transcendentals); traffic = HBM bytes per launch "
+                             "library's stream); compute bound on the vector ALU, where 2.5 of the ~30 "
+                             "instructions per interaction (rsqrt, exp2, half a shared rcp) go to the "
+                             "transcendental unit that paces the loop; packed FP32 is not used, so 78.6 of "
+                             "the 157.3 TFLOP/s peak is reachable at most; traffic = HBM bytes per launch "
                              "from rocprofv3 PMC"},
     }
     if "fp32" in res and "fp64" in res:

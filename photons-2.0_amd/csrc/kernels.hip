@@ -265,11 +265,14 @@ __device__ __forceinline__ void store_acc(const P2PArgs &a, int i, double ax, do
     a.out[o + 2] = az;
 }
 
-// FP32 pair force per interaction, about 30 VALU instructions:
+// FP32 pair force per interaction, about 28 VALU instructions:
 //   r2 = |d|^2 + 1e-30 (keeps rsqrt finite for coincident particles, whose d = 0)
 //   1/max(r, eps)^3 = min(rsqrt(r2), 1/eps)^3   (the reference's softening branch)
 //   G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2) = exp(-u^2) (erfcx(u) + 2u/sqrt(pi)),
-//          u = r/(2 rs), exp(-u^2) straight from r2, erfcx(u) = t Q7(t), t = 1/(1+u/2)
+//          u = r/(2 rs), exp(-u^2) straight from r2, erfcx(u) = P6(t), t = 1/(1+u/2)
+// Three of them (rsqrt, rcp, exp2) go to the transcendental unit, which bounds the
+// loop more than the FMAs do: measured at 256^3, dropping two FMAs per pair changed
+// nothing, dropping the exp2 saved 12% of the P2P time.
 // The particle mass multiplies the sum once at the end.
 struct PairConsts {
     float ieps, inv4rs, c2u, kexp;
@@ -285,12 +288,43 @@ __device__ __forceinline__ void pair_f32(const PairConsts &k, float dx, float dy
     const float rc = __int_as_float(min(__float_as_int(ri), __float_as_int(k.ieps)));
     const float r = r2 * ri;
     const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(r, k.inv4rs, 1.0f));
-    const float R = __builtin_fmaf(r, k.c2u, erfcx_tq7(t));
+    const float R = __builtin_fmaf(r, k.c2u, erfcx_t6(t));
     const float e = __builtin_amdgcn_exp2f(r2 * k.kexp);
     const float f = rc * rc * rc * (e * R);
     ax = __builtin_fmaf(dx, f, ax);
     ay = __builtin_fmaf(dy, f, ay);
     az = __builtin_fmaf(dz, f, az);
+}
+
+// Two interactions of one target with a shared reciprocal: 1/d1 = d2/(d1 d2) and
+// 1/d2 = d1/(d1 d2) from one v_rcp_f32 (d in [1, ~50], so the product stays far from
+// overflow; about 2 ulp in t).  The transcendental unit, not the FMAs, bounds this loop,
+// so trading one of its six operations for three multiplies pays.
+__device__ __forceinline__ void pair2_f32(const PairConsts &k, float dx, float dy, float dz, float ex, float ey,
+                                          float ez, float &ax, float &ay, float &az)
+{
+    const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, 1e-30f)));
+    const float s2 = __builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, __builtin_fmaf(ez, ez, 1e-30f)));
+    const float ri = __builtin_amdgcn_rsqf(r2);
+    const float si = __builtin_amdgcn_rsqf(s2);
+    const float rc = __int_as_float(min(__float_as_int(ri), __float_as_int(k.ieps)));
+    const float sc = __int_as_float(min(__float_as_int(si), __float_as_int(k.ieps)));
+    const float r = r2 * ri;
+    const float s = s2 * si;
+    const float dr = __builtin_fmaf(r, k.inv4rs, 1.0f);
+    const float ds = __builtin_fmaf(s, k.inv4rs, 1.0f);
+    const float q = __builtin_amdgcn_rcpf(dr * ds);
+    const float tr = ds * q;
+    const float ts = dr * q;
+    const float Rr = __builtin_fmaf(r, k.c2u, erfcx_t6(tr));
+    const float Rs = __builtin_fmaf(s, k.c2u, erfcx_t6(ts));
+    const float er = __builtin_amdgcn_exp2f(r2 * k.kexp);
+    const float es = __builtin_amdgcn_exp2f(s2 * k.kexp);
+    const float fr = rc * rc * rc * (er * Rr);
+    const float fs = sc * sc * sc * (es * Rs);
+    ax = __builtin_fmaf(ex, fs, __builtin_fmaf(dx, fr, ax));
+    ay = __builtin_fmaf(ey, fs, __builtin_fmaf(dy, fr, ay));
+    az = __builtin_fmaf(ez, fs, __builtin_fmaf(dz, fr, az));
 }
 
 // Source particles staged in LDS per batch, as float offsets from the target leaf's
@@ -335,8 +369,10 @@ __device__ __forceinline__ LaneMap lane_map(int n, int lane)
 // from the target centre, all loads of a batch issued together.  The pairs of the
 // batch are then spread over the lanes by the two-pass layout above.  Partial sums
 // are combined per target in a fixed lane order, so results are deterministic.
+// Six waves per SIMD (at most 80 VGPRs; the compiler spills one 8-byte value outside the
+// pair loops) hide the transcendental latency better than the five that 95 VGPRs allow.
 template <int P, int STAGE>
-__global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_p2p_f32(P2PArgs a)
 {
     static_assert(STAGE >= 128 && STAGE % 64 == 0, "stage size");
     __shared__ float stage[3 * STAGE];
@@ -421,18 +457,18 @@ __global__ void __launch_bounds__(64) k_p2p_f32(P2PArgs a)
         {
             int q = L.kA;
             for (; q + L.sA < tot; q += 2 * L.sA) {
-                pair_f32(pc, sx[q] - txA, sy[q] - tyA, sz[q] - tzA, axA, ayA, azA);
                 const int q2 = q + L.sA;
-                pair_f32(pc, sx[q2] - txA, sy[q2] - tyA, sz[q2] - tzA, axA, ayA, azA);
+                pair2_f32(pc, sx[q] - txA, sy[q] - tyA, sz[q] - tzA, sx[q2] - txA, sy[q2] - tyA,
+                          sz[q2] - tzA, axA, ayA, azA);
             }
             if (q < tot) pair_f32(pc, sx[q] - txA, sy[q] - tyA, sz[q] - tzA, axA, ayA, azA);
         }
         if (L.e) {
             int q = L.kB;
             for (; q + L.sB < tot; q += 2 * L.sB) {
-                pair_f32(pc, sx[q] - txB, sy[q] - tyB, sz[q] - tzB, axB, ayB, azB);
                 const int q2 = q + L.sB;
-                pair_f32(pc, sx[q2] - txB, sy[q2] - tyB, sz[q2] - tzB, axB, ayB, azB);
+                pair2_f32(pc, sx[q] - txB, sy[q] - tyB, sz[q] - tzB, sx[q2] - txB, sy[q2] - tyB,
+                          sz[q2] - tzB, axB, ayB, azB);
             }
             if (q < tot) pair_f32(pc, sx[q] - txB, sy[q] - tyB, sz[q] - tzB, axB, ayB, azB);
         }
