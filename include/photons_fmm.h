@@ -258,6 +258,11 @@ typedef struct {
 
 int photons_compat_set_precision(int precision);
 photons_ctx *photons_compat_context(void);
+/* The drop-in entry points keep the reference's void signatures and its failure
+ * behaviour (a message, then exit: src/fmm.c:411-414, src/utility.c:78-81); the
+ * message of the failure stays readable here, e.g. from an atexit handler.
+ * Empty while nothing has failed.                                              */
+const char *photons_gpu_last_error(void);
 
 /* the force call, inc/fmm.h:11-22 and src/fmm.c:180-264, 329-360, 719-1087:
  *   fmm_construct   domain box [0, BOXSIZE)^3 (one rank)

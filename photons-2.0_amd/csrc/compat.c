@@ -19,6 +19,7 @@
  * of the native layer is available from photons_last_error().
  */
 #include <math.h>
+#include <stdarg.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -77,11 +78,26 @@ static ref_node *g_node_mem;
 static int g_nleaf, g_nnode;
 static int g_shift_id;
 
-static void die(const char *where)
+/* the reference's failure behaviour (src/fmm.c:411-414, src/utility.c:78-81): a message,
+ * then exit; the message stays readable through photons_gpu_last_error() for exit
+ * handlers */
+static char g_fatal[1024];
+
+const char *photons_gpu_last_error(void) { return g_fatal; }
+
+static void fatal(const char *fmt, ...) __attribute__((noreturn, format(printf, 1, 2)));
+static void fatal(const char *fmt, ...)
 {
-    fprintf(stderr, "photons: %s: %s\n", where, photons_last_error());
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_fatal, sizeof g_fatal, fmt, ap);
+    va_end(ap);
+    fprintf(stderr, "photons: %s\n", g_fatal);
+    fflush(stderr);
     exit(1);
 }
+
+static void die(const char *where) { fatal("%s: %s", where, photons_last_error()); }
 
 static void check(int rc, const char *where)
 {
@@ -127,8 +143,7 @@ static int to_cell(int idx)
 {
     if (idx >= first_leaf && idx < last_leaf) return idx - first_leaf;
     if (idx >= first_node && idx <= last_node) return g_nleaf + (idx - first_node);
-    fprintf(stderr, "photons: index %d is neither a leaf nor a node\n", idx);
-    exit(1);
+    fatal("index %d is neither a leaf nor a node", idx);
 }
 
 static int to_ref(int cell) { return cell < g_nleaf ? first_leaf + cell : first_node + (cell - g_nleaf); }
@@ -160,8 +175,7 @@ void fmm_construct(void)
 {
     /* one rank: the domain is the whole periodic box [0, BOXSIZE)^3 (src/fmm.c:329-360) */
     if (!(BOXSIZE > 0.0)) {
-        fprintf(stderr, "photons: fmm_construct: BOXSIZE must be positive\n");
-        exit(1);
+        fatal("fmm_construct: BOXSIZE must be positive");
     }
 }
 
@@ -390,8 +404,7 @@ static void pair_p2p(int inode, int jnode, int precision)
     int32_t t = inode - first_leaf, s = jnode - first_leaf;
     int8_t sh = 0;
     if (t < 0 || t >= g_nleaf || s < 0 || s >= g_nleaf) {
-        fprintf(stderr, "photons: p2p_kernel needs two leaves\n");
-        exit(1);
+        fatal("p2p_kernel needs two leaves");
     }
     double *acc = (double *)calloc(3 * (size_t)NPART, sizeof(double));
     check(photons_p2p_pairs(x, 1, &t, &s, &sh, precision, acc), "p2p_kernel");
@@ -445,8 +458,7 @@ void walk_task_m2l(int im, int jm) { walk_eval(im, jm, 0, 1, 0, 1); }
 static void need_root_source(int jm)
 {
     if (jm != 0) {
-        fprintf(stderr, "photons: image walks start at the image tree root (jm = 0)\n");
-        exit(1);
+        fatal("image walks start at the image tree root (jm = 0)");
     }
 }
 
@@ -466,8 +478,7 @@ void fmm_remote(int idx, double shift[3])
 {
     /* one rank: idx 0 is the home rank; shift is one of the 26 box periods */
     if (idx != 0) {
-        fprintf(stderr, "photons: fmm_remote: one rank, idx must be 0\n");
-        exit(1);
+        fatal("fmm_remote: one rank, idx must be 0");
     }
     int m[3];
     for (int d = 0; d < 3; d++) m[d] = (int)lround(shift[d] / BOXSIZE);
@@ -494,8 +505,7 @@ void partmesh(void)
         if (g_pm) photons_pm_destroy(g_pm);
         g_pm = NULL;
         if (photons_pm_create(&g_pm, NSIDE, BOXSIZE, splitRadius, MASSPART, 0, NULL) != 0) {
-            fprintf(stderr, "photons: partmesh: %s\n", photons_pm_last_error());
-            exit(1);
+            fatal("partmesh: %s", photons_pm_last_error());
         }
         g_pm_nside = NSIDE;
         g_pm_box = BOXSIZE;
@@ -507,8 +517,7 @@ void partmesh(void)
         free(g_pm_io);
         g_pm_io = (double *)malloc(sizeof(double) * 6 * (size_t)NPART);
         if (!g_pm_io) {
-            fprintf(stderr, "photons: partmesh: out of host memory\n");
-            exit(1);
+            fatal("partmesh: out of host memory");
         }
         g_pm_cap = NPART;
     }
@@ -516,8 +525,7 @@ void partmesh(void)
     for (int i = 0; i < NPART; i++)
         for (int k = 0; k < 3; k++) pos[3 * i + k] = part[i].pos[k];
     if (photons_pm_force_host(g_pm, NPART, pos, acc) != 0) {
-        fprintf(stderr, "photons: partmesh: %s\n", photons_pm_last_error());
-        exit(1);
+        fatal("partmesh: %s", photons_pm_last_error());
     }
     for (int i = 0; i < NPART; i++)
         for (int k = 0; k < 3; k++) part[i].acc_pm[k] = acc[3 * i + k];

@@ -23,6 +23,13 @@ extern int NPART, MAXLEAF, first_leaf, last_leaf, first_node, last_node;
 extern double MASSPART, SoftenScale, splitRadius, cutoffRadius, open_angle, BOXSIZE;
 extern unsigned long idxP2P, idxM2L, walk_m2l_count, p2p_count_remote;
 
+/* the drop-in layer exits on a fatal error, as the reference does; report its message */
+static void report(void)
+{
+    const char *m = photons_gpu_last_error();
+    if (m[0]) fprintf(stderr, "fmm_driver: the force path stopped: %s\n", m);
+}
+
 static void die(const char *m)
 {
     fprintf(stderr, "fmm_driver: %s\n", m);
@@ -32,6 +39,7 @@ static void die(const char *m)
 int main(int argc, char **argv)
 {
     if (argc < 3) die("usage: fmm_driver IN OUT [fp32]");
+    atexit(report);
     FILE *f = fopen(argv[1], "rb");
     if (!f) die("cannot open the input");
     int32_t hdr[4];

@@ -92,6 +92,8 @@ def test_c_driver_fails_loudly_without_a_gpu(tmp_path):
     r = subprocess.run([exe, str(fin), str(fout)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0
     assert "GPU" in (r.stdout + r.stderr)
+    # the message stays readable through photons_gpu_last_error() after the exit
+    assert "fmm_driver: the force path stopped:" in r.stderr and "GPU" in r.stderr.split("stopped:")[1]
 
 
 def test_run_program_fails_loudly_without_a_gpu(tmp_path):
