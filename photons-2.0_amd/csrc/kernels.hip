@@ -273,10 +273,15 @@ __device__ __forceinline__ void store_acc(const P2PArgs &a, int i, double ax, do
 // Three of them (rsqrt, rcp, exp2) go to the transcendental unit, which bounds the
 // loop more than the FMAs do: measured at 256^3, dropping two FMAs per pair changed
 // nothing, dropping the exp2 saved 12% of the P2P time.
-// The particle mass multiplies the sum once at the end.
+// Coordinates are in units of 2 rs / sqrt(log2 e) (scaled once when staged), so that
+// exp2(-r2) = exp(-u^2) needs no multiply; the particle mass and the factor the scaling
+// leaves on the sum (the square of the scale) multiply it once at the end.
 struct PairConsts {
-    float ieps, inv4rs, c2u, kexp;
+    float ieps, inv4rs, c2u;
 };
+
+// the scale and the constants of those units
+constexpr double kSqrtLog2e = 1.2011224087864498;  // sqrt(log2 e)
 
 __device__ __forceinline__ void pair_f32(const PairConsts &k, float dx, float dy, float dz, float &ax, float &ay,
                                          float &az)
@@ -289,7 +294,7 @@ __device__ __forceinline__ void pair_f32(const PairConsts &k, float dx, float dy
     const float r = r2 * ri;
     const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(r, k.inv4rs, 1.0f));
     const float R = __builtin_fmaf(r, k.c2u, erfcx_t6(t));
-    const float e = __builtin_amdgcn_exp2f(r2 * k.kexp);
+    const float e = __builtin_amdgcn_exp2f(-r2);
     const float f = rc * rc * rc * (e * R);
     ax = __builtin_fmaf(dx, f, ax);
     ay = __builtin_fmaf(dy, f, ay);
@@ -318,8 +323,8 @@ __device__ __forceinline__ void pair2_f32(const PairConsts &k, float dx, float d
     const float ts = dr * q;
     const float Rr = __builtin_fmaf(r, k.c2u, erfcx_t6(tr));
     const float Rs = __builtin_fmaf(s, k.c2u, erfcx_t6(ts));
-    const float er = __builtin_amdgcn_exp2f(r2 * k.kexp);
-    const float es = __builtin_amdgcn_exp2f(s2 * k.kexp);
+    const float er = __builtin_amdgcn_exp2f(-r2);
+    const float es = __builtin_amdgcn_exp2f(-s2);
     const float fr = rc * rc * rc * (er * Rr);
     const float fs = sc * sc * sc * (es * Rs);
     ax = __builtin_fmaf(ex, fs, __builtin_fmaf(dx, fr, ax));
@@ -385,15 +390,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     const int n = IT.npart;
     if (n == 0) return;
     const LaneMap L = lane_map(n, lane);
-    const float txA = a.ox[IT.ipart + L.tA], tyA = a.oy[IT.ipart + L.tA], tzA = a.oz[IT.ipart + L.tA];
-    const float txB = a.ox[IT.ipart + L.tB], tyB = a.oy[IT.ipart + L.tB], tzB = a.oz[IT.ipart + L.tB];
+    const double lam_d = kSqrtLog2e * a.inv2rs;  // length -> pair-force units
+    const float lam = (float)lam_d;
+    const float txA = a.ox[IT.ipart + L.tA] * lam, tyA = a.oy[IT.ipart + L.tA] * lam,
+                tzA = a.oz[IT.ipart + L.tA] * lam;
+    const float txB = a.ox[IT.ipart + L.tB] * lam, tyB = a.oy[IT.ipart + L.tB] * lam,
+                tzB = a.oz[IT.ipart + L.tB] * lam;
     const CellGeom gT = a.geom[T];
     const double boxlen = a.bx.len_of(T);
     PairConsts pc;
-    pc.ieps = (float)(1.0 / a.soft);
-    pc.inv4rs = (float)(0.5 * a.inv2rs);
-    pc.c2u = (float)(1.1283791670955126 * a.inv2rs);               // 2/sqrt(pi) * u / r
-    pc.kexp = (float)(-1.4426950408889634 * a.inv2rs * a.inv2rs);  // -log2(e) u^2 / r^2
+    pc.ieps = (float)(1.0 / (a.soft * lam_d));
+    pc.inv4rs = (float)(0.5 / kSqrtLog2e);                // u/2 per unit length
+    pc.c2u = (float)(1.1283791670955126 / kSqrtLog2e);   // 2/sqrt(pi) u per unit length
     float axA = 0.f, ayA = 0.f, azA = 0.f, axB = 0.f, ayB = 0.f, azB = 0.f;
     const int64_t e1 = a.off[T + 1];
     for (int64_t base = a.off[T]; base < e1;) {
@@ -406,9 +414,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             const CellGeom gS = a.geom[S];
             double shx, shy, shz;
             shift_vec(sid, boxlen, shx, shy, shz);
-            ecx[lane] = (float)((gS.c[0] + shx) - gT.c[0]);
-            ecy[lane] = (float)((gS.c[1] + shy) - gT.c[1]);
-            ecz[lane] = (float)((gS.c[2] + shz) - gT.c[2]);
+            ecx[lane] = (float)(((gS.c[0] + shx) - gT.c[0]) * lam_d);
+            ecy[lane] = (float)(((gS.c[1] + shy) - gT.c[1]) * lam_d);
+            ecz[lane] = (float)(((gS.c[2] + shz) - gT.c[2]) * lam_d);
             eip[lane] = IS.ipart;
             cnt = IS.npart;
         }
@@ -448,29 +456,31 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             const int p = lane + 64 * r;
             if (p < tot) {
                 const int j = js[r];
-                sx[p] = ecx[j] + vx[r];
-                sy[p] = ecy[j] + vy[r];
-                sz[p] = ecz[j] + vz[r];
+                sx[p] = __builtin_fmaf(vx[r], lam, ecx[j]);
+                sy[p] = __builtin_fmaf(vy[r], lam, ecy[j]);
+                sz[p] = __builtin_fmaf(vz[r], lam, ecz[j]);
             }
         }
         __syncthreads();
         {
-            int q = L.kA;
-            for (; q + L.sA < tot; q += 2 * L.sA) {
-                const int q2 = q + L.sA;
-                pair2_f32(pc, sx[q] - txA, sy[q] - tyA, sz[q] - tzA, sx[q2] - txA, sy[q2] - tyA,
-                          sz[q2] - tzA, axA, ayA, azA);
-            }
-            if (q < tot) pair_f32(pc, sx[q] - txA, sy[q] - tyA, sz[q] - tzA, axA, ayA, azA);
+            // sources (q, q + s) of this lane's class while q + s < tot, then at most one
+            // more; the LDS address is the only induction variable
+            const int st = L.sA;
+            const float *p = sx + min(L.kA, tot);  // idle lanes: start at the end
+            const float *const pend = sx + max(tot - st, 0);
+            for (; p < pend; p += 2 * st)
+                pair2_f32(pc, p[0] - txA, p[STAGE] - tyA, p[2 * STAGE] - tzA, p[st] - txA, p[st + STAGE] - tyA,
+                          p[st + 2 * STAGE] - tzA, axA, ayA, azA);
+            if (p < sx + tot) pair_f32(pc, p[0] - txA, p[STAGE] - tyA, p[2 * STAGE] - tzA, axA, ayA, azA);
         }
         if (L.e) {
-            int q = L.kB;
-            for (; q + L.sB < tot; q += 2 * L.sB) {
-                const int q2 = q + L.sB;
-                pair2_f32(pc, sx[q] - txB, sy[q] - tyB, sz[q] - tzB, sx[q2] - txB, sy[q2] - tyB,
-                          sz[q2] - tzB, axB, ayB, azB);
-            }
-            if (q < tot) pair_f32(pc, sx[q] - txB, sy[q] - tyB, sz[q] - tzB, axB, ayB, azB);
+            const int st = L.sB;
+            const float *p = sx + min(L.kB, tot);  // idle lanes: start at the end
+            const float *const pend = sx + max(tot - st, 0);
+            for (; p < pend; p += 2 * st)
+                pair2_f32(pc, p[0] - txB, p[STAGE] - tyB, p[2 * STAGE] - tzB, p[st] - txB, p[st + STAGE] - tyB,
+                          p[st + 2 * STAGE] - tzB, axB, ayB, azB);
+            if (p < sx + tot) pair_f32(pc, p[0] - txB, p[STAGE] - tyB, p[2 * STAGE] - tzB, axB, ayB, azB);
         }
         base += ne;
         __syncthreads();  // the staging arrays are rewritten by the next batch
@@ -501,7 +511,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
                 bz += red[320 + l];
             }
         }
-        const float m = (float)a.mass;
+        const float m = (float)(a.mass * lam_d * lam_d);
         bx *= m;
         by *= m;
         bz *= m;
