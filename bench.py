@@ -7,12 +7,14 @@ Per-phase GPU times come from HIP events the library records on its own stream.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--nside 256] [--order 4]
 
-Two more workloads print their own JSON line (not the default):
+Three more workloads print their own JSON line (not the default):
     --workload ensemble   64 independent clustered boxes of 32^3..128^3 particles (log-uniform,
                           seed 5) in one batched evaluation, leaf size and P2P staging autotuned
     --workload serving    256 resident boxes of 16^3..192^3 particles (seed 6), single-step
                           (force + kick/drift) requests with Poisson arrivals at swept rates,
                           HBM budget 200 GB: throughput and p50/p99 step latency per rate
+    --workload treepm     whole TreePM steps (kick, drift, short-range and PM forces, kick) of the
+                          --nside^3 clustered box, particles resident in HBM
 """
 from __future__ import annotations
 
@@ -160,6 +162,62 @@ def run_ensemble(args, torch):
     }
 
 
+def run_treepm(args, torch):
+    """TreePM steps (photons_step_device: both half kicks, drift, wrap, new tree and
+    short-range force, PM force on an nside^3 mesh) of nside^3 clustered particles that
+    stay in HBM; FP32-mixed and FP64 short-range force, FP64 PM and kick/drift."""
+    log("generating particles")
+    pos, prm = ic.workload_clustered(nside=args.nside, order=args.order)
+    n = len(pos)
+    vel = np.random.Generator(np.random.PCG64(9)).normal(size=pos.shape) * 300.0
+    loga = np.log(0.5) + 0.01 * np.arange(args.warmup + args.steps + 1)
+    fac = [(0.5 * pg.kick_factor(li, lf) * ic.GRAV_CONST, pg.drift_factor(li, lf)) for li, lf in zip(loga[:-1], loga[1:])]
+    res = {}
+    for name, prec in (("fp32", pg.FP32_MIXED), ("fp64", pg.FP64)):
+        d = {k: torch.from_numpy(np.ascontiguousarray(v)).to("cuda") for k, v in (("pos", pos), ("vel", vel))}
+        d["acc"] = torch.zeros_like(d["pos"])
+        d["apm"] = torch.zeros_like(d["pos"])
+        torch.cuda.synchronize()
+        ctx = pg.Context(prm, prec)
+        mesh = pg.PM(args.nside, prm.boxsize, prm.rs, prm.mass)
+        ptr = {k: v.data_ptr() for k, v in d.items()}
+        mesh.force_device(n, ptr["pos"], ptr["apm"])
+        ctx.force_device(n, ptr["pos"], ptr["acc"])
+        mesh.sync()
+        ctx.sync()
+        t0 = time.perf_counter()
+        mesh.force_device(n, ptr["pos"], ptr["apm"])
+        mesh.sync()
+        pm_ms = (time.perf_counter() - t0) * 1e3
+        wall = []
+        for s, (dkh, dd) in enumerate(fac):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            ctx.step_device(mesh, n, ptr["pos"], ptr["vel"], ptr["acc"], ptr["apm"], dkh, dd)
+            ctx.sync()
+            if s >= args.warmup:
+                wall.append(time.perf_counter() - t0)
+                log(f"[{name}] TreePM step {s - args.warmup + 1}/{args.steps} {wall[-1] * 1e3:.1f} ms")
+        res[name] = dict(wall=np.array(wall), pm_ms=pm_ms)
+        ctx.close()
+        mesh.close()
+    ms = res["fp32"]["wall"] * 1e3
+    return {
+        "metric": METRIC, "value": n / float(np.mean(res["fp32"]["wall"])), "unit": "particles/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": float(np.mean(ms)),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32-mixed",
+        "data": "synthetic",
+        "config": {"workload": f"TreePM steps of {args.nside}^3 clustered particles resident in HBM: half kicks, "
+                               f"drift, wrap, new tree and short-range force, PM on a {args.nside}^3 mesh "
+                               "(FP64), half kicks; a from 0.5 in steps of 0.01 in ln a",
+                   "particles": n, "order": prm.order, "theta": prm.theta, "maxleaf": prm.maxleaf},
+        "latency_ms": {"p50": float(np.percentile(ms, 50)), "p99": float(np.percentile(ms, 99))},
+        "pm_ms": res["fp32"]["pm_ms"],
+        "fp64": {"value": n / float(np.mean(res["fp64"]["wall"])),
+                 "ms_per_step": float(np.mean(res["fp64"]["wall"]) * 1e3)},
+    }
+
+
 def run_serving(args, torch):
     """Resident boxes, Poisson arrivals of single-step requests at swept rates."""
     nbox = args.serve_boxes
@@ -247,7 +305,7 @@ def main():
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
-    ap.add_argument("--workload", choices=["clustered", "ensemble", "serving"], default="clustered")
+    ap.add_argument("--workload", choices=["clustered", "ensemble", "serving", "treepm"], default="clustered")
     ap.add_argument("--serve-boxes", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival span per rate")
     ap.add_argument("--rates32", type=float, nargs="+", default=[4, 8, 16, 32, 64, 128])
@@ -272,7 +330,7 @@ def main():
         log(f"copy calibration: {nbytes} bytes read and written once")
         del src, dst
     if args.workload != "clustered":
-        fn = run_ensemble if args.workload == "ensemble" else run_serving
+        fn = {"ensemble": run_ensemble, "serving": run_serving, "treepm": run_treepm}[args.workload]
         print(json.dumps(fn(args, torch)), flush=True)
         return
     log("generating particles")

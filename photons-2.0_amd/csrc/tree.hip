@@ -76,13 +76,21 @@ __global__ void k_nchunks(const int *__restrict__ cnt, int seg0, int S, int *__r
     out[s] = (s == S) ? 0 : (cnt[seg0 + s] + kChunk - 1) / kChunk;
 }
 
-__global__ void k_mean(LevelArgs a, const double *__restrict__ partial, double *__restrict__ t_split)
+// One wavefront per segment: lane l adds the chunk partials l, l + 64, ... in order and a
+// fixed butterfly combines the lanes, so the mean is deterministic, a one-chunk segment
+// gets its partial exactly, and the root's thousands of chunks are not one thread's
+// chain of dependent loads.
+__global__ void __launch_bounds__(256) k_mean(LevelArgs a, const double *__restrict__ partial,
+                                              double *__restrict__ t_split)
 {
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
+    const int s = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
+    const int lane = threadIdx.x % kWave;
     if (s >= a.S) return;
     double v = 0.0;
-    for (int c = a.seg_chunk[s]; c < a.seg_chunk[s + 1]; c++) v += partial[c];
-    t_split[a.seg0 + s] = v / (double)a.t_count[a.seg0 + s];
+    for (int c = a.seg_chunk[s] + lane; c < a.seg_chunk[s + 1]; c += kWave) v += partial[c];
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
+    if (lane == 0) t_split[a.seg0 + s] = v / (double)a.t_count[a.seg0 + s];
 }
 
 __global__ void __launch_bounds__(256) k_chunk_count(LevelArgs a, const double *__restrict__ t_split,
@@ -544,7 +552,7 @@ void build_tree(Ctx &c)
         int *flag = ensure<int>(c.seg_flag, S + 1);
         int wb = blocks(C, kWavesPerBlock);
         k_chunk_sum<<<wb, 256, 0, st>>>(a, partial);
-        k_mean<<<blocks(S, 256), 256, 0, st>>>(a, partial, T.t_split);
+        k_mean<<<blocks(S, kWavesPerBlock), 256, 0, st>>>(a, partial, T.t_split);
         k_chunk_count<<<blocks(C + 1, kWavesPerBlock), 256, 0, st>>>(a, T.t_split, rcnt);
         exclusive_scan(c, rcnt, rpre, C + 1);
         k_seg_counts<<<blocks(S, 256), 256, 0, st>>>(a, rpre, nleft, flag);
