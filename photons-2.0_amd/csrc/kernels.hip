@@ -265,6 +265,20 @@ __device__ __forceinline__ void store_acc(const P2PArgs &a, int i, double ax, do
     a.out[o + 2] = az;
 }
 
+// Target leaf of a P2P workgroup.  Workgroups go round-robin to the 8 XCDs, each with its
+// own L2; neighbouring leaves (depth-first order) list mostly the same sources, so tiles of
+// 8 consecutive leaves stay on one XCD (their source particles are fetched into that L2
+// once), and the tiles rotate over the XCDs, which keeps dense clusters spread over all
+// of them.  A bijection of [0, nleaf); the last partial round keeps the identity.
+__device__ __forceinline__ int p2p_target(int b, int nb)
+{
+    constexpr int kXcd = 8, kTile = 8;
+    const int full = nb / (kXcd * kTile) * (kXcd * kTile);
+    if (b >= full) return b;
+    const int x = b % kXcd, k = b / kXcd;
+    return ((k / kTile) * kXcd + x) * kTile + k % kTile;
+}
+
 // FP32 pair force per interaction, about 28 VALU instructions:
 //   r2 = |d|^2 + 1e-30 (keeps rsqrt finite for coincident particles, whose d = 0)
 //   1/max(r, eps)^3 = min(rsqrt(r2), 1/eps)^3   (the reference's softening branch)
@@ -386,7 +400,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     float *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ float ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
-    const int T = blockIdx.x;
+    const int T = p2p_target(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
@@ -570,7 +584,7 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     double *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ double ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
-    const int T = blockIdx.x;
+    const int T = p2p_target(blockIdx.x, gridDim.x);
     const int lane = threadIdx.x;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
