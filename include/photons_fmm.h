@@ -218,7 +218,9 @@ int photons_pm_sync(photons_pm *pm);
  * Before the first step acc (and acc_pm) hold the forces of the initial
  * positions, as the reference computes them before its loop
  * (src/photoNs.c:97-116).  Returns with the last kick queued on the context's
- * stream (photons_sync waits).                                               */
+ * stream (photons_sync waits).  All arrays live on the context's device.  A
+ * failed step leaves the particles part-way through it (the first kicks and the
+ * drift may have been applied).                                              */
 int photons_step_device(photons_ctx *ctx, photons_pm *pm, int64_t n, double *d_pos, double *d_vel, double *d_acc,
                         double *d_acc_pm, double dkh, double dd);
 
