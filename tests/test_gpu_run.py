@@ -65,3 +65,40 @@ def test_run_matches_the_oracle(tmp_path):
     assert d.max() <= 2e-7 * L
     assert np.abs(gv - v).max() <= 1e-6 * np.abs(v).max()
     assert np.abs(gv - vel).max() > 1e-2 * np.abs(vel).max()  # the kicks changed the velocities
+
+
+def test_run_options_fp32_and_no_pm(tmp_path):
+    """--fp32 and --no-pm: the FP32-mixed run stays within the FP32 tolerance of the FP64
+    run; without PM the result is the short-range-only steps of the oracle"""
+    nside, z0, af, nstep = 16, 9.0, 0.104, 1
+    pos, prm = ic.workload_clustered(nside=nside, order=3, theta=0.4)
+    L, n = prm.boxsize, len(pos)
+    vel = np.random.Generator(np.random.PCG64(32)).normal(size=pos.shape) * 300.0
+    ic_path = tmp_path / "ic.gdt2"
+    pg.snapshot_write(ic_path, pos, vel, boxsize=L, redshift=z0, mass=prm.mass)
+    outs = {}
+    for tag, extra in (("fp64", []), ("fp32", ["--fp32"]), ("nopm", ["--no-pm"])):
+        (tmp_path / f"{tag}.param").write_text(
+            f"InputPath {ic_path}\nOutputPath {tmp_path}\nOutputName {tag}\nOPENANGLE 0.4\nNumMeshSide {nside}\n"
+            f"GravConstant {ic.GRAV_CONST}\nSOFTENING 40.0\nSPLITSCALE {1.25 * L / nside}\n")
+        print(f"photons_run {tag}", flush=True)
+        r = subprocess.run([EXE, str(tmp_path / f"{tag}.param"), "--nstep", str(nstep), "--af", str(af)] + extra,
+                           capture_output=True, text=True, timeout=300)
+        assert r.returncode == 0, r.stderr
+        assert "soft = 40" in r.stdout
+        outs[tag] = pg.snapshot_read(tmp_path / f"{tag}_3.0")
+    dv = np.linalg.norm(outs["fp32"][1] - outs["fp64"][1], axis=1)
+    assert np.percentile(dv, 99) <= 1e-4 * np.abs(outs["fp64"][1]).max()
+    # the short-range-only step against the oracle's (float32 file rounding)
+    p, v = pg.snapshot_read(ic_path)
+    h = pg.snapshot_header(ic_path)
+    rs = 1.25 * L / nside
+    oprm = ic.ForceParams(boxsize=L, mass=h["mass"][1], soft=40.0, rs=rs, rc=4.5 * rs, theta=0.4, maxleaf=32,
+                          order=3, periodic=1)
+    a, _ = oracle.force(p, oprm)
+    li, lf = math.log(1 / (1 + z0)), math.log(af)
+    dkh = 0.5 * oracle.kick_loga(li, lf, h["omega_m"], h["omega_l"]) * ic.GRAV_CONST
+    p, v, a = oracle.step(p, v, a, oprm, dkh, oracle.drift_loga(li, lf, h["omega_m"], h["omega_l"]))
+    d = np.abs(outs["nopm"][0] - p)
+    assert np.minimum(d, L - d).max() <= 2e-7 * L
+    assert np.abs(outs["nopm"][1] - v).max() <= 1e-6 * np.abs(v).max()

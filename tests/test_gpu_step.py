@@ -89,3 +89,24 @@ def test_fp32_mixed_step_stays_close_to_fp64(state):
     dv = np.linalg.norm(a32["vel"] - a64["vel"], axis=1) / np.linalg.norm(a64["vel"] - state["vel"], axis=1)
     print(f"FP32-mixed vs FP64 velocity change after two steps: p99 {np.percentile(dv, 99):.2e}")
     assert np.percentile(dv, 99) <= 1e-4
+
+
+def test_step_refuses_bad_calls(state):
+    import torch
+    prm = state["prm"]
+    ctx = pg.Context(prm, pg.FP64)
+    mesh = pg.PM(state["nside"], prm.boxsize, prm.rs, prm.mass)
+    d = torch.zeros((prm.maxleaf, 3), dtype=torch.float64, device="cuda")
+    p = d.data_ptr()
+    # periodic force needs more than maxleaf particles
+    with pytest.raises(pg.PhotonsError, match="maxleaf"):
+        ctx.step_device(None, prm.maxleaf, p, p, p, 0, 1.0, 1.0)
+    big = torch.zeros((4 * prm.maxleaf, 3), dtype=torch.float64, device="cuda")
+    q = big.data_ptr()
+    # a PM handle needs the acc_pm array
+    with pytest.raises(pg.PhotonsError, match="acc_pm"):
+        ctx.step_device(mesh, 4 * prm.maxleaf, q, q, q, 0, 1.0, 1.0)
+    # n = 0 is a no-op
+    ctx.step_device(mesh, 0, q, q, q, q, 1.0, 1.0)
+    ctx.close()
+    mesh.close()
