@@ -43,7 +43,17 @@ FORCE_CASES = {
     "clus24_pm48_th0.3": lambda: _clustered(24, 48, theta=0.3, seed=7),
     # the reference's own demo initial conditions (demo/ic_lcdm.gdt2)
     "lcdm_demo": _lcdm,
+    # 262,144 clustered particles: the parity case at scale (every 64th particle stored)
+    "clus64_pm64": lambda: _clustered(64, 64),
 }
+
+# stored stride of the sampled accelerations (default 8)
+STRIDE = {"clus64_pm64": 64}
+
+# cases whose oracle run takes about a minute on one core: the CPU suite checks them
+# only with --full (the reference and the oracle agreed to 5e-15 on them when the
+# vectors were written); the GPU parity test compares with the stored vectors
+LARGE_CASES = {"clus64_pm64"}
 
 
 def case_inputs(name):

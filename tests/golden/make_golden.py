@@ -7,11 +7,12 @@ remotes.c, toptree.c, utility.c) is run by oracle/ref_harness.c on one MPI rank.
 Outputs, all committed as small fixtures next to this script:
   force_<case>.npz  inputs are regenerated from seeds (a SHA-256 of the positions is
                     stored to catch generator drift); stored: acceleration of every
-                    8th particle, full-vector checksums, list counters.
+                    8th particle (64th for the 64^3 case), full-vector checksums, list
+                    counters; written only when the oracle agrees with the reference.
   lcdm_pos.npy      positions of demo/ic_lcdm.gdt2 (float32, exactly as stored there)
   ops.npz           single-operator calls: inputs and outputs
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
 
@@ -32,10 +33,9 @@ sys.path.insert(0, HERE)
 
 import ic  # noqa: E402
 import oracle  # noqa: E402
-from cases import FORCE_CASES, case_inputs  # noqa: E402
+from cases import FORCE_CASES, STRIDE, case_inputs  # noqa: E402
 
 REFERENCE = "/root/reference"
-STRIDE = 8
 
 
 def pos_digest(pos: np.ndarray) -> str:
@@ -55,16 +55,20 @@ def read_gadget_positions(path: str) -> np.ndarray:
     return np.frombuffer(raw, dtype="<f4", count=3 * n, offset=off + 4).reshape(n, 3).copy()
 
 
-def write_force_cases(tmp: str):
+def write_force_cases(tmp: str, names):
     lcdm = read_gadget_positions(os.path.join(REFERENCE, "demo", "ic_lcdm.gdt2"))
     np.save(os.path.join(HERE, "lcdm_pos.npy"), lcdm)
-    for name in FORCE_CASES:
+    for name in names:
         pos, prm = case_inputs(name)
         tree = oracle.Tree(pos, prm)
         if not tree.fits_reference_buffers():
             raise SystemExit(f"{name}: exceeds the reference's fixed buffers")
         acc, cnt = oracle.ref_force(pos, prm, tmp)
-        idx = np.arange(0, len(pos), STRIDE)
+        oacc, _ = oracle.force(pos, prm)
+        agree = float(np.max(np.linalg.norm(oacc - acc, axis=1) / np.linalg.norm(acc, axis=1)))
+        if agree > 1e-12:
+            raise SystemExit(f"{name}: the oracle and the reference differ by {agree:.2e}")
+        idx = np.arange(0, len(pos), STRIDE.get(name, 8))
         np.savez_compressed(
             os.path.join(HERE, f"force_{name}.npz"),
             pos_sha256=np.array(pos_digest(pos)),
@@ -75,7 +79,7 @@ def write_force_cases(tmp: str):
             acc_sq_sum=(acc * acc).sum(axis=0),
             counters=np.array([cnt[k] for k in ("idxP2P", "idxM2L", "walk_m2l_count",
                                                 "p2p_count_remote", "nleaf", "nnode")], np.int64))
-        print(name, len(pos), cnt, flush=True)
+        print(name, len(pos), cnt, f"oracle vs reference {agree:.1e}", flush=True)
 
 
 def run_ops(records: bytes, count: int, tmp: str) -> bytes:
@@ -168,11 +172,14 @@ def write_ops(tmp: str):
 
 
 def main():
+    """no arguments: every fixture; otherwise only the named force cases"""
     if not oracle.ref_available():
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
+    names = sys.argv[1:] or list(FORCE_CASES)
     with tempfile.TemporaryDirectory() as tmp:
-        write_ops(tmp)
-        write_force_cases(tmp)
+        if not sys.argv[1:]:
+            write_ops(tmp)
+        write_force_cases(tmp, names)
 
 
 if __name__ == "__main__":

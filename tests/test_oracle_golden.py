@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from cases import FORCE_CASES, case_inputs
+from cases import FORCE_CASES, LARGE_CASES, case_inputs
 
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
@@ -21,7 +21,9 @@ def load(name):
 
 
 @pytest.mark.parametrize("name", sorted(FORCE_CASES))
-def test_force_matches_reference(name):
+def test_force_matches_reference(name, full_sizes):
+    if name in LARGE_CASES and not full_sizes:
+        pytest.skip("a minute of oracle time: run with --full")
     g = load(f"force_{name}.npz")
     pos, prm = case_inputs(name)
     digest = hashlib.sha256(np.ascontiguousarray(pos, np.float64).tobytes()).hexdigest()
