@@ -84,26 +84,62 @@ def _cpu_rank(args):
     return len(pos), time.perf_counter() - t0
 
 
+REF_HARNESS = os.path.join(ROOT, "oracle", "_ref", "refharness")
+
+
+def _ref_rank(args):
+    """one 'rank': the reference's own force path (its fmm.c, operator.c, remotes.c,
+    toptree.c and utility.c compiled from its sources into oracle/_ref/refharness, one MPI
+    rank) on its own clustered box, timed around the process"""
+    import subprocess
+    import tempfile
+    nside, seed = args
+    pos, prm = ic.workload_clustered(nside=nside, order=3, seed=seed)
+    with tempfile.TemporaryDirectory() as tmp:
+        fin, fout = os.path.join(tmp, "in.bin"), os.path.join(tmp, "out.bin")
+        with open(fin, "wb") as f:
+            f.write(np.array([len(pos), prm.maxleaf, 0, 0], np.int32).tobytes())
+            f.write(np.array([prm.boxsize, prm.mass, prm.soft, prm.rs, prm.rc, prm.theta]).tobytes())
+            f.write(np.ascontiguousarray(pos, np.float64).tobytes())
+        t0 = time.perf_counter()
+        subprocess.run([REF_HARNESS, "force", fin, fout], check=True, stdout=subprocess.DEVNULL,
+                       stderr=subprocess.DEVNULL)
+        return len(pos), time.perf_counter() - t0
+
+
 def cpu_baseline(nside=48, cores=None):
-    """The FP64 CPU oracle (a restatement of the reference force path, single threaded
-    like one reference rank) on clustered boxes with the 256^3 workload's density, rs,
-    rc, theta, p = 4 and leaf size (nside^3 particles each): first one rank alone, then
-    one rank per host core at once, each on its own box (the reference's one rank per
-    core, without its domain exchange)."""
+    """The reference CPU force path beside the GPU: the reference's own FMM sources compiled
+    here (oracle/_ref/refharness, -O2, single threaded like one reference rank; its expansion
+    order is the compiled p = 3) when present, else the FP64 oracle (their restatement, p = 4;
+    the two take the same time), on clustered boxes with the 256^3 workload's density, rs, rc,
+    theta and leaf size (nside^3 particles each): first one rank alone, then one rank per host
+    core at once, each on its own box (the reference's one rank per core, without its domain
+    exchange)."""
     import multiprocessing as mp
     if cores is None:
         cores = min(16, len(os.sched_getaffinity(0)))
-    n1, t1 = _cpu_rank((nside, 101))
+    reference = os.access(REF_HARNESS, os.X_OK)
+    if reference:
+        try:
+            n1, t1 = _ref_rank((nside, 101))
+        except Exception as e:  # e.g. a runtime library missing on this host
+            log(f"reference harness unusable here ({e}); timing the oracle instead")
+            reference = False
+    rank = _ref_rank if reference else _cpu_rank
+    if not reference:
+        n1, t1 = rank((nside, 101))
     ctx = mp.get_context("spawn")
     with ctx.Pool(cores) as pool:
         t0 = time.perf_counter()
-        res = pool.map(_cpu_rank, [(nside, 101 + k) for k in range(cores)])
+        res = pool.map(rank, [(nside, 101 + k) for k in range(cores)])
         wall = time.perf_counter() - t0
     ntot = sum(r[0] for r in res)
     _, prm = ic.workload_clustered(nside=nside, order=4)
-    return {"value": ntot / wall, "unit": "particles/s", "cores": cores, "kind": "port",
-            "sample": f"{cores} concurrent single-threaded ranks, each one force evaluation of its own {nside}^3 "
-                      f"clustered box at the density, rs, rc, theta, p=4 and leaf size of the 256^3 workload "
+    what = ("the reference's force path (its own sources, p=3 as compiled)" if reference else
+            "the FP64 oracle (p=4)")
+    return {"value": ntot / wall, "unit": "particles/s", "cores": cores, "kind": "reference" if reference else "port",
+            "sample": f"{cores} concurrent single-threaded ranks of {what}, each one force evaluation of its own "
+                      f"{nside}^3 clustered box at the density, rs, rc, theta and leaf size of the 256^3 workload "
                       f"(box {prm.boxsize:.0f} kpc/h), {wall:.1f} s wall",
             "one_core": {"value": n1 / t1, "seconds": t1}}
 
