@@ -126,16 +126,21 @@ int photons_create(photons_ctx **out, const photons_params *prm, int device, voi
         photons_ctx *x = new photons_ctx();
         x->c.prm = *prm;
         x->c.device = device;
-        if (stream) {
-            x->c.stream = reinterpret_cast<hipStream_t>(stream);
-        } else {
-            PH_CHECK(hipStreamCreateWithFlags(&x->c.stream, hipStreamNonBlocking));
-            x->c.own_stream = true;
+        try {
+            if (stream) {
+                x->c.stream = reinterpret_cast<hipStream_t>(stream);
+            } else {
+                PH_CHECK(hipStreamCreateWithFlags(&x->c.stream, hipStreamNonBlocking));
+                x->c.own_stream = true;
+            }
+            PH_CHECK(hipHostMalloc(&x->c.pinned.p, 4096, hipHostMallocDefault));
+            x->c.pinned.cap = 4096;
+            grow(x->c.err, sizeof(int));
+            for (int i = 0; i < 16; i++) PH_CHECK(hipEventCreate(&x->c.timing.ev[i]));
+        } catch (...) {
+            photons_destroy(x);  // releases whatever was created; null members are skipped
+            throw;
         }
-        PH_CHECK(hipHostMalloc(&x->c.pinned.p, 4096, hipHostMallocDefault));
-        x->c.pinned.cap = 4096;
-        grow(x->c.err, sizeof(int));
-        for (int i = 0; i < 16; i++) PH_CHECK(hipEventCreate(&x->c.timing.ev[i]));
         *out = x;
         return 0;
     } catch (const std::exception &e) {

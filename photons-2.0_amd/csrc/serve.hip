@@ -253,14 +253,26 @@ int photons_server_create(photons_server **out, const photons_params *prm, int d
             return fail("the lanes' evaluation buffers alone exceed the HBM budget");
         }
         s->lanes.resize(nlanes);
-        for (int l = 0; l < nlanes; l++) {
-            Lane &ln = s->lanes[l];
-            SV_CHECK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
-            if (photons_create(&ln.ctx, prm, device, ln.stream) != 0)
-                throw std::runtime_error(std::string("lane context: ") + photons_last_error());
-            ln.cap = max_batch_particles;
-            SV_CHECK(hipMalloc(&ln.in, sizeof(double) * 3 * (size_t)ln.cap));
-            SV_CHECK(hipMalloc(&ln.out, sizeof(double) * 3 * (size_t)ln.cap));
+        try {
+            for (int l = 0; l < nlanes; l++) {
+                Lane &ln = s->lanes[l];
+                SV_CHECK(hipStreamCreateWithFlags(&ln.stream, hipStreamNonBlocking));
+                if (photons_create(&ln.ctx, prm, device, ln.stream) != 0)
+                    throw std::runtime_error(std::string("lane context: ") + photons_last_error());
+                ln.cap = max_batch_particles;
+                SV_CHECK(hipMalloc(&ln.in, sizeof(double) * 3 * (size_t)ln.cap));
+                SV_CHECK(hipMalloc(&ln.out, sizeof(double) * 3 * (size_t)ln.cap));
+            }
+        } catch (...) {
+            // no lane thread runs yet: release the lanes built so far
+            for (Lane &ln : s->lanes) {
+                if (ln.ctx) photons_destroy(ln.ctx);
+                if (ln.in) (void)hipFree(ln.in);
+                if (ln.out) (void)hipFree(ln.out);
+                if (ln.stream) (void)hipStreamDestroy(ln.stream);
+            }
+            delete s;
+            throw;
         }
         for (int l = 0; l < nlanes; l++) s->lanes[l].th = std::thread(lane_loop, s, l);
         *out = s;
@@ -320,17 +332,26 @@ int photons_server_add_box(photons_server *s, int64_t n, double boxsize, const d
         s->resident += bytes;
         s->peak = std::max(s->peak, s->resident + s->reserve);
         g.unlock();
-        SV_CHECK(hipSetDevice(s->device));
         Box b;
         b.n = n;
         b.L = boxsize;
-        const size_t nb = sizeof(double) * 3 * (size_t)n;
-        SV_CHECK(hipMalloc(&b.pos, nb));
-        SV_CHECK(hipMalloc(&b.vel, nb));
-        SV_CHECK(hipMalloc(&b.acc, nb));
-        SV_CHECK(hipMemcpy(b.pos, pos, nb, hipMemcpyHostToDevice));
-        SV_CHECK(hipMemcpy(b.vel, vel, nb, hipMemcpyHostToDevice));
-        if (acc) SV_CHECK(hipMemcpy(b.acc, acc, nb, hipMemcpyHostToDevice));
+        try {
+            SV_CHECK(hipSetDevice(s->device));
+            const size_t nb = sizeof(double) * 3 * (size_t)n;
+            SV_CHECK(hipMalloc(&b.pos, nb));
+            SV_CHECK(hipMalloc(&b.vel, nb));
+            SV_CHECK(hipMalloc(&b.acc, nb));
+            SV_CHECK(hipMemcpy(b.pos, pos, nb, hipMemcpyHostToDevice));
+            SV_CHECK(hipMemcpy(b.vel, vel, nb, hipMemcpyHostToDevice));
+            if (acc) SV_CHECK(hipMemcpy(b.acc, acc, nb, hipMemcpyHostToDevice));
+        } catch (...) {
+            // give back what this box took: its allocations and its share of the budget
+            for (double *p : {b.pos, b.vel, b.acc})
+                if (p) (void)hipFree(p);
+            std::lock_guard<std::mutex> r(s->mu);
+            s->resident -= bytes;
+            throw;
+        }
         g.lock();
         const int id = s->next_box++;
         s->boxes[id] = b;
