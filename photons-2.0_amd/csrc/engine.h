@@ -100,7 +100,7 @@ struct Ctx {
     int tcap = 0, lcap = 0;        // capacities of the level-order temporaries
     DevBuf t_start, t_count, t_split, t_lo, t_hi, t_son, t_depth, t_parent;
     DevBuf l_start, l_count, l_lo, l_hi, l_parent;
-    DevBuf seg_chunk, chunk_part, chunk_rcnt, seg_nleft, seg_flag, seg_nn, seg_nl, seg_non, seg_lon;
+    DevBuf seg_chunk, chunk_part, chunk_rcnt, seg_nleft, seg_flag, seg_nn, seg_non;
     DevBuf t_size, t_lsize, t_dfs, t_lfirst, t_final, l_final;
     DevBuf err;                    // device error flag (int)
     DevBuf stats;                  // device counters (int64)

@@ -160,7 +160,7 @@ int photons_destroy(photons_ctx *x)
                           &c.m2l_off, &c.m2l_src, &c.cnt_tmp, &c.scan_tmp, &c.t_start, &c.t_count, &c.t_split,
                           &c.t_lo, &c.t_hi, &c.t_son, &c.t_depth, &c.t_parent, &c.l_start, &c.l_count, &c.l_lo,
                           &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
-                          &c.seg_flag, &c.seg_nn, &c.seg_nl, &c.seg_non, &c.seg_lon, &c.t_size, &c.t_lsize,
+                          &c.seg_flag, &c.seg_nn, &c.seg_non, &c.t_size, &c.t_lsize,
                           &c.t_dfs, &c.t_lfirst, &c.t_final, &c.l_final, &c.err, &c.stats, &c.bfs_a, &c.bfs_b,
                           &c.p2p_keys, &c.m2l_keys, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off,
                           &c.d_box_len, &c.d_box_root, &c.cell_box,

@@ -179,21 +179,20 @@ __global__ void __launch_bounds__(256) k_scatter(LevelArgs a, const double *__re
     }
 }
 
-__global__ void k_child_kinds(LevelArgs a, const int *__restrict__ nleft, int maxleaf, int *__restrict__ nn,
-                              int *__restrict__ nl)
+// new nodes (high 32 bits) and new leaves (low 32 bits) of each segment, packed so one
+// scan gives both offsets (the leaf count stays below 2^32, so no carry crosses over)
+__global__ void k_child_kinds(LevelArgs a, const int *__restrict__ nleft, int maxleaf, uint64_t *__restrict__ kinds)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s > a.S) return;
     if (s == a.S) {
-        nn[s] = 0;
-        nl[s] = 0;
+        kinds[s] = 0;
         return;
     }
     int ct = a.t_count[a.seg0 + s];
     int c0 = nleft[s], c1 = ct - c0;
     int k = (c0 > maxleaf) + (c1 > maxleaf);
-    nn[s] = k;
-    nl[s] = 2 - k;
+    kinds[s] = ((uint64_t)k << 32) | (uint64_t)(2 - k);
 }
 
 struct TreeTmp {
@@ -205,8 +204,7 @@ struct TreeTmp {
 };
 
 __global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, const int *__restrict__ nleft,
-                                const int *__restrict__ non, const int *__restrict__ lon, int node_base,
-                                int leaf_base)
+                                const uint64_t *__restrict__ kind_off, int node_base, int leaf_base)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= a.S) return;
@@ -216,7 +214,8 @@ __global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, c
     int beg[2] = {st, st + nleft[s]};
     double split = T.t_split[v];
     int depth = T.t_depth[v];
-    int nextn = node_base + non[s], nextl = leaf_base + lon[s];
+    const uint64_t ko = kind_off[s];
+    int nextn = node_base + (int)(ko >> 32), nextl = leaf_base + (int)(uint32_t)ko;
     for (int k = 0; k < 2; k++) {
         double lo[3], hi[3];
         for (int d = 0; d < 3; d++) {
@@ -558,17 +557,19 @@ void build_tree(Ctx &c)
         k_seg_counts<<<blocks(S, 256), 256, 0, st>>>(a, rpre, nleft, flag);
         k_scatter<<<wb, 256, 0, st>>>(a, T.t_split, rpre, nleft, flag, A, B);
         std::swap(A, B);  // the children's particles are in B now
-        int *nn = ensure<int>(c.seg_nn, S + 1);
-        int *nl = ensure<int>(c.seg_nl, S + 1);
-        int *non = ensure<int>(c.seg_non, S + 1);
-        int *lon = ensure<int>(c.seg_lon, S + 1);
-        k_child_kinds<<<blocks(S + 1, 256), 256, 0, st>>>(a, nleft, maxleaf, nn, nl);
-        exclusive_scan(c, nn, non, S + 1);
-        exclusive_scan(c, nl, lon, S + 1);
-        PH_CHECK(hipMemcpyAsync(pin, non + S, sizeof(int), hipMemcpyDeviceToHost, st));
-        PH_CHECK(hipMemcpyAsync(pin + 1, lon + S, sizeof(int), hipMemcpyDeviceToHost, st));
+        uint64_t *kinds = ensure<uint64_t>(c.seg_nn, S + 1);
+        uint64_t *kind_off = ensure<uint64_t>(c.seg_non, S + 1);
+        k_child_kinds<<<blocks(S + 1, 256), 256, 0, st>>>(a, nleft, maxleaf, kinds);
+        {
+            size_t bytes = 0;
+            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, kinds, kind_off, S + 1, st));
+            void *tmp = grow(c.scan_tmp, bytes);
+            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, kinds, kind_off, S + 1, st));
+        }
+        PH_CHECK(hipMemcpyAsync(pin, kind_off + S, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
         PH_CHECK(hipStreamSynchronize(st));
-        int new_nodes = pin[0], new_leaves = pin[1];
+        const uint64_t tot = *reinterpret_cast<const uint64_t *>(pin);
+        int new_nodes = (int)(tot >> 32), new_leaves = (int)(uint32_t)tot;
         if (ntmp + new_nodes > T.tcap) {
             int ncap = std::max(2 * T.tcap, ntmp + new_nodes + 1024);
             alloc_nodes(ncap, T.tcap);
@@ -576,7 +577,7 @@ void build_tree(Ctx &c)
             a.t_count = T.t_count;
         }
         if (nleaf + new_leaves > T.lcap) alloc_leaves(std::max(2 * T.lcap, nleaf + new_leaves + 1024), T.lcap);
-        k_make_children<<<blocks(S, 256), 256, 0, st>>>(a, T, axis, maxleaf, nleft, non, lon, ntmp, nleaf);
+        k_make_children<<<blocks(S, 256), 256, 0, st>>>(a, T, axis, maxleaf, nleft, kind_off, ntmp, nleaf);
         ntmp += new_nodes;
         nleaf += new_leaves;
         lvl.push_back(ntmp);

@@ -183,15 +183,24 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, RuleOut 
 }
 
 // Each level runs in two passes without atomics: k_bfs_count sums the outputs of
-// every tile of kBfsTile pairs, the host scans those sums (exact output sizes), and
-// k_bfs_write re-applies the rule and writes each output at its scanned position.
+// every tile of kBfsTile pairs, one device scan turns those sums into tile offsets (and
+// the exact output sizes, which the host reads), and k_bfs_write re-applies the rule and
+// writes each output at its scanned position.
 // Within a tile the outputs keep the input order (item k * kBfsThreads + thread), so
 // neighbouring threads store to neighbouring addresses.
 constexpr int kBfsThreads = 256;
 constexpr int kBfsPerThread = 4;
 constexpr int kBfsTile = kBfsThreads * kBfsPerThread;
 
-__global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, int64_t *__restrict__ tile_cnt, int ntile)
+// output counts of one tile (child pairs, P2P tasks, M2L tasks), scanned together
+struct Cnt3 {
+    int64_t k, p, m;
+};
+struct AddCnt3 {
+    __host__ __device__ Cnt3 operator()(const Cnt3 &x, const Cnt3 &y) const { return {x.k + y.k, x.p + y.p, x.m + y.m}; }
+};
+
+__global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, Cnt3 *__restrict__ tile_cnt, int ntile)
 {
     typedef hipcub::BlockReduce<int, kBfsThreads> Reduce;
     __shared__ typename Reduce::TempStorage tmp;
@@ -212,9 +221,8 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, int64_t *_
     __syncthreads();
     int tm = Reduce(tmp).Sum(cm);
     if (threadIdx.x == 0) {
-        tile_cnt[blockIdx.x] = tk;
-        tile_cnt[(ntile + 1) + blockIdx.x] = tp;
-        tile_cnt[2 * (ntile + 1) + blockIdx.x] = tm;
+        tile_cnt[blockIdx.x] = Cnt3{tk, tp, tm};
+        if (blockIdx.x == 0) tile_cnt[ntile] = Cnt3{0, 0, 0};  // the scan's last entry is the total
     }
 }
 
@@ -223,7 +231,7 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, int64_t *_
 // offset of every item within the tile's output.
 __device__ __forceinline__ int field(uint64_t v, int k) { return (int)((v >> (16 * k)) & 0xffff); }
 
-__global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const int64_t *__restrict__ tile_off, int ntile)
+__global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const Cnt3 *__restrict__ tile_off)
 {
     static_assert(kBfsPerThread <= 4 && 4 * kBfsThreads < 65536, "16-bit count fields");
     typedef hipcub::BlockScan<uint64_t, kBfsThreads> Scan;
@@ -246,9 +254,10 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const int6
     Scan(tmp).ExclusiveSum(cp, op, tp);
     __syncthreads();
     Scan(tmp).ExclusiveSum(cm, om, tm);
-    const int64_t bk = tile_off[blockIdx.x];
-    const int64_t bp = a.p2p_base + tile_off[(ntile + 1) + blockIdx.x];
-    const int64_t bm = a.m2l_base + tile_off[2 * (ntile + 1) + blockIdx.x];
+    const Cnt3 to = tile_off[blockIdx.x];
+    const int64_t bk = to.k;
+    const int64_t bp = a.p2p_base + to.p;
+    const int64_t bm = a.m2l_base + to.m;
     int sk = 0, sp = 0, sm = 0;  // outputs of the tile's earlier items
 #pragma unroll
     for (int k = 0; k < kBfsPerThread; k++) {
@@ -272,16 +281,6 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const int6
         sp += field(tp, k);
         sm += field(tm, k);
     }
-}
-
-__global__ void k_zero_i64(int64_t *p) { *p = 0; }
-
-static void scan64(Ctx &c, const int64_t *in, int64_t *out, int n)
-{
-    size_t bytes = 0;
-    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, c.stream));
-    void *tmp = grow(c.scan_tmp, bytes);
-    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, c.stream));
 }
 
 // first frontier: (root, root, shift) of every box for the shifts [lo, hi), or one
@@ -399,18 +398,21 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     int levels = 0;
     while (n_in > 0) {
         const int ntile = (int)((n_in + kBfsTile - 1) / kBfsTile);
-        int64_t *tile_cnt = reinterpret_cast<int64_t *>(grow(c.cnt_tmp, sizeof(int64_t) * 6 * (size_t)(ntile + 1)));
-        int64_t *tile_off = tile_cnt + 3 * (ntile + 1);
+        Cnt3 *tile_cnt = reinterpret_cast<Cnt3 *>(grow(c.cnt_tmp, sizeof(Cnt3) * 2 * (size_t)(ntile + 1)));
+        Cnt3 *tile_off = tile_cnt + (ntile + 1);
         a.in = reinterpret_cast<uint64_t *>(c.bfs_a.p);
         a.n_in = n_in;
+        // one count pass, one scan of the three counts, one copy of the three totals
         k_bfs_count<<<ntile, kBfsThreads, 0, st>>>(a, tile_cnt, ntile);
-        for (int q = 0; q < 3; q++) {
-            k_zero_i64<<<1, 1, 0, st>>>(tile_cnt + q * (ntile + 1) + ntile);
-            scan64(c, tile_cnt + q * (ntile + 1), tile_off + q * (ntile + 1), ntile + 1);
+        {
+            size_t bytes = 0;
+            PH_CHECK(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, tile_cnt, tile_off, AddCnt3(), Cnt3{0, 0, 0},
+                                                       ntile + 1, st));
+            void *tmp = grow(c.scan_tmp, bytes);
+            PH_CHECK(hipcub::DeviceScan::ExclusiveScan(tmp, bytes, tile_cnt, tile_off, AddCnt3(), Cnt3{0, 0, 0},
+                                                       ntile + 1, st));
         }
-        for (int q = 0; q < 3; q++)
-            PH_CHECK(hipMemcpyAsync(pin + q, tile_off + q * (ntile + 1) + ntile, sizeof(int64_t), hipMemcpyDeviceToHost,
-                                    st));
+        PH_CHECK(hipMemcpyAsync(pin, tile_off + ntile, sizeof(Cnt3), hipMemcpyDeviceToHost, st));
         PH_CHECK(hipStreamSynchronize(st));
         const int64_t n_out = pin[0], add_p2p = pin[1], add_m2l = pin[2];
         grow_keep(c.bfs_b, sizeof(uint64_t) * std::max<int64_t>(n_out, 1), 0, st);
@@ -422,7 +424,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
         a.p2p_base = n_p2p;
         a.m2l_base = n_m2l;
-        k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off, ntile);
+        k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off);
         n_in = n_out;
         n_p2p += add_p2p;
         n_m2l += add_m2l;
