@@ -47,18 +47,20 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
+// C is an upper bound of the level's chunk count (S + ceil(n / kChunk)), so the host need
+// not read the count back; the kernels take the exact count, seg_chunk[S], from the device
 struct LevelArgs {
     const double *coord;  // the split axis of this level
     const int *t_start, *t_count;
     int seg0, S, C;
-    const int *seg_chunk;  // [S+1]
+    const int *seg_chunk;  // [S+1], exclusive scan of the chunks per segment
 };
 
 __global__ void __launch_bounds__(256) k_chunk_sum(LevelArgs a, double *__restrict__ partial)
 {
     int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     int lane = threadIdx.x % kWave;
-    if (c >= a.C) return;
+    if (c >= a.seg_chunk[a.S]) return;
     int s = find_seg(a.seg_chunk, a.S, c);
     int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
     int b = st + (c - a.seg_chunk[s]) * kChunk;
@@ -99,7 +101,7 @@ __global__ void __launch_bounds__(256) k_chunk_count(LevelArgs a, const double *
     int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     int lane = threadIdx.x % kWave;
     if (c > a.C) return;
-    if (c == a.C) {  // sentinel so that the exclusive scan ends with the total
+    if (c >= a.seg_chunk[a.S]) {  // zeros up to the bound: the exclusive scan ends with the total
         if (lane == 0) rcnt[c] = 0;
         return;
     }
@@ -145,7 +147,7 @@ __global__ void __launch_bounds__(256) k_scatter(LevelArgs a, const double *__re
 {
     int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     int lane = threadIdx.x % kWave;
-    if (c >= a.C) return;
+    if (c >= a.seg_chunk[a.S]) return;
     int s = find_seg(a.seg_chunk, a.S, c);
     int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
     int ci = c - a.seg_chunk[s];
@@ -538,10 +540,9 @@ void build_tree(Ctx &c)
             int *nch = ensure<int>(c.seg_nn, S + 1);
             k_nchunks<<<blocks(S + 1, 256), 256, 0, st>>>(T.t_count, seg0, S, nch);
             exclusive_scan(c, nch, seg_chunk, S + 1);
-            PH_CHECK(hipMemcpyAsync(pin, seg_chunk + S, sizeof(int), hipMemcpyDeviceToHost, st));
-            PH_CHECK(hipStreamSynchronize(st));
         }
-        int C = pin[0];
+        // sum over segments of ceil(count / kChunk) <= S + ceil(n / kChunk)
+        const int C = S + (int)((c.n + kChunk - 1) / kChunk);
         const double *coord = axis == 0 ? A.x : (axis == 1 ? A.y : A.z);
         LevelArgs a{coord, T.t_start, T.t_count, seg0, S, C, seg_chunk};
         double *partial = ensure<double>(c.chunk_part, C + 1);
