@@ -105,6 +105,7 @@ struct Ctx {
     DevBuf err;                    // device error flag (int)
     DevBuf stats;                  // device counters (int64)
     DevBuf pinned;                 // small host-pinned staging
+    hipEvent_t counts_ready = nullptr;  // a count copied to `pinned` while later work runs
 
     int64_t n_p2p = 0, n_m2l = 0;  // list lengths
     bool lists_valid = false;      // CSR lists built by a walk on the current tree

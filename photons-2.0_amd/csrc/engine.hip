@@ -137,6 +137,7 @@ int photons_create(photons_ctx **out, const photons_params *prm, int device, voi
             x->c.pinned.cap = 4096;
             grow(x->c.err, sizeof(int));
             for (int i = 0; i < 16; i++) PH_CHECK(hipEventCreate(&x->c.timing.ev[i]));
+            PH_CHECK(hipEventCreateWithFlags(&x->c.counts_ready, hipEventDisableTiming));
         } catch (...) {
             photons_destroy(x);  // releases whatever was created; null members are skipped
             throw;
@@ -168,6 +169,7 @@ int photons_destroy(photons_ctx *x)
         for (DevBuf *b : bufs) release(*b);
         if (c.pinned.p) (void)hipHostFree(c.pinned.p);
         for (int i = 0; i < 16; i++) (void)hipEventDestroy(c.timing.ev[i]);
+        if (c.counts_ready) (void)hipEventDestroy(c.counts_ready);
         if (c.own_stream) (void)hipStreamDestroy(c.stream);
         delete x;
         return 0;

@@ -117,11 +117,18 @@ __global__ void __launch_bounds__(256) k_chunk_count(LevelArgs a, const double *
     if (lane == 0) rcnt[c] = n;
 }
 
-__global__ void k_seg_counts(LevelArgs a, const int *__restrict__ rpre, int *__restrict__ nleft,
-                             int *__restrict__ flag)
+// per segment: particles left of the split, the degenerate-partition flag, and the kinds
+// of its two children (new nodes in the high 32 bits, new leaves in the low 32 bits, packed
+// so that one scan gives both offsets; the leaf count stays below 2^32, so no carry crosses)
+__global__ void k_seg_counts(LevelArgs a, const int *__restrict__ rpre, int maxleaf, int *__restrict__ nleft,
+                             int *__restrict__ flag, uint64_t *__restrict__ kinds)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.S) return;
+    if (s > a.S) return;
+    if (s == a.S) {  // the scan's last entry is the total
+        kinds[s] = 0;
+        return;
+    }
     int ct = a.t_count[a.seg0 + s];
     int nr = rpre[a.seg_chunk[s + 1]] - rpre[a.seg_chunk[s]];
     int nl = ct - nr, f = 0;
@@ -134,6 +141,8 @@ __global__ void k_seg_counts(LevelArgs a, const int *__restrict__ rpre, int *__r
     }
     nleft[s] = nl;
     flag[s] = f;
+    const int k = (nl > maxleaf) + (ct - nl > maxleaf);
+    kinds[s] = ((uint64_t)k << 32) | (uint64_t)(2 - k);
 }
 
 struct Particles {
@@ -179,22 +188,6 @@ __global__ void __launch_bounds__(256) k_scatter(LevelArgs a, const double *__re
         rbase += nr;
         lbase += nact - nr;
     }
-}
-
-// new nodes (high 32 bits) and new leaves (low 32 bits) of each segment, packed so one
-// scan gives both offsets (the leaf count stays below 2^32, so no carry crosses over)
-__global__ void k_child_kinds(LevelArgs a, const int *__restrict__ nleft, int maxleaf, uint64_t *__restrict__ kinds)
-{
-    int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > a.S) return;
-    if (s == a.S) {
-        kinds[s] = 0;
-        return;
-    }
-    int ct = a.t_count[a.seg0 + s];
-    int c0 = nleft[s], c1 = ct - c0;
-    int k = (c0 > maxleaf) + (c1 > maxleaf);
-    kinds[s] = ((uint64_t)k << 32) | (uint64_t)(2 - k);
 }
 
 struct TreeTmp {
@@ -555,12 +548,9 @@ void build_tree(Ctx &c)
         k_mean<<<blocks(S, kWavesPerBlock), 256, 0, st>>>(a, partial, T.t_split);
         k_chunk_count<<<blocks(C + 1, kWavesPerBlock), 256, 0, st>>>(a, T.t_split, rcnt);
         exclusive_scan(c, rcnt, rpre, C + 1);
-        k_seg_counts<<<blocks(S, 256), 256, 0, st>>>(a, rpre, nleft, flag);
-        k_scatter<<<wb, 256, 0, st>>>(a, T.t_split, rpre, nleft, flag, A, B);
-        std::swap(A, B);  // the children's particles are in B now
         uint64_t *kinds = ensure<uint64_t>(c.seg_nn, S + 1);
         uint64_t *kind_off = ensure<uint64_t>(c.seg_non, S + 1);
-        k_child_kinds<<<blocks(S + 1, 256), 256, 0, st>>>(a, nleft, maxleaf, kinds);
+        k_seg_counts<<<blocks(S + 1, 256), 256, 0, st>>>(a, rpre, maxleaf, nleft, flag, kinds);
         {
             size_t bytes = 0;
             PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, kinds, kind_off, S + 1, st));
@@ -568,7 +558,11 @@ void build_tree(Ctx &c)
             PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, kinds, kind_off, S + 1, st));
         }
         PH_CHECK(hipMemcpyAsync(pin, kind_off + S, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        PH_CHECK(hipStreamSynchronize(st));
+        PH_CHECK(hipEventRecord(c.counts_ready, st));
+        // the partition runs while the host reads the counts
+        k_scatter<<<wb, 256, 0, st>>>(a, T.t_split, rpre, nleft, flag, A, B);
+        std::swap(A, B);  // the children's particles are in B now
+        PH_CHECK(hipEventSynchronize(c.counts_ready));
         const uint64_t tot = *reinterpret_cast<const uint64_t *>(pin);
         int new_nodes = (int)(tot >> 32), new_leaves = (int)(uint32_t)tot;
         if (ntmp + new_nodes > T.tcap) {
