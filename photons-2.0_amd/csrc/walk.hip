@@ -413,18 +413,37 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
                                                        ntile + 1, st));
         }
         PH_CHECK(hipMemcpyAsync(pin, tile_off + ntile, sizeof(Cnt3), hipMemcpyDeviceToHost, st));
-        PH_CHECK(hipStreamSynchronize(st));
-        const int64_t n_out = pin[0], add_p2p = pin[1], add_m2l = pin[2];
-        grow_keep(c.bfs_b, sizeof(uint64_t) * std::max<int64_t>(n_out, 1), 0, st);
-        const size_t np = std::max<int64_t>(n_p2p + add_p2p, 1), nm = std::max<int64_t>(n_m2l + add_m2l, 1);
-        a.p2p_t = static_cast<uint32_t *>(grow_keep(c.p2p_keys, 4 * np, 4 * n_p2p, st));
-        a.p2p_s = static_cast<int32_t *>(grow_keep(c.p2p_src, 4 * np, 4 * n_p2p, st));
-        a.m2l_t = static_cast<uint32_t *>(grow_keep(c.m2l_keys, 4 * nm, 4 * n_m2l, st));
-        a.m2l_s = static_cast<int32_t *>(grow_keep(c.m2l_src, 4 * nm, 4 * n_m2l, st));
-        a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
+        PH_CHECK(hipEventRecord(c.counts_ready, st));
         a.p2p_base = n_p2p;
         a.m2l_base = n_m2l;
-        k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off);
+        // A pair has at most 4 children, 4 P2P tasks and 1 M2L task.  When the buffers
+        // already hold that bound (repeated evaluations keep their capacity) the write is
+        // queued at once and the host reads the counts while it runs; otherwise the
+        // buffers grow to the exact counts first.
+        const bool fits = c.bfs_b.cap >= sizeof(uint64_t) * 4 * (size_t)n_in &&
+                          c.p2p_keys.cap >= 4 * (size_t)(n_p2p + 4 * n_in) &&
+                          c.p2p_src.cap >= 4 * (size_t)(n_p2p + 4 * n_in) &&
+                          c.m2l_keys.cap >= 4 * (size_t)(n_m2l + n_in) && c.m2l_src.cap >= 4 * (size_t)(n_m2l + n_in);
+        if (fits) {
+            a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
+            a.p2p_t = static_cast<uint32_t *>(c.p2p_keys.p);
+            a.p2p_s = static_cast<int32_t *>(c.p2p_src.p);
+            a.m2l_t = static_cast<uint32_t *>(c.m2l_keys.p);
+            a.m2l_s = static_cast<int32_t *>(c.m2l_src.p);
+            k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off);
+        }
+        PH_CHECK(hipEventSynchronize(c.counts_ready));
+        const int64_t n_out = pin[0], add_p2p = pin[1], add_m2l = pin[2];
+        if (!fits) {
+            grow_keep(c.bfs_b, sizeof(uint64_t) * std::max<int64_t>(n_out, 1), 0, st);
+            const size_t np = std::max<int64_t>(n_p2p + add_p2p, 1), nm = std::max<int64_t>(n_m2l + add_m2l, 1);
+            a.p2p_t = static_cast<uint32_t *>(grow_keep(c.p2p_keys, 4 * np, 4 * n_p2p, st));
+            a.p2p_s = static_cast<int32_t *>(grow_keep(c.p2p_src, 4 * np, 4 * n_p2p, st));
+            a.m2l_t = static_cast<uint32_t *>(grow_keep(c.m2l_keys, 4 * nm, 4 * n_m2l, st));
+            a.m2l_s = static_cast<int32_t *>(grow_keep(c.m2l_src, 4 * nm, 4 * n_m2l, st));
+            a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
+            k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off);
+        }
         n_in = n_out;
         n_p2p += add_p2p;
         n_m2l += add_m2l;
