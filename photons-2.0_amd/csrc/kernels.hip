@@ -239,6 +239,7 @@ struct P2PArgs {
     int accumulate;       // out += instead of out =
     int out_caller_order; // scatter through pidx into out[3*pidx]; else tree order
     double *out;          // [n][3]
+    const int32_t *order; // target leaf of each launch slot (heaviest first)
 };
 
 __device__ __forceinline__ void shift_vec(int sid, double L, double &sx, double &sy, double &sz)
@@ -277,6 +278,34 @@ __device__ __forceinline__ int p2p_target(int b, int nb)
     if (b >= full) return b;
     const int x = b % kXcd, k = b / kXcd;
     return ((k / kTile) * kXcd + x) * kTile + k % kTile;
+}
+
+// Launch order of the P2P workgroups: target leaves by decreasing work (list length times
+// particle count, a stable sort, so equal work keeps the depth-first order).  The longest
+// leaves of dense clusters take ~10-20 ms alone; started last they would leave the GPU
+// part-idle at the end of the launch.
+__global__ void k_leaf_work(int nleaf, const int64_t *__restrict__ off, const CellInfo *__restrict__ info,
+                            uint32_t *__restrict__ key, int32_t *__restrict__ val)
+{
+    const int T = blockIdx.x * blockDim.x + threadIdx.x;
+    if (T >= nleaf) return;
+    const uint64_t w = (uint64_t)(off[T + 1] - off[T]) * (uint64_t)info[T].npart;
+    key[T] = ~(uint32_t)min(w, (uint64_t)0xffffffffu);  // ascending keys = decreasing work
+    val[T] = T;
+}
+
+static void p2p_order(Ctx &c)
+{
+    const int n = c.nleaf;
+    hipStream_t st = c.stream;
+    char *base = static_cast<char *>(grow(c.p2p_order, 16 * (size_t)n));
+    uint32_t *k0 = reinterpret_cast<uint32_t *>(base), *k1 = k0 + n;
+    int32_t *v0 = reinterpret_cast<int32_t *>(k1 + n), *v1 = v0 + n;
+    k_leaf_work<<<(n + 255) / 256, 256, 0, st>>>(n, dptr<int64_t>(c.p2p_off), dptr<CellInfo>(c.info), k0, v0);
+    size_t bytes = 0;
+    PH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, k0, k1, v0, v1, (size_t)n, 0, 32, st));
+    void *tmp = grow(c.p2p_order_tmp, bytes);
+    PH_CHECK(rocprim::radix_sort_pairs(tmp, bytes, k0, k1, v0, v1, (size_t)n, 0, 32, st));
 }
 
 // FP32 pair force per interaction, about 28 VALU instructions:
@@ -400,7 +429,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     float *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ float ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
-    const int T = p2p_target(blockIdx.x, gridDim.x);
+    const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
     const int lane = threadIdx.x;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
@@ -584,7 +613,7 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     double *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ double ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
-    const int T = p2p_target(blockIdx.x, gridDim.x);
+    const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
     const int lane = threadIdx.x;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
@@ -854,6 +883,8 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     a.accumulate = accumulate;
     a.out_caller_order = caller_order;
     a.out = out;
+    if (c.nleaf > 0) p2p_order(c);
+    a.order = reinterpret_cast<const int32_t *>(static_cast<char *>(c.p2p_order.p) + 12 * (size_t)c.nleaf);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
     hipStream_t st = c.stream;
 #define P2P_L(PP)                                              \
