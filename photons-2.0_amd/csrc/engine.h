@@ -95,6 +95,7 @@ struct Ctx {
     DevBuf p2p_off, p2p_src, m2l_off, m2l_src, cnt_tmp;
     DevBuf bfs_a, bfs_b, p2p_keys, m2l_keys;  // walk frontier and task keys
     DevBuf p2p_order, p2p_order_tmp;          // P2P launch order (heaviest target leaves first)
+    DevBuf bfs_act;                           // the walk rule's decision per frontier pair
     DevBuf m2l_item_off, m2l_item_tgt, m2l_part;  // balanced M2L work items
     DevBuf scan_tmp;
     // build temporaries (level order)

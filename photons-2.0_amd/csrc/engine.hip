@@ -164,7 +164,7 @@ int photons_destroy(photons_ctx *x)
                           &c.seg_flag, &c.seg_nn, &c.seg_non, &c.t_size, &c.t_lsize,
                           &c.t_dfs, &c.t_lfirst, &c.t_final, &c.l_final, &c.err, &c.stats, &c.bfs_a, &c.bfs_b,
                           &c.p2p_keys, &c.m2l_keys, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off,
-                          &c.d_box_len, &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp,
+                          &c.d_box_len, &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp, &c.bfs_act,
                           &x->io_pos, &x->io_acc, &x->op_a, &x->op_b, &x->op_c, &x->op_d, &x->op_e};
         for (DevBuf *b : bufs) release(*b);
         if (c.pinned.p) (void)hipHostFree(c.pinned.p);

@@ -55,6 +55,7 @@ struct BfsArgs {
     uint32_t *p2p_t, *m2l_t;  // task targets
     int32_t *p2p_s, *m2l_s;   // task source entries
     int64_t p2p_base, m2l_base;  // tasks emitted by earlier levels
+    uint8_t *act;                // [n_in] the rule's decision per pair, from the count pass
 };
 
 __host__ __device__ inline void shift_of(int sid, double L, double &sx, double &sy, double &sz)
@@ -113,33 +114,28 @@ __device__ __forceinline__ void put_child(const BfsArgs &a, RuleOut &o, int c, i
     }
 }
 
-// The rule applied to one pair (src/fmm.c:406-712, src/remotes.c:213-552): its
-// children (up to 4, leaf-leaf children as P2P tasks), its P2P task or its M2L task.
-__device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, RuleOut &o)
+// The rule applied to one pair (src/fmm.c:406-712, src/remotes.c:213-552), in two parts:
+// bfs_decide reads the geometry and returns what the pair does; bfs_emit turns that
+// into its children (up to 4, leaf-leaf children as P2P tasks), its P2P task or its M2L
+// task, reading only child ids.  The count pass stores the decision (one byte per pair),
+// so the write pass does not load the geometry or evaluate the acceptance again.
+enum : uint8_t {
+    kActNone = 0,  // pruned
+    kActSelfLeaf,  // a leaf with itself: P2P
+    kActSelfNode,  // a node with itself: its 4 child pairs
+    kActLeafLeaf,  // two leaves: P2P, whatever the acceptance says
+    kActM2L,       // accepted (or a target leaf against a pruned image node): M2L
+    kActOpenA,     // open the target
+    kActOpenB      // open the source
+};
+
+__device__ __forceinline__ int bfs_decide(const BfsArgs &a, uint64_t pr)
 {
-    o.kid = o.p2p = 0;
-    o.has_m2l = 0;
     const int A = (int)(pr & 0x7ffffff), B = (int)((pr >> 27) & 0x7ffffff), sid = (int)(pr >> 54);
     const int nleaf = a.nleaf;
     const bool aleaf = A < nleaf, bleaf = B < nleaf;
-    if (sid == 0 && A == B) {  // self pair
-        if (aleaf) {
-            o.slot[0] = task_of(A, B, 0);
-            o.p2p = 1;
-        } else {
-            const CellInfo IA = a.info[A];
-            put_child(a, o, 0, IA.son0, IA.son0, 0);
-            put_child(a, o, 1, IA.son0, IA.son1, 0);
-            put_child(a, o, 2, IA.son1, IA.son0, 0);
-            put_child(a, o, 3, IA.son1, IA.son1, 0);
-        }
-        return;
-    }
-    if (aleaf && bleaf) {  // leaf-leaf pairs are always P2P
-        o.slot[0] = task_of(A, B, sid);
-        o.p2p = 1;
-        return;
-    }
+    if (sid == 0 && A == B) return aleaf ? kActSelfLeaf : kActSelfNode;
+    if (aleaf && bleaf) return kActLeafLeaf;  // leaf-leaf pairs are always P2P
     double sx = 0.0, sy = 0.0, sz = 0.0, L = 0.0;
     if (sid != 0) {
         L = a.bx.len_of(A);
@@ -150,32 +146,43 @@ __device__ __forceinline__ void bfs_rule(const BfsArgs &a, uint64_t pr, RuleOut 
     const double d1 = gA.c[1] - (gB.c[1] + sy);
     const double d2 = gA.c[2] - (gB.c[2] + sz);
     const int flag = mac(gA.w, gB.w, d0, d1, d2, a.theta2, a.rc2);
-    if (flag < 0) return;
+    if (flag < 0) return kActNone;
     bool pruned = false;
     if (sid != 0 && !bleaf) {
         const CellGeom box = {{0.5 * L, 0.5 * L, 0.5 * L}, {L, L, L}};
         pruned = let_pruned(box, gB, sx, sy, sz, a.rc, a.theta);
     }
-    int open;  // 1 target, 2 source
-    if (flag == 1 || (aleaf && pruned)) {
-        // accepted; in the image walk a target leaf against a pruned node takes M2L
-        // whatever the acceptance says (src/remotes.c:442)
+    // accepted; in the image walk a target leaf against a pruned node takes M2L whatever
+    // the acceptance says (src/remotes.c:442)
+    if (flag == 1 || (aleaf && pruned)) return kActM2L;
+    if (aleaf) return kActOpenB;
+    if (bleaf) return kActOpenA;
+    const bool bigger = (gA.w[0] + gA.w[1] + gA.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]);
+    return (bigger || pruned) ? kActOpenA : kActOpenB;  // ties open the source
+}
+
+__device__ __forceinline__ void bfs_emit(const BfsArgs &a, uint64_t pr, int act, RuleOut &o)
+{
+    o.kid = o.p2p = 0;
+    o.has_m2l = 0;
+    const int A = (int)(pr & 0x7ffffff), B = (int)((pr >> 27) & 0x7ffffff), sid = (int)(pr >> 54);
+    if (act == kActSelfLeaf || act == kActLeafLeaf) {
+        o.slot[0] = task_of(A, B, sid);
+        o.p2p = 1;
+    } else if (act == kActSelfNode) {
+        const CellInfo IA = a.info[A];
+        put_child(a, o, 0, IA.son0, IA.son0, 0);
+        put_child(a, o, 1, IA.son0, IA.son1, 0);
+        put_child(a, o, 2, IA.son1, IA.son0, 0);
+        put_child(a, o, 3, IA.son1, IA.son1, 0);
+    } else if (act == kActM2L) {
         o.m2l = task_of(A, B, sid);
         o.has_m2l = 1;
-        return;
-    } else if (aleaf) {
-        open = 2;
-    } else if (bleaf) {
-        open = 1;
-    } else {
-        const bool bigger = (gA.w[0] + gA.w[1] + gA.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]);
-        open = (bigger || pruned) ? 1 : 2;  // ties open the source
-    }
-    if (open == 1) {
+    } else if (act == kActOpenA) {
         const CellInfo IA = a.info[A];
         put_child(a, o, 0, IA.son0, B, sid);
         put_child(a, o, 1, IA.son1, B, sid);
-    } else {
+    } else if (act == kActOpenB) {
         const CellInfo IB = a.info[B];
         put_child(a, o, 0, A, IB.son0, sid);
         put_child(a, o, 1, A, IB.son1, sid);
@@ -209,7 +216,10 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, Cnt3 *__re
         const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
         if (i < a.n_in) {
             RuleOut o;
-            bfs_rule(a, a.in[i], o);
+            const uint64_t pr = a.in[i];
+            const int act = bfs_decide(a, pr);
+            a.act[i] = (uint8_t)act;
+            bfs_emit(a, pr, act, o);
             ck += __popc(o.kid);
             cp += __popc(o.p2p);
             cm += o.has_m2l;
@@ -243,7 +253,7 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const Cnt3
         const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
         o[k].kid = o[k].p2p = 0;
         o[k].has_m2l = 0;
-        if (i < a.n_in) bfs_rule(a, a.in[i], o[k]);
+        if (i < a.n_in) bfs_emit(a, a.in[i], a.act[i], o[k]);
         ck |= (uint64_t)__popc(o[k].kid) << (16 * k);
         cp |= (uint64_t)__popc(o[k].p2p) << (16 * k);
         cm |= (uint64_t)o[k].has_m2l << (16 * k);
@@ -402,6 +412,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         Cnt3 *tile_off = tile_cnt + (ntile + 1);
         a.in = reinterpret_cast<uint64_t *>(c.bfs_a.p);
         a.n_in = n_in;
+        a.act = static_cast<uint8_t *>(grow(c.bfs_act, (size_t)n_in));
         // one count pass, one scan of the three counts, one copy of the three totals
         k_bfs_count<<<ntile, kBfsThreads, 0, st>>>(a, tile_cnt, ntile);
         {
