@@ -54,7 +54,13 @@ struct LevelArgs {
     const int *t_start, *t_count;
     int seg0, S, C;
     const int *seg_chunk;  // [S+1], exclusive scan of the chunks per segment
+    int axis;              // 0, 1, 2: which of x, y, z coord is
 };
+
+__device__ __forceinline__ double axis_value(int axis, double x, double y, double z)
+{
+    return axis == 0 ? x : (axis == 1 ? y : z);
+}
 
 __global__ void __launch_bounds__(256) k_chunk_sum(LevelArgs a, double *__restrict__ partial)
 {
@@ -167,26 +173,44 @@ __global__ void __launch_bounds__(256) k_scatter(LevelArgs a, const double *__re
     int nl = nleft[s];
     int rbase = rpre[c] - rpre[a.seg_chunk[s]];
     int lbase = ci * kChunk - rbase;
-    for (int i0 = b; i0 < e; i0 += kWave) {
-        int i = i0 + lane;
-        bool active = i < e;
-        bool right = false;
-        if (active) right = (f == 0) ? (a.coord[i] > split) : false;
-        uint64_t bal = __ballot(active && right);
-        uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-        int rr = __popcll(bal & lt);
-        int rl = lane - rr;
-        if (active) {
-            int d = (f != 0) ? i : (right ? st + nl + rbase + rr : st + lbase + rl);
-            dst.x[d] = src.x[i];
-            dst.y[d] = src.y[i];
-            dst.z[d] = src.z[i];
-            dst.idx[d] = src.idx[i];
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    // four groups of 64 particles per pass: their loads are all in flight before the
+    // first ballot, so a wave waits on memory once per 256 particles instead of per 64
+    constexpr int G = 4;
+    for (int i0 = b; i0 < e; i0 += G * kWave) {
+        double px[G] = {}, py[G] = {}, pz[G] = {};
+        int pi[G] = {};
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int i = i0 + g * kWave + lane;
+            if (i < e) {
+                px[g] = src.x[i];
+                py[g] = src.y[i];
+                pz[g] = src.z[i];
+                pi[g] = src.idx[i];
+            }
         }
-        int nact = min(kWave, e - i0);
-        int nr = __popcll(bal);
-        rbase += nr;
-        lbase += nact - nr;
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int i = i0 + g * kWave + lane;
+            const bool active = i < e;
+            const double v = axis_value(a.axis, px[g], py[g], pz[g]);
+            const bool right = active && f == 0 && v > split;
+            const uint64_t bal = __ballot(right);
+            const int rr = __popcll(bal & lt);
+            const int rl = lane - rr;
+            if (active) {
+                const int d = (f != 0) ? i : (right ? st + nl + rbase + rr : st + lbase + rl);
+                dst.x[d] = px[g];
+                dst.y[d] = py[g];
+                dst.z[d] = pz[g];
+                dst.idx[d] = pi[g];
+            }
+            const int nact = max(0, min(kWave, e - (i0 + g * kWave)));
+            const int nr = __popcll(bal);
+            rbase += nr;
+            lbase += nact - nr;
+        }
     }
 }
 
@@ -537,7 +561,7 @@ void build_tree(Ctx &c)
         // sum over segments of ceil(count / kChunk) <= S + ceil(n / kChunk)
         const int C = S + (int)((c.n + kChunk - 1) / kChunk);
         const double *coord = axis == 0 ? A.x : (axis == 1 ? A.y : A.z);
-        LevelArgs a{coord, T.t_start, T.t_count, seg0, S, C, seg_chunk};
+        LevelArgs a{coord, T.t_start, T.t_count, seg0, S, C, seg_chunk, axis};
         double *partial = ensure<double>(c.chunk_part, C + 1);
         int *rcnt = ensure<int>(c.chunk_rcnt, 2 * (size_t)(C + 1));
         int *rpre = rcnt + (C + 1);
