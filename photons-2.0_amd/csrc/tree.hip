@@ -72,7 +72,17 @@ __global__ void __launch_bounds__(256) k_chunk_sum(LevelArgs a, double *__restri
     int b = st + (c - a.seg_chunk[s]) * kChunk;
     int e = min(b + kChunk, st + ct);
     double v = 0.0;
-    for (int i = b + lane; i < e; i += kWave) v += a.coord[i];
+    // four loads in flight per step, added in the same order as one at a time
+    int i = b + lane;
+    for (; i + 3 * kWave < e; i += 4 * kWave) {
+        const double x0 = a.coord[i], x1 = a.coord[i + kWave], x2 = a.coord[i + 2 * kWave],
+                     x3 = a.coord[i + 3 * kWave];
+        v += x0;
+        v += x1;
+        v += x2;
+        v += x3;
+    }
+    for (; i < e; i += kWave) v += a.coord[i];
     v = wave_sum(v);
     if (lane == 0) partial[c] = v;
 }
@@ -117,7 +127,11 @@ __global__ void __launch_bounds__(256) k_chunk_count(LevelArgs a, const double *
     int b = st + (c - a.seg_chunk[s]) * kChunk;
     int e = min(b + kChunk, st + ct);
     int n = 0;
-    for (int i = b + lane; i < e; i += kWave) n += a.coord[i] > split;
+    int i = b + lane;
+    for (; i + 3 * kWave < e; i += 4 * kWave)
+        n += (a.coord[i] > split) + (a.coord[i + kWave] > split) + (a.coord[i + 2 * kWave] > split) +
+             (a.coord[i + 3 * kWave] > split);
+    for (; i < e; i += kWave) n += a.coord[i] > split;
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) n += __shfl_xor(n, o, kWave);
     if (lane == 0) rcnt[c] = n;
