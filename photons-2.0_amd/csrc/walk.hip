@@ -212,18 +212,35 @@ __global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, Cnt3 *__re
     typedef hipcub::BlockReduce<int, kBfsThreads> Reduce;
     __shared__ typename Reduce::TempStorage tmp;
     int ck = 0, cp = 0, cm = 0;
+    // all items' pairs and decisions first, the byte stores last: a byte store may alias
+    // anything, and between the loads it would keep them from overlapping
+    uint64_t pr[kBfsPerThread];
+    int act[kBfsPerThread];
+#pragma unroll
+    for (int k = 0; k < kBfsPerThread; k++) {
+        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
+        pr[k] = i < a.n_in ? a.in[i] : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < kBfsPerThread; k++) {
+        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
+        act[k] = i < a.n_in ? bfs_decide(a, pr[k]) : kActNone;
+    }
+#pragma unroll
     for (int k = 0; k < kBfsPerThread; k++) {
         const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
         if (i < a.n_in) {
             RuleOut o;
-            const uint64_t pr = a.in[i];
-            const int act = bfs_decide(a, pr);
-            a.act[i] = (uint8_t)act;
-            bfs_emit(a, pr, act, o);
+            bfs_emit(a, pr[k], act[k], o);
             ck += __popc(o.kid);
             cp += __popc(o.p2p);
             cm += o.has_m2l;
         }
+    }
+#pragma unroll
+    for (int k = 0; k < kBfsPerThread; k++) {
+        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
+        if (i < a.n_in) a.act[i] = (uint8_t)act[k];
     }
     int tk = Reduce(tmp).Sum(ck);
     __syncthreads();
