@@ -351,7 +351,7 @@ def main():
                          "to scale hardware byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (each box or domain runs on one GPU);"
+        print("bench.py measures one GPU: the library has no multi-GPU path (each box or domain runs on one GPU);"
               " run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 
