@@ -45,15 +45,18 @@ FORCE_CASES = {
     "lcdm_demo": _lcdm,
     # 262,144 clustered particles: the parity case at scale (every 64th particle stored)
     "clus64_pm64": lambda: _clustered(64, 64),
+    # other leaf size, opening angle and cutoff on clustered inputs
+    "clus32_leaf16_th0.5_rc4.5": lambda: _clustered(32, 32, rc_factor=4.5, seed=11, theta=0.5, maxleaf=16),
+    "clus48_pm48_th0.3": lambda: _clustered(48, 48, seed=13, theta=0.3),
 }
 
 # stored stride of the sampled accelerations (default 8)
-STRIDE = {"clus64_pm64": 64}
+STRIDE = {"clus64_pm64": 64, "clus48_pm48_th0.3": 32}
 
 # cases whose oracle run takes about a minute on one core: the CPU suite checks them
 # only with --full (the reference and the oracle agreed to 5e-15 on them when the
 # vectors were written); the GPU parity test compares with the stored vectors
-LARGE_CASES = {"clus64_pm64"}
+LARGE_CASES = {"clus64_pm64", "clus48_pm48_th0.3"}
 
 
 def case_inputs(name):
