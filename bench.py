@@ -169,6 +169,29 @@ def run_ensemble(args, torch):
         res[name] = dict(wall=np.array(wall), acc=d_acc.cpu().numpy(), leaf=leaf, stage=stage,
                          table={f"leaf{k[0]}_stage{k[1]}": v for k, v in table.items()}, stats=ctx.stats())
         ctx.close()
+    # the same evaluation planned and captured in a hipGraph (autotuned FP32 parameters):
+    # replays timed on the host clock per launch + wait, and on the GPU (HIP events)
+    p32 = ic.ForceParams(**{**prm.as_dict(), "maxleaf": res["fp32"]["leaf"]})
+    gctx = pg.Context(p32, pg.FP32_MIXED)
+    gctx.set_p2p_stage(res["fp32"]["stage"])
+    gctx.set_timing(True)
+    gctx.graph_create(counts, lengths, d_pos.data_ptr(), d_acc.data_ptr(), margin=0.25)
+    gwall, gms = [], []
+    for s in range(args.warmup + args.steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        gctx.graph_launch()
+        ovf, ms = gctx.graph_check()
+        if ovf:
+            raise RuntimeError(f"graph replay overflowed its plan ({ovf})")
+        if s >= args.warmup:
+            gwall.append(time.perf_counter() - t0)
+            gms.append(ms)
+            log(f"[fp32 graph] replay {s - args.warmup + 1}/{args.steps} {gwall[-1] * 1e3:.1f} ms (GPU {ms:.1f} ms)")
+    graph_same = bool(np.array_equal(d_acc.cpu().numpy(), res["fp32"]["acc"]))
+    gctx.graph_destroy()
+    gctx.close()
+    small = small_box_latency(torch, args)
     # the precision error at equal tree parameters: FP64 with the FP32 leaf size
     p64 = ic.ForceParams(**{**prm.as_dict(), "maxleaf": res["fp32"]["leaf"]})
     ref = pg.Context(p64, pg.FP64)
@@ -195,7 +218,48 @@ def run_ensemble(args, torch):
         "fp32_over_fp64": float(np.mean(res["fp64"]["wall"]) / np.mean(res["fp32"]["wall"])),
         "error_fp32_vs_fp64": {"p50": float(np.median(e)), "p99": float(np.percentile(e, 99)), "max": float(e.max())},
         "p2p_interactions": res["fp32"]["stats"]["p2p_interactions"],
+        "graph": {"what": "the FP32-mixed evaluation of all 64 boxes planned (capacities 25% above one measured "
+                          "evaluation) and captured in one hipGraph; one graph launch per evaluation, no host "
+                          "round trip inside it",
+                  "ms_per_step": float(np.mean(gwall) * 1e3), "gpu_ms": float(np.mean(gms)),
+                  "value": n / float(np.mean(gwall)), "same_as_ordinary": graph_same},
+        "small_box_latency_ms": small,
     }
+
+
+def small_box_latency(torch, args):
+    """one clustered box of 16^3 / 32^3 particles: ordinary evaluation (host round trips
+    per tree level and walk depth) against the replayed graph, wall clock per evaluation"""
+    out = {}
+    for ns in (16, 32):
+        n = ns ** 3
+        box = 500000.0 * (n / 256.0 ** 3) ** (1.0 / 3.0)
+        pos, _ = ic.clustered(n, box, seed=1)
+        prm = ic.force_params(n, box, ns, rc_factor=6.0, order=args.order)
+        d_pos = torch.from_numpy(np.ascontiguousarray(pos)).to("cuda")
+        d_acc = torch.empty_like(d_pos)
+        counts, lengths = np.array([n]), np.array([box])
+        ctx = pg.Context(prm, pg.FP32_MIXED)
+        reps = 50
+        for k in range(reps + 5):
+            if k == 5:
+                ctx.sync()
+                t0 = time.perf_counter()
+            ctx.force_boxes_device(counts, lengths, d_pos.data_ptr(), d_acc.data_ptr())
+        ctx.sync()
+        t_ord = (time.perf_counter() - t0) / reps * 1e3
+        ctx.graph_create(counts, lengths, d_pos.data_ptr(), d_acc.data_ptr(), margin=0.25)
+        for k in range(reps + 5):
+            if k == 5:
+                ctx.graph_check()
+                t0 = time.perf_counter()
+            ctx.graph_launch()
+        ctx.graph_check()
+        t_graph = (time.perf_counter() - t0) / reps * 1e3
+        ctx.close()
+        out[f"{ns}^3"] = {"ordinary": t_ord, "graph": t_graph}
+        log(f"{ns}^3 box: ordinary {t_ord:.3f} ms, replayed graph {t_graph:.3f} ms per evaluation")
+    return out
 
 
 def run_treepm(args, torch):

@@ -115,6 +115,28 @@ int photons_autotune_boxes(photons_ctx *ctx, int nbox, const int64_t *counts, co
                            const double *d_pos, double *d_acc, int reps, int *best_leaf, int *best_stage,
                            double *ms);
 
+/* A planned evaluation of a fixed set of boxes, captured in a hipGraph.  No
+ * counterpart in the reference, whose task pipeline never waits on a count
+ * (src/fmm.c:381-404, 909-984): here, too, one evaluation runs from the upload to
+ * P2P + L2P without a host round trip, an allocation or a memset, so that it can be
+ * captured and replayed.  photons_graph_create measures one ordinary evaluation of
+ * the boxes at d_pos, allocates every buffer `margin` (e.g. 0.25) above what it
+ * measured, runs the planned evaluation once and, with capture != 0, captures it into
+ * a hipGraph on the context's stream; d_acc then holds the force of d_pos.
+ * photons_graph_launch queues one replay (one graph launch; without capture the same
+ * launches one by one): it reads d_pos and writes d_acc (caller order), whose contents
+ * may change between launches (a drift), not their addresses or the box set.
+ * photons_graph_check waits and reports the overflow bits of the last launch: 0, or
+ * the particles have moved so far that the tree or the lists outgrew the plan, the
+ * result is incomplete and the graph must be created again; ms gets its GPU time when
+ * timing is on (else -1).  While a planned evaluation is held, the context refuses
+ * other evaluations; photons_graph_destroy releases it.                          */
+int photons_graph_create(photons_ctx *ctx, int nbox, const int64_t *counts, const double *lengths,
+                         const double *d_pos, double *d_acc, double margin, int capture);
+int photons_graph_launch(photons_ctx *ctx);
+int photons_graph_check(photons_ctx *ctx, int *overflow, double *ms);
+int photons_graph_destroy(photons_ctx *ctx);
+
 /* The same force call in two steps, as the reference splits it:
  *   photons_prepare_host   upload, tree, P2M and M2M   (fmm_prepare)
  *   photons_evaluate_host  walk, M2L, L2L, P2P and L2P (fmm_task + fmm_ext);
@@ -132,6 +154,9 @@ int photons_tree_cells(const photons_ctx *ctx, int32_t *npart, int32_t *ipart, i
                        double *center /* [ncell][3] */, double *width /* [ncell][3] */);
 int photons_p2p_list(const photons_ctx *ctx, int64_t cap, int32_t *tgt, int32_t *src, int8_t *shift, int64_t *count);
 int photons_m2l_list(const photons_ctx *ctx, int64_t cap, int32_t *tgt, int32_t *src, int8_t *shift, int64_t *count);
+/* entries per target: len[t] for every target leaf (which = 0, P2P) or cell
+ * (which = 1, M2L) of the last force call                                    */
+int photons_list_lengths(const photons_ctx *ctx, int which, int64_t *len);
 /* expansions of every cell, [ncell][nterms(p)] doubles (FP32-mixed values are
  * returned converted to kpc/h units)                                         */
 int photons_expansions(const photons_ctx *ctx, double *M, double *L);
@@ -183,6 +208,8 @@ int photons_server_add_box(photons_server *srv, int64_t n, double boxsize, const
                            const double *acc, int *box_id);
 int photons_server_remove_box(photons_server *srv, int box_id);
 int photons_server_submit(photons_server *srv, int box_id, double dkh, double dd, int64_t *request_id);
+/* waits for a request and reports it (latency: submit to completion, host clock);
+ * each request is waited on once, the report releases it                     */
 int photons_server_wait(photons_server *srv, int64_t request_id, double *latency_ms);
 int photons_server_drain(photons_server *srv);
 int photons_server_read_box(photons_server *srv, int box_id, double *pos, double *vel, double *acc);
@@ -200,6 +227,7 @@ double photons_drift_factor(double loga_i, double loga_f, double omega_m, double
  * reference's multi-rank slab exchange: the whole mesh lives on the GPU.      */
 typedef struct photons_pm photons_pm;
 const char *photons_pm_last_error(void);
+/* nside even, in [8, 1624]: the mesh indices are 32-bit (N^3 < 2^32)          */
 int photons_pm_create(photons_pm **pm, int nside, double boxsize, double rs, double mass, int device, void *stream);
 int photons_pm_destroy(photons_pm *pm);
 int photons_pm_force_device(photons_pm *pm, int64_t n, const double *d_pos, double *d_acc);
@@ -265,6 +293,14 @@ photons_ctx *photons_compat_context(void);
  * message of the failure stays readable here, e.g. from an atexit handler.
  * Empty while nothing has failed.                                              */
 const char *photons_gpu_last_error(void);
+
+/* phase timers of inc/photoNs.h:95-111 [s], written by the calls below as the
+ * reference writes them: dtime_prep (fmm_prepare), dtime_task (fmm_task),
+ * dtime_ext (fmm_ext), dtime_fmm (fmm_prepare + L2L), dtime_p2p (P2P + L2P kernel
+ * time; reset by fmm_prepare, src/fmm.c:182), dtime_m2l (M2L kernel time, summed
+ * over calls, src/fmm.c:906), dtime_fmm_remote (0: the image walks are fused with
+ * the home walk), dtime_pm (partmesh).  Kernel times are GPU times (HIP events). */
+extern double dtime_prep, dtime_task, dtime_ext, dtime_fmm, dtime_p2p, dtime_m2l, dtime_fmm_remote, dtime_pm;
 
 /* the force call, inc/fmm.h:11-22 and src/fmm.c:180-264, 329-360, 719-1087:
  *   fmm_construct   domain box [0, BOXSIZE)^3 (one rank)

@@ -81,6 +81,9 @@ def lib():
         L.orc_step.restype = C.c_int
         L.orc_step.argtypes = [C.POINTER(OrcParams), C.c_int, dp, dp, dp, C.c_double, C.c_double]
         L.orc_direct.argtypes = [C.POINTER(OrcParams), C.c_int, dp, C.c_int, ip, C.c_double, dp, C.c_int]
+        L.orc_force_leaves.restype = C.c_int64
+        L.orc_force_leaves.argtypes = [C.POINTER(OrcParams), C.c_int, dp, C.c_int, ip, dp, ip,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
         _lib = L
     return _lib
 
@@ -116,6 +119,26 @@ def force(pos, prm):
     if rc != 0:
         raise RuntimeError(f"orc_force failed ({rc})")
     return acc, st.as_dict()
+
+
+def force_leaves(pos, prm, leaves):
+    """the force on the particles of the given target leaves only (leaf ids of the
+    tree), with their P2P list lengths and the M2L entries of each leaf and its
+    ancestors: returns (acc [m][3], caller index [m], p2p_len [k], m2l_len [k]); the
+    particles come leaf after leaf in tree order"""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    sel = np.ascontiguousarray(leaves, dtype=np.int32)
+    p = params(prm)
+    cap = len(sel) * p.maxleaf
+    acc = np.zeros((cap, 3))
+    ids = np.zeros(cap, np.int32)
+    plen = np.zeros(len(sel), np.int64)
+    mlen = np.zeros(len(sel), np.int64)
+    m = lib().orc_force_leaves(C.byref(p), pos.shape[0], _dp(pos), len(sel), _ip(sel), _dp(acc), _ip(ids),
+                               plen.ctypes.data_as(C.POINTER(C.c_int64)), mlen.ctypes.data_as(C.POINTER(C.c_int64)))
+    if m < 0:
+        raise RuntimeError(f"orc_force_leaves failed ({m})")
+    return acc[:m], ids[:m], plen, mlen
 
 
 def kick_loga(loga_i, loga_f, omega_m=0.25, omega_x=0.75):
@@ -322,3 +345,24 @@ def ref_force(pos, prm, workdir):
     cnt = np.frombuffer(raw[24 * n: 24 * n + 48], dtype=np.int64)
     keys = ["idxP2P", "idxM2L", "walk_m2l_count", "p2p_count_remote", "nleaf", "nnode"]
     return acc, dict(zip(keys, [int(c) for c in cnt]))
+
+
+def ref_tree_multipoles(pos, prm, workdir):
+    """fmm_prepare of the reference FMM subset: the multipoles of btree[] (from
+    first_node on) and of leaf[] (from first_leaf on), p = 3"""
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    n = pos.shape[0]
+    d = prm if isinstance(prm, dict) else prm.as_dict()
+    fin = os.path.join(workdir, "ref_in.bin")
+    fout = os.path.join(workdir, "ref_tree.bin")
+    with open(fin, "wb") as f:
+        f.write(np.array([n, d["maxleaf"], 0, 0], np.int32).tobytes())
+        f.write(np.array([d["boxsize"], d["mass"], d["soft"], d["rs"], d["rc"], d["theta"]]).tobytes())
+        f.write(pos.tobytes())
+    subprocess.run([REF_HARNESS, "tree", fin, fout], check=True)
+    raw = open(fout, "rb").read()
+    nleaf, nnode = np.frombuffer(raw[:16], dtype=np.int64)
+    M = np.frombuffer(raw[16:], dtype=np.float64)
+    node_M = M[: nnode * 20].reshape(nnode, 20).copy()
+    leaf_M = M[nnode * 20:].reshape(nleaf, 20).copy()
+    return node_M, leaf_M

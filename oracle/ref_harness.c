@@ -8,11 +8,14 @@
  * driver's order (src/photoNs.c:97-116 without PM), and writes raw outputs.
  *
  *   refharness force <in> <out>   one short-range force evaluation
+ *   refharness tree  <in> <out>   fmm_prepare only: the multipoles of every node and leaf
  *   refharness ops   <in> <out>   single operator calls (p2m, m2m, m2l, l2l, l2p, acceptance)
  *
  * force input : int32 n, maxleaf, pad, pad; double boxsize, mass, soft, rs, rc, theta; double pos[n][3]
  * force output: double acc[n][3] (input order); int64 idxP2P, idxM2L, walk_m2l_count,
  *               p2p_count_remote, nleaf, nnode
+ * tree input  : as force; output: int64 nleaf, nnode; double M[nnode][NMULTI] of btree[]
+ *               from first_node on, then M[nleaf][NMULTI] of leaf[] from first_leaf on
  * ops input   : int32 count, then records (see run_ops); ops output: raw doubles / int32.
  */
 #include "photoNs.h"
@@ -51,7 +54,7 @@ static void setup_one_rank(void)
     domtree = (DomainNode *)calloc(1, sizeof(DomainNode));
 }
 
-static int run_force(FILE *in, FILE *out)
+static int run_force(FILE *in, FILE *out, int tree_only)
 {
     int32_t hdr[4];
     double prm[6];
@@ -79,6 +82,16 @@ static int run_force(FILE *in, FILE *out)
 
     fmm_construct();
     fmm_prepare();
+    if (tree_only) {
+        /* build_localtree, p2m of every leaf and walk_m2m(first_node) have run */
+        int64_t sz[2] = {(int64_t)(last_leaf - first_leaf), (int64_t)(last_node - first_node + 1)};
+        fwrite(sz, sizeof(int64_t), 2, out);
+        for (int k = first_node; k <= last_node; k++) fwrite(btree[k].M, sizeof(double), NMULTI, out);
+        for (int k = first_leaf; k < last_leaf; k++) fwrite(leaf[k].M, sizeof(double), NMULTI, out);
+        fmm_deconstruct();
+        free(part);
+        return 0;
+    }
     fmm_task();
     int64_t cnt[6];
     cnt[0] = (int64_t)idxP2P;
@@ -179,12 +192,15 @@ static int run_ops(FILE *in, FILE *out)
 int main(int argc, char **argv)
 {
     MPI_Init(&argc, &argv);
-    if (argc != 4) die("usage: refharness force|ops <in> <out>");
+    if (argc != 4) die("usage: refharness force|tree|ops <in> <out>");
     setup_one_rank();
     FILE *in = fopen(argv[2], "rb");
     FILE *out = fopen(argv[3], "wb");
     if (!in || !out) die("cannot open files");
-    int rc = strcmp(argv[1], "force") == 0 ? run_force(in, out) : run_ops(in, out);
+    int rc;
+    if (strcmp(argv[1], "force") == 0) rc = run_force(in, out, 0);
+    else if (strcmp(argv[1], "tree") == 0) rc = run_force(in, out, 1);
+    else rc = run_ops(in, out);
     fclose(in);
     fclose(out);
     MPI_Finalize();

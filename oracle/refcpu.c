@@ -505,6 +505,7 @@ typedef struct {
     int shift_id;
     double shift[3];
     double top_c[3], top_w[3]; /* domain box (toptree[this_domain]) */
+    const uint8_t *tflag;      /* NULL, or by cell id: walk only targets flagged here */
 } owalk;
 
 static int cell_id(const orc_tree *t, int c) { return isleaf(c) ? leafid(c) : t->nleaf + c; }
@@ -519,6 +520,7 @@ static void walk_local(owalk *w, int im, int jm)
 {
     const orc_tree *t = w->t;
     if (im == ORC_NONE || jm == ORC_NONE) return;
+    if (w->tflag && !w->tflag[cell_id(t, im)]) return; /* no selected leaf below the target */
     if (im == jm) {
         if (isleaf(im)) {
             list_push(&w->p2p, leafid(im), leafid(jm), 0);
@@ -587,6 +589,7 @@ static void walk_ext(owalk *w, int im, int jm)
 {
     const orc_tree *t = w->t;
     int8_t sh = (int8_t)w->shift_id;
+    if (w->tflag && !w->tflag[cell_id(t, im)]) return; /* no selected leaf below the target */
     if (isleaf(im) && isleaf(jm)) {
         list_push(&w->p2p, leafid(im), leafid(jm), sh);
         return;
@@ -641,9 +644,10 @@ static void shift_vec(int sid, double L, double *s)
     s[0] = s[1] = s[2] = 0.0;
 }
 
-static void run_walks(const orc_tree *t, owalk *w, int sid)
+static void run_walks(const orc_tree *t, owalk *w, int sid, const uint8_t *tflag)
 {
     memset(w, 0, sizeof *w);
+    w->tflag = tflag;
     w->t = t;
     w->shift_id = sid;
     shift_vec(sid, t->prm.boxsize, w->shift);
@@ -662,7 +666,7 @@ static int64_t collect(const orc_tree *t, int which, int32_t *tg, int32_t *sr, i
     int nshift = t->prm.periodic ? 27 : 1;
     for (int sid = 0; sid < nshift; sid++) {
         owalk w;
-        run_walks(t, &w, sid);
+        run_walks(t, &w, sid, NULL);
         olist *l = which ? &w.m2l : &w.p2p;
         for (int64_t k = 0; k < l->n; k++) {
             if (tg && total < cap) {
@@ -803,7 +807,7 @@ int orc_force(const orc_params *prm, int n, const double *pos, double *acc, orc_
     int nshift = prm->periodic ? 27 : 1;
     for (int sid = 0; sid < nshift; sid++) {
         owalk w;
-        run_walks(t, &w, sid);
+        run_walks(t, &w, sid, NULL);
         /* P2P tasks */
         for (int64_t k = 0; k < w.p2p.n; k++) {
             const oleaf *lt = &t->leaf[w.p2p.t[k]];
@@ -875,6 +879,128 @@ int orc_force(const orc_params *prm, int n, const double *pos, double *acc, orc_
     free(Mleaf); free(Lleaf); free(Mnode); free(Lnode); free(a); free(xyz);
     orc_free(t);
     return 0;
+}
+
+/* flag[cell] = 1 for the selected leaves and every node above one of them */
+static int flag_up(const orc_tree *t, int c, uint8_t *flag)
+{
+    if (c == ORC_NONE) return 0;
+    if (isleaf(c)) return flag[leafid(c)];
+    int f = flag_up(t, t->node[c].son[0], flag) | flag_up(t, t->node[c].son[1], flag);
+    flag[t->nleaf + c] = (uint8_t)f;
+    return f;
+}
+
+/* orc_force restricted to selected target leaves: the whole tree and every multipole,
+ * the walks of orc_force entered only for targets with a selected leaf below them (the
+ * same decisions, so the selected leaves get the same lists), M2L into those targets,
+ * L2L down, then P2P and L2P of the selected leaves alone.  acc_sel: the selected
+ * leaves' particles in tree order, leaf after leaf; id_sel: their caller indices;
+ * p2p_len / m2l_len: per selected leaf, the entries of its P2P list (local and image)
+ * and of the M2L lists of the leaf and its ancestors.  Returns the particle count, or
+ * < 0 on bad arguments. */
+int64_t orc_force_leaves(const orc_params *prm, int n, const double *pos, int nsel, const int32_t *sel,
+                         double *acc_sel, int32_t *id_sel, int64_t *p2p_len, int64_t *m2l_len)
+{
+    tab_init();
+    if (n < 1 || prm->order < 1 || prm->order > ORC_MAXP || prm->maxleaf < 2) return -1;
+    if (prm->periodic && n <= prm->maxleaf) return -2;
+    orc_tree *t = orc_build(prm, n, pos);
+    for (int k = 0; k < nsel; k++)
+        if (sel[k] < 0 || sel[k] >= t->nleaf) {
+            orc_free(t);
+            return -3;
+        }
+    const int ncell = t->nleaf + t->nnode;
+    int order = prm->order, nt = orc_nterms(order);
+    uint8_t *flag = (uint8_t *)calloc((size_t)ncell, 1);
+    for (int k = 0; k < nsel; k++) flag[sel[k]] = 1;
+    flag_up(t, 0, flag);
+    double *Mleaf = (double *)calloc((size_t)nt * t->nleaf, sizeof(double));
+    double *Lleaf = (double *)calloc((size_t)nt * t->nleaf, sizeof(double));
+    double *Mnode = (double *)calloc((size_t)nt * t->nnode, sizeof(double));
+    double *Lnode = (double *)calloc((size_t)nt * t->nnode, sizeof(double));
+    double *a = (double *)calloc((size_t)3 * n, sizeof(double)); /* tree order */
+    double *xyz = (double *)malloc(sizeof(double) * 3 * n);
+    int64_t *plen = (int64_t *)calloc((size_t)t->nleaf, sizeof(int64_t));
+    int64_t *mlen = (int64_t *)calloc((size_t)ncell, sizeof(int64_t));
+    for (int i = 0; i < n; i++)
+        for (int k = 0; k < 3; k++) xyz[3 * i + k] = t->p[i].x[k];
+    for (int l = 0; l < t->nleaf; l++)
+        orc_p2m(order, t->leaf[l].npart, xyz + 3 * t->leaf[l].ipart, prm->mass, t->leaf[l].center,
+                Mleaf + (size_t)nt * l);
+    walk_m2m(t, order, 0, Mleaf, Mnode);
+    int nshift = prm->periodic ? 27 : 1;
+    for (int sid = 0; sid < nshift; sid++) {
+        owalk w;
+        run_walks(t, &w, sid, flag);
+        for (int64_t k = 0; k < w.p2p.n; k++) {
+            const oleaf *lt = &t->leaf[w.p2p.t[k]];
+            const oleaf *ls = &t->leaf[w.p2p.s[k]];
+            plen[w.p2p.t[k]]++;
+            for (int q = 0; q < lt->npart; q++) {
+                int ip = lt->ipart + q;
+                const double *xt = xyz + 3 * ip;
+                if (sid == 0) {
+                    double sx = 0.0, sy = 0.0, sz = 0.0;
+                    for (int jp = ls->ipart; jp < ls->ipart + ls->npart; jp++) {
+                        if (jp == ip) continue;
+                        const double *xs = xyz + 3 * jp;
+                        pair_add(xs[0] - xt[0], xs[1] - xt[1], xs[2] - xt[2], prm->mass, prm->soft, prm->rs,
+                                 &sx, &sy, &sz);
+                    }
+                    a[3 * ip] += sx;
+                    a[3 * ip + 1] += sy;
+                    a[3 * ip + 2] += sz;
+                } else {
+                    for (int jp = ls->ipart; jp < ls->ipart + ls->npart; jp++) {
+                        const double *xs = xyz + 3 * jp;
+                        double bx = xs[0] + w.shift[0], by = xs[1] + w.shift[1], bz = xs[2] + w.shift[2];
+                        pair_add(bx - xt[0], by - xt[1], bz - xt[2], prm->mass, prm->soft, prm->rs, &a[3 * ip],
+                                 &a[3 * ip + 1], &a[3 * ip + 2]);
+                    }
+                }
+            }
+        }
+        for (int64_t k = 0; k < w.m2l.n; k++) {
+            int ct = w.m2l.t[k], cs = w.m2l.s[k];
+            const double *c1 = cell_center(t, ct), *c2 = cell_center(t, cs);
+            double d[3];
+            for (int i = 0; i < 3; i++) d[i] = c1[i] - (c2[i] + w.shift[i]);
+            orc_m2l(order, prm->rs, d, cell_vec(t, nt, Mleaf, Mnode, cs), cell_vec(t, nt, Lleaf, Lnode, ct));
+            mlen[ct]++;
+        }
+        list_free(&w.p2p);
+        list_free(&w.m2l);
+    }
+    walk_l2l(t, order, 0, Lleaf, Lnode);
+    /* the M2L entries above each leaf: its own and those of its ancestors */
+    int64_t *above = (int64_t *)calloc((size_t)ncell, sizeof(int64_t));
+    for (int v = 0; v < t->nnode; v++) { /* pre-order: a parent before its children */
+        int64_t mine = above[t->nleaf + v] + mlen[t->nleaf + v];
+        for (int s2 = 0; s2 < 2; s2++) {
+            int c = t->node[v].son[s2];
+            if (c == ORC_NONE) continue;
+            above[cell_id(t, c)] = mine;
+        }
+    }
+    int64_t m = 0;
+    for (int k = 0; k < nsel; k++) {
+        const oleaf *lf = &t->leaf[sel[k]];
+        orc_l2p(order, Lleaf + (size_t)nt * sel[k], lf->center, lf->npart, xyz + 3 * lf->ipart, a + 3 * lf->ipart);
+        for (int q = 0; q < lf->npart; q++) {
+            int ip = lf->ipart + q;
+            for (int d = 0; d < 3; d++) acc_sel[3 * m + d] = a[3 * ip + d];
+            id_sel[m] = t->p[ip].id;
+            m++;
+        }
+        if (p2p_len) p2p_len[k] = plen[sel[k]];
+        if (m2l_len) m2l_len[k] = above[sel[k]] + mlen[sel[k]];
+    }
+    free(above); free(plen); free(mlen); free(flag);
+    free(Mleaf); free(Lleaf); free(Mnode); free(Lnode); free(a); free(xyz);
+    orc_free(t);
+    return m;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -81,6 +81,17 @@ void orc_let_size(const orc_tree *t, int sid, int64_t *nnodes, int64_t *nbodies)
 /* acc is written (not accumulated) in the caller's particle order.        */
 int orc_force(const orc_params *prm, int n, const double *pos, double *acc, orc_stats *st);
 
+/* The same force for the particles of selected target leaves only (leaf ids of the
+ * tree; it may hold a leaf more than once only if the caller wants it twice): the
+ * whole tree and multipoles, the walks entered only for targets above a selected
+ * leaf, P2P and L2P of the selected leaves.  acc_sel/id_sel: their particles leaf
+ * after leaf in tree order (acceleration, caller index); p2p_len[k]: entries of leaf
+ * sel[k]'s P2P list; m2l_len[k]: M2L entries of the leaf and its ancestors.
+ * Returns the number of particles written (the caller sizes acc_sel for the sum of
+ * the selected leaves' counts), < 0 on bad arguments.                      */
+int64_t orc_force_leaves(const orc_params *prm, int n, const double *pos, int nsel, const int32_t *sel,
+                         double *acc_sel, int32_t *id_sel, int64_t *p2p_len, int64_t *m2l_len);
+
 /* ---- single operators (src/operator.c) --------------------------------- */
 void orc_p2m(int order, int n, const double *pos, double mass, const double *center, double *M);
 void orc_m2m(int order, const double *d, const double *M, double *tM);

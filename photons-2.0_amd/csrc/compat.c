@@ -20,6 +20,7 @@
  */
 #include <math.h>
 #include <stdarg.h>
+#include <time.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -57,7 +58,18 @@ int NPART, MAXLEAF, NLEAF, NNODE;
 int first_node, last_node, first_leaf, last_leaf;
 double MASSPART, SoftenScale, splitRadius, cutoffRadius, open_angle, BOXSIZE;
 unsigned long idxP2P, idxM2L, walk_m2l_count, p2p_count_remote;
-double dtime_prep, dtime_task, dtime_ext, dtime_fmm;
+/* phase timers of inc/photoNs.h:95-111, in seconds, written as the reference writes them:
+ *   dtime_prep   wall time of fmm_prepare (src/fmm.c:725-764)
+ *   dtime_fmm    fmm_prepare's time, plus L2L in fmm_ext (src/fmm.c:766-767, 1061)
+ *   dtime_task   wall time of fmm_task (src/fmm.c:911, 981)
+ *   dtime_ext    wall time of fmm_ext (src/fmm.c:995, 1073)
+ *   dtime_p2p    P2P (+ L2P) kernel time, reset by build_localtree (src/fmm.c:182, 871)
+ *   dtime_m2l    M2L kernel time, accumulated over calls (src/fmm.c:906)
+ *   dtime_fmm_remote  the image walks' own time: they run fused with the home walk
+ *                here, so it stays 0 (src/fmm.c:1071)
+ *   dtime_pm     wall time of partmesh (src/partmesh.c:808, 1576-1577)
+ * The kernel times are GPU times of the context's stream (HIP events). */
+double dtime_prep, dtime_task, dtime_ext, dtime_fmm, dtime_p2p, dtime_m2l, dtime_fmm_remote, dtime_pm;
 
 int photons_prepare_host(photons_ctx *ctx, int64_t n, const double *pos);
 int photons_evaluate_host(photons_ctx *ctx, double *acc_tree_order);
@@ -72,6 +84,7 @@ int photons_copy(photons_ctx *x, void *dst, const void *src, size_t bytes, int t
 
 static photons_ctx *g_ctx;
 static int g_precision = PHOTONS_FP64;
+static double g_ph_ms[PHOTONS_NPHASE]; /* phase times of the last fmm_task */
 static double *g_acc;       /* result of fmm_task, tree order */
 static ref_pack *g_leaf_mem; /* mirrors, unshifted base pointers */
 static ref_node *g_node_mem;
@@ -104,6 +117,14 @@ static void check(int rc, const char *where)
     if (rc != 0) die(where);
 }
 
+/* the reference's dtime() (src/utility.c): wall clock in seconds */
+static double wall(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 int photons_compat_set_precision(int precision)
 {
     if (precision != PHOTONS_FP64 && precision != PHOTONS_FP32_MIXED) return -1;
@@ -132,8 +153,12 @@ static photons_params current_params(void)
 static photons_ctx *ctx(void)
 {
     photons_params p = current_params();
-    if (!g_ctx) check(photons_create(&g_ctx, &p, 0, NULL), "photons_create");
-    else check(photons_set_params(g_ctx, &p), "photons_set_params");
+    if (!g_ctx) {
+        check(photons_create(&g_ctx, &p, 0, NULL), "photons_create");
+        check(photons_set_timing(g_ctx, 1), "photons_set_timing"); /* feeds the dtime_* timers */
+    } else {
+        check(photons_set_params(g_ctx, &p), "photons_set_params");
+    }
     return g_ctx;
 }
 
@@ -182,6 +207,7 @@ void fmm_construct(void)
 void build_localtree(void)
 {
     photons_ctx *x = ctx();
+    dtime_p2p = 0.0; /* src/fmm.c:182 */
     free_mirrors();
     double *pos = (double *)malloc(sizeof(double) * 3 * (size_t)(NPART > 0 ? NPART : 1));
     for (int i = 0; i < NPART; i++) memcpy(pos + 3 * (size_t)i, part[i].pos, 3 * sizeof(double));
@@ -259,16 +285,23 @@ void build_localtree(void)
 
 void fmm_prepare(void)
 {
+    const double t0 = wall();
     build_localtree();
     idxP2P = idxM2L = 0;
+    dtime_prep = wall() - t0;
+    dtime_fmm = dtime_prep;
 }
 
 void fmm_task(void)
 {
+    const double t0 = wall();
     photons_ctx *x = ctx();
     free(g_acc);
     g_acc = (double *)calloc(3 * (size_t)(NPART > 0 ? NPART : 1), sizeof(double));
     check(photons_evaluate_host(x, g_acc), "fmm_task");
+    check(photons_get_phase_ms(x, g_ph_ms), "fmm_task");
+    dtime_p2p += 1e-3 * g_ph_ms[PHOTONS_PH_P2P];
+    dtime_m2l += 1e-3 * g_ph_ms[PHOTONS_PH_M2L];
     photons_stats st;
     check(photons_get_stats(x, &st), "stats");
     idxP2P = (unsigned long)st.p2p_local_pairs;
@@ -276,15 +309,20 @@ void fmm_task(void)
     walk_m2l_count += (unsigned long)st.m2l_pairs;
     p2p_count_remote += (unsigned long)st.p2p_image_interactions;
     pull_local_expansions();
+    dtime_task = wall() - t0;
 }
 
 void fmm_ext(void)
 {
+    const double t0 = wall();
     if (!g_acc) return;
     for (int i = 0; i < NPART; i++)
         for (int d = 0; d < 3; d++) part[i].acc[d] += g_acc[3 * (size_t)i + d];
     free(g_acc);
     g_acc = NULL;
+    dtime_fmm += 1e-3 * g_ph_ms[PHOTONS_PH_L2L]; /* walk_l2l; L2P runs fused with P2P */
+    dtime_fmm_remote = 0.0;
+    dtime_ext = wall() - t0;
 }
 
 void fmm_deconstruct(void) { free_mirrors(); }
@@ -501,6 +539,7 @@ static int64_t g_pm_cap;
 /* part[].acc_pm = long-range force of part[] (overwritten, no G), on the GPU */
 void partmesh(void)
 {
+    const double t0 = wall();
     if (!g_pm || g_pm_nside != NSIDE || g_pm_box != BOXSIZE || g_pm_rs != splitRadius || g_pm_mass != MASSPART) {
         if (g_pm) photons_pm_destroy(g_pm);
         g_pm = NULL;
@@ -529,6 +568,7 @@ void partmesh(void)
     }
     for (int i = 0; i < NPART; i++)
         for (int k = 0; k < 3; k++) part[i].acc_pm[k] = acc[3 * i + k];
+    dtime_pm = wall() - t0;
 }
 
 void partmesh_thread(void) { partmesh(); }

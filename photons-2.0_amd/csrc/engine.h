@@ -34,6 +34,9 @@ __host__ __device__ inline int32_t pack_entry(int cell, int sid)
 }
 __host__ __device__ inline int entry_cell(int32_t e) { return (int)((uint32_t)e & (uint32_t)kCellMask); }
 __host__ __device__ inline int entry_shift(int32_t e) { return (int)((uint32_t)e >> kShiftBits); }
+// the entries of target t in a list: src[off[2 t]] .. src[off[2 t + 1] - 1]
+__host__ __device__ inline int64_t span_beg(const int64_t *off, int t) { return off[2 * (size_t)t]; }
+__host__ __device__ inline int64_t span_end(const int64_t *off, int t) { return off[2 * (size_t)t + 1]; }
 constexpr int kMaxDepth = 240;  // levels of the kd tree
 
 struct CellGeom {
@@ -54,11 +57,58 @@ struct DevBuf {
     size_t cap = 0;
 };
 
+// Sizes of the current tree and lists, written on the device by the kernels that
+// produce them and read there by every later kernel, so that an evaluation needs no
+// host round trip (launch grids are sized by host-side bounds; a kernel's work past
+// the device count is empty).
+constexpr int kMaxLevels = kMaxDepth + 2;
+enum : int {  // overflow bits: the evaluation exceeded the capacities it was planned for
+    kOvfNodes = 1, kOvfLeaves = 2, kOvfLevels = 4, kOvfDepths = 8, kOvfPairs = 16, kOvfP2P = 32, kOvfM2L = 64,
+    kOvfDown = 128, kOvfItems = 256, kOvfTargets = 512
+};
+struct DevSizes {
+    int nleaf, nnode, ncell, nlevel;  // cells; nlevel = depths holding nodes (0 .. nlevel-1)
+    int overflow;                     // kOvf* bits
+    int n_items;                      // M2L work items
+    int pad[2];
+    int64_t n_p2p, n_m2l;             // list lengths
+    int64_t npair;                    // walk pairs of the current depth
+    int64_t p2p_base, m2l_base;       // list entries written by the walk's earlier depths
+    int64_t dl_total;                 // down-list entries written by the walk's current depth
+    int lvl[kMaxLevels + 1];          // first temporary node of depth d
+    int ldo[kMaxLevels + 1];          // first temporary leaf of depth d
+    int64_t tgt_off[kMaxLevels + 1];  // first walk target of depth d (walk_tgt)
+};
+
 // per-box data the kernels read: box length by cell
 struct BoxView {
     const double *len;    // [nbox]
     const int *cell_box;  // [ncell]
     __device__ __forceinline__ double len_of(int cell) const { return len[cell_box[cell]]; }
+};
+
+// Capacities of a planned evaluation (photons_plan_*): every buffer is allocated for them
+// before the evaluation is queued or captured, the launches are sized by them, and the
+// loops over tree levels and walk depths run a fixed number of steps.  What does not fit
+// sets an overflow bit in DevSizes instead of being written.
+struct Plan {
+    bool active = false;
+    bool alloc = false;    // the first planned evaluation allocates the capacities it uses
+    int levels = 0;        // tree levels split (depths 0 .. levels-1)
+    int depths = 0;        // walk depths (0 .. depths-1)
+    int scap = 0;          // nodes of one level
+    int tcap = 0, lcap = 0;  // temporary nodes, leaves
+    int64_t tgt_cap = 0;   // walk targets of one depth
+    int64_t pair_cap = 0;  // walk pairs of one depth
+    int64_t dl_cap = 0;    // down-list entries of one depth
+    int64_t p2p_cap = 0, m2l_cap = 0;  // list entries
+    int item_cap = 0;      // M2L work items
+};
+
+// What an ordinary evaluation measured, for planning: the largest count of each kind
+struct Measure {
+    int levels = 0, scap = 0, ntmp = 0, nleaf = 0, depths = 0, items = 0;
+    int64_t tgt = 0, pairs = 0, dl = 0, p2p = 0, m2l = 0;
 };
 
 struct Timing {
@@ -92,10 +142,17 @@ struct Ctx {
     DevBuf geom, info;             // CellGeom / CellInfo [ncell]
     DevBuf level_nodes;            // node cell ids by depth
     DevBuf M, L;                   // expansions
+    // lists: entries grouped by target; p2p_off / m2l_off hold one span (begin, end) of
+    // entries per target leaf / cell: off[2 t] .. off[2 t + 1]
     DevBuf p2p_off, p2p_src, m2l_off, m2l_src, cnt_tmp;
-    DevBuf bfs_a, bfs_b, p2p_keys, m2l_keys;  // walk frontier and task keys
+    // walk: the cells by depth-first rank (id and subtree size; geometry), the targets
+    // grouped by depth, each node's list of sources passed down to its two children
+    // (ping-pong by depth) with its span, and per-depth work arrays
+    DevBuf walk_rec, walk_rgeom, walk_tgt, walk_tgt_off, dl_a, dl_b, dl_span;
+    DevBuf pair_off, pend_span, pair_cnt, start_list;
+    std::vector<int64_t> tgt_off;             // walk targets of depth d: walk_tgt[tgt_off[d] .. tgt_off[d+1])
+    std::vector<int> leaf_depth_off;          // tree temporaries: leaves of depth d are [off[d], off[d+1])
     DevBuf p2p_order, p2p_order_tmp;          // P2P launch order (heaviest target leaves first)
-    DevBuf bfs_act;                           // the walk rule's decision per frontier pair
     DevBuf m2l_item_off, m2l_item_tgt, m2l_part;  // balanced M2L work items
     DevBuf scan_tmp;
     // build temporaries (level order)
@@ -105,9 +162,19 @@ struct Ctx {
     DevBuf seg_chunk, chunk_part, chunk_rcnt, seg_nleft, seg_flag, seg_nn, seg_non;
     DevBuf t_size, t_lsize, t_dfs, t_lfirst, t_final, l_final;
     DevBuf err;                    // device error flag (int)
+    DevBuf sizes;                  // DevSizes of the current evaluation
     DevBuf stats;                  // device counters (int64)
     DevBuf pinned;                 // small host-pinned staging
     hipEvent_t counts_ready = nullptr;  // a count copied to `pinned` while later work runs
+
+    Plan plan;                     // active: the evaluation runs without host round trips
+    Measure meas;                  // filled by ordinary evaluations
+    hipGraph_t graph = nullptr;    // a captured planned evaluation (photons_graph_*)
+    hipGraphExec_t graph_exec = nullptr;
+    bool graph_ready = false;      // planned buffers allocated (captured or not)
+    bool graph_captured = false;
+    const double *graph_pos = nullptr;
+    double *graph_acc = nullptr;
 
     int64_t n_p2p = 0, n_m2l = 0;  // list lengths
     bool lists_valid = false;      // CSR lists built by a walk on the current tree
@@ -130,6 +197,14 @@ struct Ctx {
     } while (0)
 
 void *grow(DevBuf &b, size_t bytes);
+// a buffer of the current evaluation: grown (ordinary) or checked against the plan
+// (a planned evaluation allocates nothing)
+void *ctx_buf(Ctx &c, DevBuf &b, size_t bytes, const char *what);
+inline void *scan_storage(Ctx &c, size_t bytes) { return ctx_buf(c, c.scan_tmp, bytes, "scan storage"); }
+// launch bounds of the cells: the counts of the tree (ordinary) or the plan's capacities
+inline int nleaf_bound(const Ctx &c) { return c.plan.active ? c.plan.lcap : c.nleaf; }
+inline int nnode_bound(const Ctx &c) { return c.plan.active ? c.plan.tcap : c.nnode; }
+inline int ncell_bound(const Ctx &c) { return nleaf_bound(c) + nnode_bound(c); }
 // throws unless b holds at least `bytes` (guards every launch that reads b)
 void need(const DevBuf &b, size_t bytes, const char *what);
 void upload_positions(Ctx &c, const double *d_pos_aos);  // AoS [n][3] -> px, py, pz, pidx

@@ -10,6 +10,7 @@
 //   p2p       P2P + L2P, one wavefront per target leaf, result in caller order
 // The tree build and the walk read a few counts back to size the next step.
 #include <algorithm>
+#include <cstddef>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -46,6 +47,22 @@ void *grow(DevBuf &b, size_t bytes)
 void need(const DevBuf &b, size_t bytes, const char *what)
 {
     if (!b.p || b.cap < bytes) throw std::runtime_error(std::string(what) + " is not available for this tree");
+}
+
+void *ctx_buf(Ctx &c, DevBuf &b, size_t bytes, const char *what)
+{
+    if (c.plan.active && !c.plan.alloc) {
+        need(b, bytes == 0 ? 1 : bytes, what);
+        return b.p;
+    }
+    return grow(b, bytes);
+}
+
+// zeros in place of hipMemsetAsync (a captured evaluation records no memset)
+__global__ void k_zero_words(uint32_t *__restrict__ p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 static void release(DevBuf &b)
@@ -136,6 +153,7 @@ int photons_create(photons_ctx **out, const photons_params *prm, int device, voi
             PH_CHECK(hipHostMalloc(&x->c.pinned.p, 4096, hipHostMallocDefault));
             x->c.pinned.cap = 4096;
             grow(x->c.err, sizeof(int));
+            grow(x->c.sizes, sizeof(DevSizes));
             for (int i = 0; i < 16; i++) PH_CHECK(hipEventCreate(&x->c.timing.ev[i]));
             PH_CHECK(hipEventCreateWithFlags(&x->c.counts_ready, hipEventDisableTiming));
         } catch (...) {
@@ -156,15 +174,18 @@ int photons_destroy(photons_ctx *x)
         Ctx &c = x->c;
         PH_CHECK(hipSetDevice(c.device));
         PH_CHECK(hipStreamSynchronize(c.stream));
-        DevBuf *bufs[] = {&c.px, &c.py, &c.pz, &c.pidx, &c.qx, &c.qy, &c.qz, &c.qidx, &c.ox, &c.oy, &c.oz,
+        if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+        if (c.graph) (void)hipGraphDestroy(c.graph);
+        DevBuf *bufs[] = {&c.sizes, &c.px, &c.py, &c.pz, &c.pidx, &c.qx, &c.qy, &c.qz, &c.qidx, &c.ox, &c.oy, &c.oz,
                           &c.acc, &c.geom, &c.info, &c.level_nodes, &c.M, &c.L, &c.p2p_off, &c.p2p_src,
                           &c.m2l_off, &c.m2l_src, &c.cnt_tmp, &c.scan_tmp, &c.t_start, &c.t_count, &c.t_split,
                           &c.t_lo, &c.t_hi, &c.t_son, &c.t_depth, &c.t_parent, &c.l_start, &c.l_count, &c.l_lo,
                           &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
                           &c.seg_flag, &c.seg_nn, &c.seg_non, &c.t_size, &c.t_lsize,
-                          &c.t_dfs, &c.t_lfirst, &c.t_final, &c.l_final, &c.err, &c.stats, &c.bfs_a, &c.bfs_b,
-                          &c.p2p_keys, &c.m2l_keys, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off,
-                          &c.d_box_len, &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp, &c.bfs_act,
+                          &c.t_dfs, &c.t_lfirst, &c.t_final, &c.l_final, &c.err, &c.stats, &c.walk_rec,
+                          &c.walk_rgeom, &c.walk_tgt, &c.walk_tgt_off, &c.dl_a, &c.dl_b, &c.dl_span, &c.pair_off,
+                          &c.pend_span, &c.pair_cnt, &c.start_list, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part,
+                          &c.d_box_off, &c.d_box_len, &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp,
                           &x->io_pos, &x->io_acc, &x->op_a, &x->op_b, &x->op_c, &x->op_d, &x->op_e};
         for (DevBuf *b : bufs) release(*b);
         if (c.pinned.p) (void)hipHostFree(c.pinned.p);
@@ -222,10 +243,11 @@ static void prepare(Ctx &c, const double *d_pos)
     if (c.prm.precision == PHOTONS_FP32_MIXED) compute_offsets_f32(c);
     phase_mark(c);
     upward_pass(c);
-    // local expansions start at zero for this tree (M2L may accumulate into them)
-    const size_t lbytes = (c.prm.precision == PHOTONS_FP32_MIXED ? sizeof(float) : sizeof(double)) *
-                          (size_t)c.ncell * nterms_of(c.prm.order);
-    PH_CHECK(hipMemsetAsync(grow(c.L, lbytes), 0, lbytes, c.stream));
+    // local expansions start at zero for this tree (the drop-in walks accumulate into them)
+    const size_t lwords = (c.prm.precision == PHOTONS_FP32_MIXED ? 1 : 2) * (size_t)ncell_bound(c) *
+                          nterms_of(c.prm.order);
+    uint32_t *L = static_cast<uint32_t *>(ctx_buf(c, c.L, 4 * lwords, "local expansions"));
+    k_zero_words<<<(unsigned)((lwords + 255) / 256), 256, 0, c.stream>>>(L, lwords);
     phase_mark(c);
 }
 
@@ -242,20 +264,22 @@ static void evaluate(Ctx &c, double *d_out, bool caller_order)
     phase_mark(c);
 }
 
-static void collect_timing(Ctx &c, bool with_io)
+// Phase times from the marks, interval i giving phase first + i (marks: start, after
+// upload, after tree, after upward, after walk, after m2l, after l2l, after p2p).  The
+// two-step form (prepare, then evaluate) times its second call from a mark of its own
+// (first = PHOTONS_PH_WALK), keeping the first call's phases.
+static void collect_timing(Ctx &c, int first = PHOTONS_PH_UPLOAD)
 {
-    for (int i = 0; i < PHOTONS_NPHASE; i++) c.phase_ms[i] = 0.0;
-    if (!c.timing.enabled) return;
+    for (int i = first; i < PHOTONS_NPHASE; i++) c.phase_ms[i] = 0.0;
+    if (!c.timing.enabled || c.timing.nev == 0) return;
     PH_CHECK(hipEventSynchronize(c.timing.ev[c.timing.nev - 1]));
-    // marks: [upload] start, after upload, after tree, after upward, after walk, after m2l, after l2l, after p2p
     const int ph[] = {PHOTONS_PH_UPLOAD, PHOTONS_PH_TREE, PHOTONS_PH_UPWARD, PHOTONS_PH_WALK,
                       PHOTONS_PH_M2L, PHOTONS_PH_L2L, PHOTONS_PH_P2P};
-    for (int i = 0; i + 1 < c.timing.nev && i < 7; i++) {
+    for (int i = 0; i + 1 < c.timing.nev && first + i <= PHOTONS_PH_P2P; i++) {
         float ms = 0.f;
         PH_CHECK(hipEventElapsedTime(&ms, c.timing.ev[i], c.timing.ev[i + 1]));
-        c.phase_ms[ph[i]] = ms;
+        c.phase_ms[ph[first + i]] = ms;
     }
-    (void)with_io;
 }
 
 static std::string check_n(const Ctx &c, int64_t n)
@@ -284,6 +308,10 @@ static std::string check_boxes(const Ctx &c, int nbox, const int64_t *counts, co
 // one box of the context's own length
 static void single_box(Ctx &c, int64_t n) { set_boxes(c, 1, &n, &c.prm.boxsize); }
 
+// A context that holds a planned evaluation keeps its buffers and boxes for it: other
+// evaluations would move them under the captured graph.
+static const char *kGraphHeld = "the context holds a planned evaluation (photons_graph_destroy releases it)";
+
 }  // namespace photons
 
 extern "C" {
@@ -292,6 +320,7 @@ int photons_force_device(photons_ctx *x, int64_t n, const double *d_pos, double 
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         std::string m = check_n(c, n);
         if (!m.empty()) return fail_msg(m);
         PH_CHECK(hipSetDevice(c.device));
@@ -303,7 +332,7 @@ int photons_force_device(photons_ctx *x, int64_t n, const double *d_pos, double 
         single_box(c, n);
         prepare(c, d_pos);
         evaluate(c, d_acc, true);
-        collect_timing(c, false);
+        collect_timing(c);
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_force_device", e);
@@ -315,6 +344,7 @@ int photons_force_boxes_device(photons_ctx *x, int nbox, const int64_t *counts, 
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         std::string m = check_boxes(c, nbox, counts, lengths);
         if (!m.empty()) return fail_msg(m);
         PH_CHECK(hipSetDevice(c.device));
@@ -322,12 +352,224 @@ int photons_force_boxes_device(photons_ctx *x, int nbox, const int64_t *counts, 
         c.n = c.box_off[nbox];
         prepare(c, d_pos);
         evaluate(c, d_acc, true);
-        collect_timing(c, false);
+        collect_timing(c);
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_force_boxes_device", e);
     }
 }
+}  // extern "C"
+
+// ---------------------------------------------------------------------------- planned evaluation, hipGraph
+
+namespace photons {
+
+// capacities `margin` above what the ordinary evaluation measured (plus a fixed slack)
+static void plan_from_measure(Ctx &c, double margin)
+{
+    const Measure &m = c.meas;
+    auto up = [&](int64_t v, int64_t slack) { return (int64_t)std::ceil((double)v * (1.0 + margin)) + slack; };
+    Plan p;
+    p.levels = std::min(m.levels + 2, kMaxDepth);  // moving particles may deepen the tree
+    p.depths = p.levels + 1;                      // the deepest targets are leaves
+    p.scap = (int)up(m.scap, 64);
+    p.tcap = (int)up(m.ntmp, 1024);
+    p.lcap = (int)up(m.nleaf, 1024);
+    p.tgt_cap = up(m.tgt, 64);
+    p.pair_cap = up(m.pairs, 1024);
+    p.dl_cap = up(m.dl, 1024);
+    p.p2p_cap = up(m.p2p, 4096);
+    p.m2l_cap = up(m.m2l, 4096);
+    p.item_cap = (int)up(m.items, 1024);
+    if ((int64_t)p.tcap + p.lcap >= (1 << 27)) throw std::runtime_error("planned cells exceed 2^27");
+    c.plan = p;
+}
+
+// the planned evaluation: every launch, no host round trip, no allocation, no memset
+static void planned_evaluation(Ctx &c, const double *d_pos, double *d_acc)
+{
+    c.plan.active = true;
+    try {
+        prepare(c, d_pos);
+        evaluate(c, d_acc, true);
+    } catch (...) {
+        c.plan.active = false;
+        throw;
+    }
+    c.plan.active = false;
+}
+
+// the sizes the device wrote: the host's copies of the counts (stats, list exports)
+static int read_sizes(Ctx &c)
+{
+    DevSizes h;
+    PH_CHECK(hipMemcpyAsync(&h, c.sizes.p, offsetof(DevSizes, lvl), hipMemcpyDeviceToHost, c.stream));
+    PH_CHECK(hipStreamSynchronize(c.stream));
+    if (!h.overflow) {
+        c.nleaf = h.nleaf;
+        c.nnode = h.nnode;
+        c.ncell = h.ncell;
+        c.nlevel = c.maxdepth = h.nlevel;
+        c.n_p2p = h.n_p2p;
+        c.n_m2l = h.n_m2l;
+        c.lists_valid = true;
+    } else {
+        c.lists_valid = false;
+    }
+    return h.overflow;
+}
+
+static void release_graph(Ctx &c)
+{
+    if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
+    if (c.graph) (void)hipGraphDestroy(c.graph);
+    c.graph_exec = nullptr;
+    c.graph = nullptr;
+    c.graph_ready = c.graph_captured = false;
+    c.plan = Plan();
+}
+
+}  // namespace photons
+
+extern "C" {
+
+// Plans the evaluation of a fixed set of boxes: one ordinary evaluation of d_pos
+// measures the tree and the lists, a first planned evaluation allocates the capacities,
+// and (capture != 0) a third is captured into a hipGraph on the context's stream.
+// d_acc holds the force of d_pos afterwards.
+int photons_graph_create(photons_ctx *x, int nbox, const int64_t *counts, const double *lengths,
+                         const double *d_pos, double *d_acc, double margin, int capture)
+{
+    try {
+        Ctx &c = x->c;
+        if (!(margin >= 0.0) || margin > 16.0) return fail_msg("margin must be in [0, 16]");
+        std::string m = check_boxes(c, nbox, counts, lengths);
+        if (!m.empty()) return fail_msg(m);
+        PH_CHECK(hipSetDevice(c.device));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        release_graph(c);
+        set_boxes(c, nbox, counts, lengths);
+        c.n = c.box_off[nbox];
+        // 1. measure
+        prepare(c, d_pos);
+        evaluate(c, d_acc, true);
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        // 2. plan and allocate
+        plan_from_measure(c, margin);
+        c.plan.alloc = true;
+        try {
+            planned_evaluation(c, d_pos, d_acc);
+        } catch (...) {
+            c.plan.alloc = false;
+            throw;
+        }
+        c.plan.alloc = false;
+        if (const int ovf = read_sizes(c)) {
+            release_graph(c);
+            return fail_msg("planned evaluation overflowed its own measurement (bits " + std::to_string(ovf) + ")");
+        }
+        // 3. capture
+        if (capture) {
+            const bool timing = c.timing.enabled;
+            c.timing.enabled = false;  // no event records inside the graph
+            PH_CHECK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+            try {
+                planned_evaluation(c, d_pos, d_acc);
+            } catch (...) {
+                hipGraph_t g = nullptr;
+                (void)hipStreamEndCapture(c.stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                c.timing.enabled = timing;
+                release_graph(c);
+                throw;
+            }
+            c.timing.enabled = timing;
+            PH_CHECK(hipStreamEndCapture(c.stream, &c.graph));
+            PH_CHECK(hipGraphInstantiate(&c.graph_exec, c.graph, nullptr, nullptr, 0));
+            c.graph_captured = true;
+        }
+        c.graph_pos = d_pos;
+        c.graph_acc = d_acc;
+        c.graph_ready = true;
+        return 0;
+    } catch (const std::exception &e) {
+        release_graph(x->c);
+        return fail("photons_graph_create", e);
+    }
+}
+
+// Queues one evaluation of the planned boxes (the captured graph, or the same launches
+// one by one when nothing was captured) on the context's stream; returns at once.
+int photons_graph_launch(photons_ctx *x)
+{
+    try {
+        Ctx &c = x->c;
+        if (!c.graph_ready) return fail_msg("no planned evaluation (photons_graph_create)");
+        PH_CHECK(hipSetDevice(c.device));
+        if (c.timing.enabled) PH_CHECK(hipEventRecord(c.timing.ev[0], c.stream));
+        if (c.graph_captured) {
+            PH_CHECK(hipGraphLaunch(c.graph_exec, c.stream));
+        } else {
+            const bool timing = c.timing.enabled;
+            c.timing.enabled = false;
+            try {
+                planned_evaluation(c, c.graph_pos, c.graph_acc);
+            } catch (...) {
+                c.timing.enabled = timing;
+                throw;
+            }
+            c.timing.enabled = timing;
+        }
+        if (c.timing.enabled) PH_CHECK(hipEventRecord(c.timing.ev[1], c.stream));
+        c.timing.nev = c.timing.enabled ? 2 : 0;
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_graph_launch", e);
+    }
+}
+
+// Waits for the launched evaluations; *overflow = the kOvf bits of the last one (0: its
+// result is complete; else the boxes outgrew the plan and the graph must be created
+// again); *ms = its GPU time when timing is on (else -1).
+int photons_graph_check(photons_ctx *x, int *overflow, double *ms)
+{
+    try {
+        Ctx &c = x->c;
+        if (!c.graph_ready) return fail_msg("no planned evaluation (photons_graph_create)");
+        PH_CHECK(hipSetDevice(c.device));
+        const int ovf = read_sizes(c);
+        if (overflow) *overflow = ovf;
+        if (ms) {
+            *ms = -1.0;
+            if (c.timing.enabled && c.timing.nev == 2) {
+                float t = 0.f;
+                PH_CHECK(hipEventElapsedTime(&t, c.timing.ev[0], c.timing.ev[1]));
+                *ms = t;
+            }
+        }
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_graph_check", e);
+    }
+}
+
+int photons_graph_destroy(photons_ctx *x)
+{
+    try {
+        Ctx &c = x->c;
+        PH_CHECK(hipSetDevice(c.device));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        release_graph(c);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_graph_destroy", e);
+    }
+}
+
+}  // extern "C"
+
+extern "C" {
+
 
 int photons_set_p2p_stage(photons_ctx *x, int stage)
 {
@@ -348,6 +590,7 @@ int photons_autotune_boxes(photons_ctx *x, int nbox, const int64_t *counts, cons
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         if (reps < 1) return fail_msg("reps must be positive");
         const photons_params keep = c.prm;
         const int keep_stage = c.p2p_stage;
@@ -403,6 +646,7 @@ int photons_force_boxes_host(photons_ctx *x, int nbox, const int64_t *counts, co
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         std::string m = check_boxes(c, nbox, counts, lengths);
         if (!m.empty()) return fail_msg(m);
         PH_CHECK(hipSetDevice(c.device));
@@ -415,7 +659,7 @@ int photons_force_boxes_host(photons_ctx *x, int nbox, const int64_t *counts, co
         evaluate(c, da, true);
         PH_CHECK(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c.stream));
         PH_CHECK(hipStreamSynchronize(c.stream));
-        collect_timing(c, true);
+        collect_timing(c);
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_force_boxes_host", e);
@@ -426,6 +670,7 @@ int photons_force_host(photons_ctx *x, int64_t n, const double *pos, double *acc
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         std::string m = check_n(c, n);
         if (!m.empty()) return fail_msg(m);
         PH_CHECK(hipSetDevice(c.device));
@@ -439,7 +684,7 @@ int photons_force_host(photons_ctx *x, int64_t n, const double *pos, double *acc
         evaluate(c, da, true);
         PH_CHECK(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c.stream));
         PH_CHECK(hipStreamSynchronize(c.stream));
-        collect_timing(c, true);
+        collect_timing(c);
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_force_host", e);
@@ -452,6 +697,7 @@ int photons_prepare_host(photons_ctx *x, int64_t n, const double *pos)
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         std::string m = check_n(c, n);
         if (!m.empty()) return fail_msg(m);
         PH_CHECK(hipSetDevice(c.device));
@@ -462,6 +708,7 @@ int photons_prepare_host(photons_ctx *x, int64_t n, const double *pos)
         PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
         prepare(c, dp);
         PH_CHECK(hipStreamSynchronize(c.stream));
+        collect_timing(c);
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_prepare_host", e);
@@ -472,13 +719,17 @@ int photons_evaluate_host(photons_ctx *x, double *acc_tree_order)
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         PH_CHECK(hipSetDevice(c.device));
         if (c.n == 0) return 0;
         double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)c.n));
+        c.timing.nev = 0;  // this call's phases are timed from here
+        phase_mark(c);
         evaluate(c, da, false);
         PH_CHECK(hipMemcpyAsync(acc_tree_order, da, sizeof(double) * 3 * (size_t)c.n, hipMemcpyDeviceToHost,
                                 c.stream));
         PH_CHECK(hipStreamSynchronize(c.stream));
+        collect_timing(c, PHOTONS_PH_WALK);
         return 0;
     } catch (const std::exception &e) {
         return fail("photons_evaluate_host", e);
@@ -543,7 +794,7 @@ __global__ void k_list_stats(int nleaf, const CellInfo *info, const int64_t *off
     if (T >= nleaf) return;
     long long nT = info[T].npart;
     unsigned long long local = 0, inter = 0, image = 0;
-    for (int64_t e = off[T]; e < off[T + 1]; e++) {
+    for (int64_t e = span_beg(off, T); e < span_end(off, T); e++) {
         int s = src[e], S = entry_cell(s), sid = entry_shift(s);
         long long nS = info[S].npart;
         long long k = nT * nS;
@@ -565,7 +816,7 @@ __global__ void k_m2l_local(int ncell, const int64_t *off, const int32_t *src, u
     int X = blockIdx.x * blockDim.x + threadIdx.x;
     if (X >= ncell) return;
     unsigned long long local = 0;
-    for (int64_t e = off[X]; e < off[X + 1]; e++) local += entry_shift(src[e]) == 0;
+    for (int64_t e = span_beg(off, X); e < span_end(off, X); e++) local += entry_shift(src[e]) == 0;
     atomicAdd(&out[3], local);
 }
 
@@ -582,9 +833,9 @@ int photons_get_stats(const photons_ctx *xc, photons_stats *st)
         st->p2p_pairs = c.n_p2p;
         st->m2l_pairs = c.n_m2l;
         if (c.n == 0 || c.ncell == 0 || !c.lists_valid) return 0;
-        need(c.p2p_off, sizeof(int64_t) * (size_t)(c.nleaf + 1), "P2P list");
+        need(c.p2p_off, sizeof(int64_t) * 2 * (size_t)c.nleaf, "P2P list");
         need(c.p2p_src, sizeof(int32_t) * (size_t)c.n_p2p, "P2P list");
-        need(c.m2l_off, sizeof(int64_t) * (size_t)(c.ncell + 1), "M2L list");
+        need(c.m2l_off, sizeof(int64_t) * 2 * (size_t)c.ncell, "M2L list");
         need(c.m2l_src, sizeof(int32_t) * (size_t)c.n_m2l, "M2L list");
         unsigned long long *d = reinterpret_cast<unsigned long long *>(grow(c.stats, 4 * sizeof(unsigned long long)));
         PH_CHECK(hipMemsetAsync(d, 0, 4 * sizeof(unsigned long long), c.stream));
@@ -649,34 +900,29 @@ int photons_tree_cells(const photons_ctx *xc, int32_t *npart, int32_t *ipart, in
     }
 }
 
-// Lists are exported in the reference's order: per target by (shift id, depth-first
-// rank of the source cell), the order its depth-first walks emit them in.
+// Lists are exported target by target, each target's entries as the list holds them:
+// in the reference's order, (shift id, depth-first rank of the source cell), the
+// order its depth-first walks emit them in (the tests check it entry by entry).
 static int export_list(Ctx &c, int ntarget, DevBuf &offb, DevBuf &srcb, int64_t total, int64_t cap, int32_t *tgt,
                        int32_t *src, int8_t *shift, int64_t *count)
 {
     if (!c.lists_valid) total = 0;
     *count = total;
     if (!tgt || total == 0) return 0;
-    need(offb, sizeof(int64_t) * (size_t)(ntarget + 1), "list offsets");
+    need(offb, sizeof(int64_t) * 2 * (size_t)ntarget, "list spans");
     need(srcb, sizeof(int32_t) * (size_t)total, "list entries");
-    need(c.info, sizeof(CellInfo) * (size_t)c.ncell, "cells");
-    std::vector<int64_t> off(ntarget + 1);
+    std::vector<int64_t> off(2 * (size_t)ntarget);
     std::vector<int32_t> s(total);
-    std::vector<CellInfo> info(c.ncell);
-    PH_CHECK(hipMemcpyAsync(off.data(), offb.p, sizeof(int64_t) * (ntarget + 1), hipMemcpyDeviceToHost, c.stream));
+    PH_CHECK(hipMemcpyAsync(off.data(), offb.p, sizeof(int64_t) * 2 * ntarget, hipMemcpyDeviceToHost, c.stream));
     PH_CHECK(hipMemcpyAsync(s.data(), srcb.p, sizeof(int32_t) * total, hipMemcpyDeviceToHost, c.stream));
-    PH_CHECK(hipMemcpyAsync(info.data(), c.info.p, sizeof(CellInfo) * c.ncell, hipMemcpyDeviceToHost, c.stream));
     PH_CHECK(hipStreamSynchronize(c.stream));
-    auto rank = [&](int32_t e) { return ((int64_t)entry_shift(e) << 32) | (int64_t)info[entry_cell(e)].dfs; };
-    for (int t = 0; t < ntarget; t++) {
-        std::sort(s.begin() + off[t], s.begin() + off[t + 1],
-                  [&](int32_t x, int32_t y) { return rank(x) < rank(y); });
-        for (int64_t e = off[t]; e < off[t + 1] && e < cap; e++) {
-            tgt[e] = t;
-            src[e] = entry_cell(s[e]);
-            shift[e] = (int8_t)entry_shift(s[e]);
+    int64_t k = 0;
+    for (int t = 0; t < ntarget && k < cap; t++)
+        for (int64_t e = off[2 * (size_t)t]; e < off[2 * (size_t)t + 1] && k < cap; e++, k++) {
+            tgt[k] = t;
+            src[k] = entry_cell(s[e]);
+            shift[k] = (int8_t)entry_shift(s[e]);
         }
-    }
     return 0;
 }
 
@@ -697,6 +943,29 @@ int photons_m2l_list(const photons_ctx *xc, int64_t cap, int32_t *tgt, int32_t *
         return export_list(x->c, x->c.ncell, x->c.m2l_off, x->c.m2l_src, x->c.n_m2l, cap, tgt, src, shift, count);
     } catch (const std::exception &e) {
         return fail("photons_m2l_list", e);
+    }
+}
+
+int photons_list_lengths(const photons_ctx *xc, int which, int64_t *len)
+{
+    try {
+        photons_ctx *x = const_cast<photons_ctx *>(xc);
+        Ctx &c = x->c;
+        if (which != 0 && which != 1) return fail_msg("which must be 0 (P2P) or 1 (M2L)");
+        const int nt = which == 0 ? c.nleaf : c.ncell;
+        if (!c.lists_valid) {
+            std::fill(len, len + nt, (int64_t)0);
+            return 0;
+        }
+        DevBuf &offb = which == 0 ? c.p2p_off : c.m2l_off;
+        need(offb, sizeof(int64_t) * 2 * (size_t)nt, "list spans");
+        std::vector<int64_t> off(2 * (size_t)nt);
+        PH_CHECK(hipMemcpyAsync(off.data(), offb.p, sizeof(int64_t) * 2 * nt, hipMemcpyDeviceToHost, c.stream));
+        PH_CHECK(hipStreamSynchronize(c.stream));
+        for (int t = 0; t < nt; t++) len[t] = off[2 * (size_t)t + 1] - off[2 * (size_t)t];
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_list_lengths", e);
     }
 }
 
@@ -826,6 +1095,7 @@ int photons_walk_eval(photons_ctx *x, int start_target, int start_source, int sh
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         PH_CHECK(hipSetDevice(c.device));
         if (c.ncell == 0) return 0;
         if (start_target < 0 || start_target >= c.ncell || start_source < 0 || start_source >= c.ncell)
@@ -863,6 +1133,7 @@ int photons_p2p_pairs(photons_ctx *x, int count, const int32_t *tgt, const int32
 {
     try {
         Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
         PH_CHECK(hipSetDevice(c.device));
         if (count <= 0 || c.nleaf == 0) return 0;
         // CSR over all leaves from the (unsorted) pairs, keeping their order per target
@@ -876,9 +1147,14 @@ int photons_p2p_pairs(photons_ctx *x, int count, const int32_t *tgt, const int32
         std::vector<int32_t> ent(count);
         std::vector<int64_t> fill(off.begin(), off.end() - 1);
         for (int k = 0; k < count; k++) ent[fill[tgt[k]]++] = pack_entry(src[k], shift[k]);
-        int64_t *doff = reinterpret_cast<int64_t *>(grow(c.p2p_off, sizeof(int64_t) * (c.nleaf + 1)));
+        std::vector<int64_t> span(2 * (size_t)c.nleaf);
+        for (int t = 0; t < c.nleaf; t++) {
+            span[2 * (size_t)t] = off[t];
+            span[2 * (size_t)t + 1] = off[t + 1];
+        }
+        int64_t *doff = reinterpret_cast<int64_t *>(grow(c.p2p_off, sizeof(int64_t) * 2 * (size_t)c.nleaf));
         int32_t *dsrc = reinterpret_cast<int32_t *>(grow(c.p2p_src, sizeof(int32_t) * count));
-        PH_CHECK(hipMemcpyAsync(doff, off.data(), sizeof(int64_t) * (c.nleaf + 1), hipMemcpyHostToDevice, c.stream));
+        PH_CHECK(hipMemcpyAsync(doff, span.data(), sizeof(int64_t) * 2 * c.nleaf, hipMemcpyHostToDevice, c.stream));
         PH_CHECK(hipMemcpyAsync(dsrc, ent.data(), sizeof(int32_t) * count, hipMemcpyHostToDevice, c.stream));
         c.n_p2p = count;
         c.lists_valid = false;  // the M2L list no longer matches this P2P list

@@ -26,12 +26,13 @@ constexpr int kP2PMaxLeaf = 64;
 // ---------------------------------------------------------------------------- upward
 
 template <int P>
-__global__ void __launch_bounds__(128) k_p2m(int nleaf, const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
-                      const double *__restrict__ px, const double *__restrict__ py, const double *__restrict__ pz,
-                      double mass, double *__restrict__ M)
+__global__ void __launch_bounds__(128) k_p2m(const DevSizes *__restrict__ sz, const CellGeom *__restrict__ geom,
+                                             const CellInfo *__restrict__ info, const double *__restrict__ px,
+                                             const double *__restrict__ py, const double *__restrict__ pz, double mass,
+                                             double *__restrict__ M)
 {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nleaf) return;
+    if (sz->overflow || l >= sz->nleaf) return;
     constexpr int NT = nterms(P);
     double m[NT];
 #pragma unroll
@@ -44,12 +45,14 @@ __global__ void __launch_bounds__(128) k_p2m(int nleaf, const CellGeom *__restri
     for (int k = 0; k < NT; k++) M[(size_t)l * NT + k] = m[k];
 }
 
+// nodes of level d: level_nodes[lvl[d] .. lvl[d+1])
 template <int P>
-__global__ void __launch_bounds__(128) k_m2m_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
-                            const CellInfo *__restrict__ info, double *__restrict__ M)
+__global__ void __launch_bounds__(128) k_m2m_level(const int *__restrict__ nodes, const DevSizes *__restrict__ sz, int d,
+                                                   const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
+                                                   double *__restrict__ M)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+    int t = sz->lvl[d] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (sz->overflow || t >= sz->lvl[d + 1]) return;
     constexpr int NT = nterms(P);
     const int v = nodes[t];
     const CellInfo I = info[v];
@@ -71,10 +74,11 @@ __global__ void __launch_bounds__(128) k_m2m_level(const int *__restrict__ nodes
 }
 
 template <int P>
-__global__ void __launch_bounds__(128) k_scale_down(int ncell, const double *__restrict__ M, double inv_rs, float *__restrict__ Ms)
+__global__ void __launch_bounds__(128) k_scale_down(const DevSizes *__restrict__ sz, const double *__restrict__ M,
+                                                    double inv_rs, float *__restrict__ Ms)
 {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncell) return;
+    if (sz->overflow || c >= sz->ncell) return;
     constexpr int NT = nterms(P);
     double s[P + 1];
     s[0] = 1.0;
@@ -103,33 +107,47 @@ __device__ __forceinline__ void shift_xyz(int sid, double L, double &sx, double 
     }
 }
 
-__global__ void k_m2l_nitems(const int64_t *__restrict__ off, int ncell, int *__restrict__ nitems)
+// items per target cell, zeros up to the bound (the exclusive scan ends with the total)
+__global__ void k_m2l_nitems(const int64_t *__restrict__ off, const DevSizes *__restrict__ sz, int bound,
+                             int *__restrict__ nitems)
 {
     int X = blockIdx.x * blockDim.x + threadIdx.x;
-    if (X > ncell) return;
-    nitems[X] = X == ncell ? 0 : (int)((off[X + 1] - off[X] + kM2LChunk - 1) / kM2LChunk);
+    if (X > bound) return;
+    nitems[X] = (sz->overflow || X >= sz->ncell)
+                    ? 0
+                    : (int)((span_end(off, X) - span_beg(off, X) + kM2LChunk - 1) / kM2LChunk);
 }
 
-__global__ void k_m2l_item_targets(const int *__restrict__ item_off, int ncell, int *__restrict__ item_tgt)
+// the item count (item_off[ncell]) against the capacity of the item arrays
+__global__ void k_m2l_check(const int *__restrict__ item_off, DevSizes *sz, int item_cap)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int n = sz->overflow ? 0 : item_off[sz->ncell];
+    if (n > item_cap) sz->overflow |= kOvfItems;
+    sz->n_items = sz->overflow ? 0 : n;
+}
+
+__global__ void k_m2l_item_targets(const int *__restrict__ item_off, const DevSizes *__restrict__ sz,
+                                   int *__restrict__ item_tgt)
 {
     int X = blockIdx.x * blockDim.x + threadIdx.x;
-    if (X >= ncell) return;
+    if (sz->overflow || X >= sz->ncell) return;
     for (int j = item_off[X]; j < item_off[X + 1]; j++) item_tgt[j] = X;
 }
 
 template <int P, typename T>
-__global__ void __launch_bounds__(128) k_m2l_items(int nitems, const int *__restrict__ item_tgt,
+__global__ void __launch_bounds__(128) k_m2l_items(const DevSizes *__restrict__ sz, const int *__restrict__ item_tgt,
                                                    const int *__restrict__ item_off, const CellGeom *__restrict__ geom,
                                                    const int64_t *__restrict__ off, const int32_t *__restrict__ src,
                                                    const T *__restrict__ M, double scale, double inv2rs,
                                                    BoxView bx, T *__restrict__ part)
 {
     int it = blockIdx.x * blockDim.x + threadIdx.x;
-    if (it >= nitems) return;
+    if (it >= sz->n_items) return;  // 0 after an overflow
     constexpr int NT = nterms(P);
     const int X = item_tgt[it];
-    const int64_t e0 = off[X] + (int64_t)(it - item_off[X]) * kM2LChunk;
-    const int64_t e1 = min(off[X + 1], e0 + kM2LChunk);
+    const int64_t e0 = span_beg(off, X) + (int64_t)(it - item_off[X]) * kM2LChunk;
+    const int64_t e1 = min(span_end(off, X), e0 + kM2LChunk);
     T L[NT];
     static_for<0, NT>([&](auto k) { L[k.value] = T(0); });
     const CellGeom gX = geom[X];
@@ -158,11 +176,11 @@ __global__ void __launch_bounds__(128) k_m2l_items(int nitems, const int *__rest
 }
 
 template <int P, typename T>
-__global__ void __launch_bounds__(128) k_m2l_reduce(int ncell, const int *__restrict__ item_off,
+__global__ void __launch_bounds__(128) k_m2l_reduce(const DevSizes *__restrict__ sz, const int *__restrict__ item_off,
                                                     const T *__restrict__ part, T *__restrict__ Lout, int accumulate)
 {
     int X = blockIdx.x * blockDim.x + threadIdx.x;
-    if (X >= ncell) return;
+    if (sz->overflow || X >= sz->ncell) return;
     constexpr int NT = nterms(P);
     T L[NT];
     static_for<0, NT>([&](auto k) { L[k.value] = accumulate ? Lout[(size_t)X * NT + k.value] : T(0); });
@@ -174,11 +192,12 @@ __global__ void __launch_bounds__(128) k_m2l_reduce(int ncell, const int *__rest
 // ---------------------------------------------------------------------------- L2L
 
 template <int P, typename T>
-__global__ void __launch_bounds__(128) k_l2l_level(const int *__restrict__ nodes, int count, const CellGeom *__restrict__ geom,
-                            const CellInfo *__restrict__ info, double scale, T *__restrict__ L)
+__global__ void __launch_bounds__(128) k_l2l_level(const int *__restrict__ nodes, const DevSizes *__restrict__ sz, int d,
+                                                   const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
+                                                   double scale, T *__restrict__ L)
 {
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= count) return;
+    int t = sz->lvl[d] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (sz->overflow || t >= sz->lvl[d + 1]) return;
     constexpr int NT = nterms(P);
     const int v = nodes[t];
     const CellInfo I = info[v];
@@ -202,15 +221,15 @@ __global__ void __launch_bounds__(128) k_l2l_level(const int *__restrict__ nodes
 
 // ---------------------------------------------------------------------------- offsets
 
-__global__ void k_offsets_f32(int nleaf, const CellGeom *__restrict__ geom, const CellInfo *__restrict__ info,
-                              const double *__restrict__ px, const double *__restrict__ py,
-                              const double *__restrict__ pz, float *__restrict__ ox, float *__restrict__ oy,
-                              float *__restrict__ oz)
+__global__ void k_offsets_f32(const DevSizes *__restrict__ sz, const CellGeom *__restrict__ geom,
+                              const CellInfo *__restrict__ info, const double *__restrict__ px,
+                              const double *__restrict__ py, const double *__restrict__ pz, float *__restrict__ ox,
+                              float *__restrict__ oy, float *__restrict__ oz)
 {
     // one wavefront per leaf
     int l = blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64;
     int lane = threadIdx.x % 64;
-    if (l >= nleaf) return;
+    if (sz->overflow || l >= sz->nleaf) return;
     const CellInfo I = info[l];
     const CellGeom g = geom[l];
     for (int q = lane; q < I.npart; q += 64) {
@@ -226,7 +245,7 @@ __global__ void k_offsets_f32(int nleaf, const CellGeom *__restrict__ geom, cons
 struct P2PArgs {
     const CellGeom *geom;
     const CellInfo *info;
-    const int64_t *off;
+    const int64_t *off;  // spans of the P2P list per target leaf
     const int32_t *src;
     const double *px, *py, *pz;
     const float *ox, *oy, *oz;
@@ -234,12 +253,12 @@ struct P2PArgs {
     const void *L;  // leaf expansions (double or float)
     double mass, soft, inv2rs, inv_rs;
     BoxView bx;
-    int nleaf;
     int with_l2p;
     int accumulate;       // out += instead of out =
     int out_caller_order; // scatter through pidx into out[3*pidx]; else tree order
     double *out;          // [n][3]
-    const int32_t *order; // target leaf of each launch slot (heaviest first)
+    const int32_t *order; // target leaf of each launch slot (heaviest first; past nleaf: none)
+    const DevSizes *sz;
 };
 
 __device__ __forceinline__ void shift_vec(int sid, double L, double &sx, double &sy, double &sz)
@@ -287,28 +306,85 @@ __device__ __forceinline__ int p2p_target(int b, int nb)
 // a bin keeps neighbouring leaves, which list mostly the same sources, together in the
 // launch (and on one XCD): a full sort by work scattered them and fetched ten times the
 // bytes from HBM, bins of a factor 2 still more than twice.
-__global__ void k_leaf_work(int nleaf, const int64_t *__restrict__ off, const CellInfo *__restrict__ info,
-                            uint32_t *__restrict__ key, int32_t *__restrict__ val)
+// work bin of leaf slot T (launch slots past the leaves take the last bin)
+__device__ __forceinline__ int work_bin(int T, const DevSizes *__restrict__ sz, const int64_t *__restrict__ off,
+                                        const CellInfo *__restrict__ info)
 {
-    const int T = blockIdx.x * blockDim.x + threadIdx.x;
-    if (T >= nleaf) return;
-    const uint64_t w = (uint64_t)(off[T + 1] - off[T]) * (uint64_t)info[T].npart;
-    key[T] = (uint32_t)__clzll(w | 1) >> 2;  // factor-16 bins of work, ascending = decreasing
-    val[T] = T;
+    if (sz->overflow || T >= sz->nleaf) return 31;
+    const uint64_t w = (uint64_t)(span_end(off, T) - span_beg(off, T)) * (uint64_t)info[T].npart;
+    return (int)((uint32_t)__clzll(w | 1) >> 2);  // factor-16 bins of work, ascending = decreasing
 }
 
-static void p2p_order(Ctx &c)
+// A stable counting sort of the leaves by work bin (32 bins), tiles of kOrderTile slots:
+// per tile and bin the count (bin-major, so one exclusive scan gives every tile's first
+// slot of each bin), then a stable scatter.  Within a wave, a slot's rank among equal
+// bins comes from one ballot per bin; the waves of a tile add up in order.
+constexpr int kOrderTile = 256;
+
+__device__ __forceinline__ void tile_bins(int key, int wave, int lane, int (*wcnt)[32], int &rank)
 {
-    const int n = c.nleaf;
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    for (int b = 0; b < 32; b++) {
+        const uint64_t m = __ballot(key == b);
+        if (key == b) rank = __popcll(m & lt);
+        if (lane == 0) wcnt[wave][b] = __popcll(m);
+    }
+}
+
+__global__ void __launch_bounds__(kOrderTile) k_order_hist(int bound, const DevSizes *__restrict__ sz,
+                                                           const int64_t *__restrict__ off,
+                                                           const CellInfo *__restrict__ info, int ntile,
+                                                           int *__restrict__ hist)
+{
+    __shared__ int wcnt[kOrderTile / 64][32];
+    const int T = blockIdx.x * kOrderTile + threadIdx.x;
+    const int key = T < bound ? work_bin(T, sz, off, info) : -1;
+    int rank = 0;
+    tile_bins(key, threadIdx.x / 64, threadIdx.x % 64, wcnt, rank);
+    __syncthreads();
+    if (threadIdx.x < 32) {
+        int t = 0;
+        for (int w = 0; w < kOrderTile / 64; w++) t += wcnt[w][threadIdx.x];
+        hist[threadIdx.x * ntile + blockIdx.x] = t;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) hist[32 * ntile] = 0;  // the scan's last entry is the total
+}
+
+__global__ void __launch_bounds__(kOrderTile) k_order_scatter(int bound, const DevSizes *__restrict__ sz,
+                                                              const int64_t *__restrict__ off,
+                                                              const CellInfo *__restrict__ info, int ntile,
+                                                              const int *__restrict__ base, int32_t *__restrict__ order)
+{
+    __shared__ int wcnt[kOrderTile / 64][32];
+    const int T = blockIdx.x * kOrderTile + threadIdx.x;
+    const int key = T < bound ? work_bin(T, sz, off, info) : -1;
+    const int wave = threadIdx.x / 64;
+    int rank = 0;
+    tile_bins(key, wave, threadIdx.x % 64, wcnt, rank);
+    __syncthreads();
+    if (key < 0) return;
+    int before = 0;
+    for (int w = 0; w < wave; w++) before += wcnt[w][key];
+    order[base[key * ntile + blockIdx.x] + before + rank] = T;
+}
+
+static void p2p_order(Ctx &c, int n)
+{
     hipStream_t st = c.stream;
-    char *base = static_cast<char *>(grow(c.p2p_order, 16 * (size_t)n));
-    uint32_t *k0 = reinterpret_cast<uint32_t *>(base), *k1 = k0 + n;
-    int32_t *v0 = reinterpret_cast<int32_t *>(k1 + n), *v1 = v0 + n;
-    k_leaf_work<<<(n + 255) / 256, 256, 0, st>>>(n, dptr<int64_t>(c.p2p_off), dptr<CellInfo>(c.info), k0, v0);
+    const int ntile = (n + kOrderTile - 1) / kOrderTile;
+    const size_t nh = 32 * (size_t)ntile + 1;
+    int *hist = static_cast<int *>(ctx_buf(c, c.p2p_order_tmp, sizeof(int) * 2 * nh, "P2P order"));
+    int *base = hist + nh;
+    int32_t *order = static_cast<int32_t *>(ctx_buf(c, c.p2p_order, sizeof(int32_t) * (size_t)n, "P2P order"));
+    const DevSizes *sz = dptr<DevSizes>(c.sizes);
+    const int64_t *off = dptr<int64_t>(c.p2p_off);
+    const CellInfo *info = dptr<CellInfo>(c.info);
+    k_order_hist<<<ntile, kOrderTile, 0, st>>>(n, sz, off, info, ntile, hist);
     size_t bytes = 0;
-    PH_CHECK(rocprim::radix_sort_pairs(nullptr, bytes, k0, k1, v0, v1, (size_t)n, 0, 5, st));
-    void *tmp = grow(c.p2p_order_tmp, bytes);
-    PH_CHECK(rocprim::radix_sort_pairs(tmp, bytes, k0, k1, v0, v1, (size_t)n, 0, 5, st));
+    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, hist, base, (int)nh, st));
+    void *tmp = scan_storage(c, bytes);
+    PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, hist, base, (int)nh, st));
+    k_order_scatter<<<ntile, kOrderTile, 0, st>>>(n, sz, off, info, ntile, base, order);
 }
 
 // FP32 pair force per interaction, about 28 VALU instructions:
@@ -434,6 +510,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     __shared__ int eincl[64], ecnt[64], eip[64];
     const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
     const int lane = threadIdx.x;
+    if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
     if (n == 0) return;
@@ -447,12 +524,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     const CellGeom gT = a.geom[T];
     const double boxlen = a.bx.len_of(T);
     PairConsts pc;
-    pc.ieps = (float)(1.0 / (a.soft * lam_d));
+    // 1/eps in these units, capped so that its cube stays finite: with eps = 0 a particle
+    // with itself (r2 = 1e-30, d = 0) then adds 0 * finite = 0, as the reference's skip
+    // of that pair does; the cap only touches pairs closer than 1e-12 units
+    pc.ieps = (float)fmin(1.0 / (a.soft * lam_d), 1e12);
     pc.inv4rs = (float)(0.5 / kSqrtLog2e);                // u/2 per unit length
     pc.c2u = (float)(1.1283791670955126 / kSqrtLog2e);   // 2/sqrt(pi) u per unit length
     float axA = 0.f, ayA = 0.f, azA = 0.f, axB = 0.f, ayB = 0.f, azB = 0.f;
-    const int64_t e1 = a.off[T + 1];
-    for (int64_t base = a.off[T]; base < e1;) {
+    const int64_t e1 = span_end(a.off, T);
+    for (int64_t base = span_beg(a.off, T); base < e1;) {
         const int nb = (int)min((int64_t)64, e1 - base);
         int cnt = 0;
         if (lane < nb) {
@@ -598,7 +678,9 @@ __device__ __forceinline__ void pair_f64(double dx, double dy, double dz, double
     ri = ri * fma(-h * ri, ri, 1.5);
     ri = ri * fma(-h * ri, ri, 1.5);
     const double dr = r2 * ri;
-    double f = dr < eps ? meps3 : m * (ri * ri * ri);
+    // a particle with itself (d = 0) adds exactly nothing, also with eps = 0, where the
+    // reference skips the pair (src/fmm.c:829) and 1/r^3 would overflow
+    double f = r2 == 0.0 ? 0.0 : (dr < eps ? meps3 : m * (ri * ri * ri));
     f *= split_factor_f64(dr * inv2rs);
     ax += dx * f;
     ay += dy * f;
@@ -618,17 +700,18 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     __shared__ int eincl[64], ecnt[64], eip[64];
     const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
     const int lane = threadIdx.x;
+    if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
     if (n == 0) return;
     const LaneMap L = lane_map(n, lane);
     const double xA = a.px[IT.ipart + L.tA], yA = a.py[IT.ipart + L.tA], zA = a.pz[IT.ipart + L.tA];
     const double xB = a.px[IT.ipart + L.tB], yB = a.py[IT.ipart + L.tB], zB = a.pz[IT.ipart + L.tB];
-    const double m = a.mass, eps = a.soft, meps3 = m / (eps * eps * eps), inv2rs = a.inv2rs;
+    const double m = a.mass, eps = a.soft, meps3 = eps > 0.0 ? m / (eps * eps * eps) : 0.0, inv2rs = a.inv2rs;
     const double boxlen = a.bx.len_of(T);
     double axA = 0.0, ayA = 0.0, azA = 0.0, axB = 0.0, ayB = 0.0, azB = 0.0;
-    const int64_t e1 = a.off[T + 1];
-    for (int64_t base = a.off[T]; base < e1;) {
+    const int64_t e1 = span_end(a.off, T);
+    for (int64_t base = span_beg(a.off, T); base < e1;) {
         const int nb = (int)min((int64_t)64, e1 - base);
         int cnt = 0;
         if (lane < nb) {
@@ -734,45 +817,55 @@ int nterms_of(int order) { return nterms(order); }
 
 void compute_offsets_f32(Ctx &c)
 {
-    float *ox = reinterpret_cast<float *>(grow(c.ox, sizeof(float) * c.n));
-    float *oy = reinterpret_cast<float *>(grow(c.oy, sizeof(float) * c.n));
-    float *oz = reinterpret_cast<float *>(grow(c.oz, sizeof(float) * c.n));
-    k_offsets_f32<<<nblk(c.nleaf, 4), 256, 0, c.stream>>>(c.nleaf, dptr<CellGeom>(c.geom), dptr<CellInfo>(c.info),
-                                                          dptr<double>(c.px), dptr<double>(c.py), dptr<double>(c.pz),
-                                                          ox, oy, oz);
+    float *ox = static_cast<float *>(ctx_buf(c, c.ox, sizeof(float) * c.n, "float offsets"));
+    float *oy = static_cast<float *>(ctx_buf(c, c.oy, sizeof(float) * c.n, "float offsets"));
+    float *oz = static_cast<float *>(ctx_buf(c, c.oz, sizeof(float) * c.n, "float offsets"));
+    k_offsets_f32<<<nblk(nleaf_bound(c), 4), 256, 0, c.stream>>>(dptr<DevSizes>(c.sizes), dptr<CellGeom>(c.geom),
+                                                                  dptr<CellInfo>(c.info), dptr<double>(c.px),
+                                                                  dptr<double>(c.py), dptr<double>(c.pz), ox, oy, oz);
 }
+
+// nodes of level d (launch bound): the count (ordinary evaluation) or the plan's capacity
+static int level_bound(const Ctx &c, int d) { return c.plan.active ? c.plan.scap : c.level_off[d + 1] - c.level_off[d]; }
+static int levels_run(const Ctx &c) { return c.plan.active ? c.plan.levels : c.nlevel; }
 
 void upward_pass(Ctx &c)
 {
     const int NT = nterms(c.prm.order);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
+    const int ncb = ncell_bound(c);
     // FP64 multipoles always; the FP32-mixed path stores a scaled float copy after them
-    size_t mbytes = sizeof(double) * (size_t)c.ncell * NT;
-    double *M = reinterpret_cast<double *>(grow(c.M, mbytes + (f32 ? sizeof(float) * (size_t)c.ncell * NT : 0)));
+    size_t mbytes = sizeof(double) * (size_t)ncb * NT;
+    double *M = static_cast<double *>(
+        ctx_buf(c, c.M, mbytes + (f32 ? sizeof(float) * (size_t)ncb * NT : 0), "multipoles"));
     hipStream_t st = c.stream;
+    const DevSizes *sz = dptr<DevSizes>(c.sizes);
     const CellGeom *geom = dptr<CellGeom>(c.geom);
     const CellInfo *info = dptr<CellInfo>(c.info);
     const int *lv = dptr<int>(c.level_nodes);
-#define P2M_L(PP)                                                                                          \
-    k_p2m<PP><<<nblk(c.nleaf, 128), 128, 0, st>>>(c.nleaf, geom, info, dptr<double>(c.px), dptr<double>(c.py), \
-                                                  dptr<double>(c.pz), c.prm.mass, M);                       \
-    for (int d = c.nlevel - 1; d >= 0; d--) {                                                              \
-        int b = c.level_off[d], e = c.level_off[d + 1];                                                    \
-        k_m2m_level<PP><<<nblk(e - b, 128), 128, 0, st>>>(lv + b, e - b, geom, info, M);                  \
-    }                                                                                                      \
-    if (f32)                                                                                               \
-        k_scale_down<PP><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, M, 1.0 / c.prm.rs,                  \
-                                                             reinterpret_cast<float *>(M + (size_t)c.ncell * NT));
+    const int nlev = levels_run(c);
+#define P2M_L(PP)                                                                                                 \
+    k_p2m<PP><<<nblk(nleaf_bound(c), 128), 128, 0, st>>>(sz, geom, info, dptr<double>(c.px), dptr<double>(c.py), \
+                                                         dptr<double>(c.pz), c.prm.mass, M);                      \
+    for (int d = nlev - 1; d >= 0; d--)                                                                           \
+        k_m2m_level<PP><<<nblk(level_bound(c, d), 128), 128, 0, st>>>(lv, sz, d, geom, info, M);                  \
+    if (f32)                                                                                                      \
+        k_scale_down<PP><<<nblk(ncb, 128), 128, 0, st>>>(sz, M, 1.0 / c.prm.rs,                                   \
+                                                         reinterpret_cast<float *>(M + (size_t)ncb * NT));
     PH_DISPATCH(c.prm.order, P2M_L)
 #undef P2M_L
     PH_CHECK(hipGetLastError());
 }
 
+// the FP32 copy of the multipoles follows the FP64 ones, whose rows the buffer holds for
+// the launch bound of the cells
+static size_t m32_offset(const Ctx &c) { return (size_t)ncell_bound(c) * nterms(c.prm.order); }
+
 static void exclusive_scan_int(Ctx &c, const int *in, int *out, int n)
 {
     size_t bytes = 0;
     PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, c.stream));
-    void *tmp = grow(c.scan_tmp, bytes);
+    void *tmp = scan_storage(c, bytes);
     PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, c.stream));
 }
 
@@ -781,47 +874,57 @@ void m2l_pass(Ctx &c, bool accumulate)
     const int NT = nterms(c.prm.order);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
     const size_t tsize = f32 ? sizeof(float) : sizeof(double);
-    need(c.L, tsize * (size_t)c.ncell * NT, "local expansions");
-    need(c.M, (sizeof(double) + (f32 ? sizeof(float) : 0)) * (size_t)c.ncell * NT, "multipoles");
-    need(c.m2l_off, sizeof(int64_t) * (size_t)(c.ncell + 1), "M2L list");
-    need(c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_m2l), "M2L list");
+    const int ncb = ncell_bound(c);
+    need(c.L, tsize * (size_t)ncb * NT, "local expansions");
+    need(c.M, (sizeof(double) + (f32 ? sizeof(float) : 0)) * (size_t)ncb * NT, "multipoles");
+    need(c.m2l_off, sizeof(int64_t) * 2 * (size_t)ncb, "M2L list");
+    need(c.m2l_src, sizeof(int32_t), "M2L list");
     void *L = c.L.p;
     hipStream_t st = c.stream;
-    need(c.cell_box, sizeof(int) * (size_t)c.ncell, "cell boxes");
+    need(c.cell_box, sizeof(int) * (size_t)ncb, "cell boxes");
+    DevSizes *sz = dptr<DevSizes>(c.sizes);
     const BoxView bx = box_view(c);
     const CellGeom *geom = dptr<CellGeom>(c.geom);
     const int64_t *off = dptr<int64_t>(c.m2l_off);
     const int32_t *src = dptr<int32_t>(c.m2l_src);
     double *M = dptr<double>(c.M);
-    // work items
-    int *item_off = reinterpret_cast<int *>(grow(c.m2l_item_off, sizeof(int) * 2 * (size_t)(c.ncell + 1)));
-    int *nitems = item_off + (c.ncell + 1);
-    k_m2l_nitems<<<nblk(c.ncell + 1, 256), 256, 0, st>>>(off, c.ncell, nitems);
-    exclusive_scan_int(c, nitems, item_off, c.ncell + 1);
-    int *pin = reinterpret_cast<int *>(c.pinned.p) + 64;
-    PH_CHECK(hipMemcpyAsync(pin, item_off + c.ncell, sizeof(int), hipMemcpyDeviceToHost, st));
-    PH_CHECK(hipStreamSynchronize(st));
-    const int n_items = *pin;
-    int *item_tgt = reinterpret_cast<int *>(grow(c.m2l_item_tgt, sizeof(int) * (size_t)std::max(1, n_items)));
-    void *part = grow(c.m2l_part, tsize * (size_t)std::max(1, n_items) * NT);
-    k_m2l_item_targets<<<nblk(c.ncell, 256), 256, 0, st>>>(item_off, c.ncell, item_tgt);
+    // work items: counted per target, scanned, their number checked on the device; an
+    // ordinary evaluation reads it back to size the item arrays
+    int *item_off = static_cast<int *>(ctx_buf(c, c.m2l_item_off, sizeof(int) * 2 * (size_t)(ncb + 1), "M2L items"));
+    int *nitems = item_off + (ncb + 1);
+    k_m2l_nitems<<<nblk(ncb + 1, 256), 256, 0, st>>>(off, sz, ncb, nitems);
+    exclusive_scan_int(c, nitems, item_off, ncb + 1);
+    int item_bound;
+    if (c.plan.active) {
+        item_bound = c.plan.item_cap;
+    } else {
+        int *pin = reinterpret_cast<int *>(c.pinned.p) + 64;
+        PH_CHECK(hipMemcpyAsync(pin, item_off + ncb, sizeof(int), hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipStreamSynchronize(st));
+        item_bound = *pin;
+        c.meas.items = item_bound;
+    }
+    int *item_tgt = static_cast<int *>(ctx_buf(c, c.m2l_item_tgt, sizeof(int) * (size_t)std::max(1, item_bound),
+                                               "M2L items"));
+    void *part = ctx_buf(c, c.m2l_part, tsize * (size_t)std::max(1, item_bound) * NT, "M2L items");
+    k_m2l_check<<<1, 1, 0, st>>>(item_off, sz, item_bound);
+    k_m2l_item_targets<<<nblk(ncb, 256), 256, 0, st>>>(item_off, sz, item_tgt);
 #define M2L_L(PP)                                                                                                  \
     if (f32) {                                                                                                     \
-        if (n_items)                                                                                               \
-            k_m2l_items<PP, float><<<nblk(n_items, 128), 128, 0, st>>>(                                            \
-                n_items, item_tgt, item_off, geom, off, src, reinterpret_cast<const float *>(M + (size_t)c.ncell * NT), \
-                1.0 / c.prm.rs, 0.5, bx, reinterpret_cast<float *>(part));                                \
-        k_m2l_reduce<PP, float><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, item_off,                           \
-                                                                    reinterpret_cast<const float *>(part),         \
-                                                                    reinterpret_cast<float *>(L), accumulate);     \
+        if (item_bound)                                                                                            \
+            k_m2l_items<PP, float><<<nblk(item_bound, 128), 128, 0, st>>>(                                         \
+                sz, item_tgt, item_off, geom, off, src, reinterpret_cast<const float *>(M + m32_offset(c)),        \
+                1.0 / c.prm.rs, 0.5, bx, reinterpret_cast<float *>(part));                                        \
+        k_m2l_reduce<PP, float><<<nblk(ncb, 128), 128, 0, st>>>(sz, item_off, reinterpret_cast<const float *>(part), \
+                                                                reinterpret_cast<float *>(L), accumulate);         \
     } else {                                                                                                       \
-        if (n_items)                                                                                               \
-            k_m2l_items<PP, double><<<nblk(n_items, 128), 128, 0, st>>>(n_items, item_tgt, item_off, geom, off, src, \
-                                                                        M, 1.0, 0.5 / c.prm.rs, bx,                \
-                                                                        reinterpret_cast<double *>(part));         \
-        k_m2l_reduce<PP, double><<<nblk(c.ncell, 128), 128, 0, st>>>(c.ncell, item_off,                          \
-                                                                     reinterpret_cast<const double *>(part),       \
-                                                                     reinterpret_cast<double *>(L), accumulate);   \
+        if (item_bound)                                                                                            \
+            k_m2l_items<PP, double><<<nblk(item_bound, 128), 128, 0, st>>>(sz, item_tgt, item_off, geom, off, src,  \
+                                                                           M, 1.0, 0.5 / c.prm.rs, bx,             \
+                                                                           reinterpret_cast<double *>(part));      \
+        k_m2l_reduce<PP, double><<<nblk(ncb, 128), 128, 0, st>>>(sz, item_off,                                     \
+                                                                 reinterpret_cast<const double *>(part),           \
+                                                                 reinterpret_cast<double *>(L), accumulate);       \
     }
     PH_DISPATCH(c.prm.order, M2L_L)
 #undef M2L_L
@@ -831,20 +934,21 @@ void m2l_pass(Ctx &c, bool accumulate)
 void downward_pass(Ctx &c)
 {
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
-    need(c.L, (f32 ? sizeof(float) : sizeof(double)) * (size_t)c.ncell * nterms(c.prm.order), "local expansions");
+    need(c.L, (f32 ? sizeof(float) : sizeof(double)) * (size_t)ncell_bound(c) * nterms(c.prm.order),
+         "local expansions");
     hipStream_t st = c.stream;
+    const DevSizes *sz = dptr<DevSizes>(c.sizes);
     const CellGeom *geom = dptr<CellGeom>(c.geom);
     const CellInfo *info = dptr<CellInfo>(c.info);
     const int *lv = dptr<int>(c.level_nodes);
-#define L2L_L(PP)                                                                                              \
-    for (int d = 0; d < c.nlevel; d++) {                                                                       \
-        int b = c.level_off[d], e = c.level_off[d + 1];                                                        \
-        if (f32)                                                                                               \
-            k_l2l_level<PP, float><<<nblk(e - b, 128), 128, 0, st>>>(lv + b, e - b, geom, info, 1.0 / c.prm.rs, \
-                                                                     dptr<float>(c.L));                        \
-        else                                                                                                   \
-            k_l2l_level<PP, double><<<nblk(e - b, 128), 128, 0, st>>>(lv + b, e - b, geom, info, 1.0,          \
-                                                                      dptr<double>(c.L));                      \
+    const int nlev = levels_run(c);
+#define L2L_L(PP)                                                                                                  \
+    for (int d = 0; d < nlev; d++) {                                                                               \
+        const int nb = nblk(level_bound(c, d), 128);                                                              \
+        if (f32)                                                                                                   \
+            k_l2l_level<PP, float><<<nb, 128, 0, st>>>(lv, sz, d, geom, info, 1.0 / c.prm.rs, dptr<float>(c.L));   \
+        else                                                                                                       \
+            k_l2l_level<PP, double><<<nb, 128, 0, st>>>(lv, sz, d, geom, info, 1.0, dptr<double>(c.L));            \
     }
     PH_DISPATCH(c.prm.order, L2L_L)
 #undef L2L_L
@@ -855,13 +959,14 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
 {
     if (c.prm.maxleaf > kP2PMaxLeaf) throw std::runtime_error("maxleaf above 64 is not supported");
     const bool f32p = c.prm.precision == PHOTONS_FP32_MIXED;
-    need(c.p2p_off, sizeof(int64_t) * (size_t)(c.nleaf + 1), "P2P list");
-    need(c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(1, c.n_p2p), "P2P list");
-    need(c.geom, sizeof(CellGeom) * (size_t)c.ncell, "cells");
-    need(c.cell_box, sizeof(int) * (size_t)c.ncell, "cell boxes");
+    const int nlb = nleaf_bound(c), ncb = ncell_bound(c);
+    need(c.p2p_off, sizeof(int64_t) * 2 * (size_t)nlb, "P2P list");
+    need(c.p2p_src, sizeof(int32_t), "P2P list");
+    need(c.geom, sizeof(CellGeom) * (size_t)ncb, "cells");
+    need(c.cell_box, sizeof(int) * (size_t)ncb, "cell boxes");
     need(c.px, sizeof(double) * (size_t)c.n, "positions");
     if (f32p) need(c.ox, sizeof(float) * (size_t)c.n, "float offsets");
-    if (with_l2p) need(c.L, (f32p ? sizeof(float) : sizeof(double)) * (size_t)c.ncell * nterms(c.prm.order),
+    if (with_l2p) need(c.L, (f32p ? sizeof(float) : sizeof(double)) * (size_t)ncb * nterms(c.prm.order),
                        "local expansions");
     P2PArgs a;
     a.geom = dptr<CellGeom>(c.geom);
@@ -881,20 +986,20 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     a.inv2rs = 0.5 / c.prm.rs;
     a.inv_rs = 1.0 / c.prm.rs;
     a.bx = box_view(c);
-    a.nleaf = c.nleaf;
     a.with_l2p = with_l2p;
     a.accumulate = accumulate;
     a.out_caller_order = caller_order;
     a.out = out;
-    if (c.nleaf > 0) p2p_order(c);
-    a.order = reinterpret_cast<const int32_t *>(static_cast<char *>(c.p2p_order.p) + 12 * (size_t)c.nleaf);
+    a.sz = dptr<DevSizes>(c.sizes);
+    if (nlb > 0) p2p_order(c, nlb);
+    a.order = dptr<int32_t>(c.p2p_order);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
     hipStream_t st = c.stream;
 #define P2P_L(PP)                                              \
-    if (f32 && c.p2p_stage == 256) k_p2p_f32<PP, 256><<<c.nleaf, 64, 0, st>>>(a); \
-    else if (f32) k_p2p_f32<PP, 512><<<c.nleaf, 64, 0, st>>>(a);          \
-    else k_p2p_f64<PP><<<c.nleaf, 64, 0, st>>>(a);
-    if (c.nleaf > 0) PH_DISPATCH(c.prm.order, P2P_L)
+    if (f32 && c.p2p_stage == 256) k_p2p_f32<PP, 256><<<nlb, 64, 0, st>>>(a); \
+    else if (f32) k_p2p_f32<PP, 512><<<nlb, 64, 0, st>>>(a);          \
+    else k_p2p_f64<PP><<<nlb, 64, 0, st>>>(a);
+    if (nlb > 0) PH_DISPATCH(c.prm.order, P2P_L)
 #undef P2P_L
     PH_CHECK(hipGetLastError());
 }

@@ -279,7 +279,9 @@ int photons_pm_create(photons_pm **out, int nside, double boxsize, double rs, do
     photons_pm *p = nullptr;
     try {
         if (!out) return fail("null handle");
-        if (nside < 8 || nside > 2048 || nside % 2) return fail("nside must be even, in [8, 2048]");
+        // the CIC corner keys (k_corner_keys) are 32-bit mesh indices: N^3 < 2^32
+        if (nside < 8 || nside > 1624 || nside % 2)
+            return fail("nside must be even, in [8, 1624] (32-bit mesh indices)");
         if (!(boxsize > 0.0) || !(rs > 0.0)) return fail("boxsize and rs must be positive");
         int ndev = 0;
         if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev)

@@ -198,6 +198,8 @@ void lane_loop(photons_server *s, int lane_id)
                     s->n_done++;
                     if (r.kind == kStep) s->particles_stepped += b.n;
                 }
+                // a box's initial force request has no id a caller could wait on
+                if (r.kind == kForce) s->reqs.erase(id);
             }
             s->batches++;
             s->batch_boxes += (int64_t)ids.size();
@@ -406,14 +408,24 @@ int photons_server_submit(photons_server *s, int box_id, double dkh, double dd, 
     return 0;
 }
 
+// Each request is waited on once: the wait that reports it releases it, so a server's
+// bookkeeping does not grow with the requests it has served.
 int photons_server_wait(photons_server *s, int64_t request_id, double *latency_ms)
 {
     std::unique_lock<std::mutex> g(s->mu);
+    if (s->reqs.find(request_id) == s->reqs.end()) return fail("no such request (or already waited on)");
+    bool gone = false;
+    s->cv_done.wait(g, [&] {
+        auto it = s->reqs.find(request_id);
+        gone = it == s->reqs.end();  // another wait reported it meanwhile
+        return gone || it->second.done;
+    });
+    if (gone) return fail("request already waited on");
     auto it = s->reqs.find(request_id);
-    if (it == s->reqs.end()) return fail("no such request");
-    s->cv_done.wait(g, [&] { return it->second.done; });
-    if (latency_ms) *latency_ms = 1e3 * (it->second.t_done - it->second.t_submit);
-    if (!it->second.error.empty()) return fail(it->second.error);
+    const Request r = it->second;
+    s->reqs.erase(it);
+    if (latency_ms) *latency_ms = 1e3 * (r.t_done - r.t_submit);
+    if (!r.error.empty()) return fail(r.error);
     return 0;
 }
 

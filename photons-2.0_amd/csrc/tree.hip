@@ -5,7 +5,11 @@
 // (x at the root, cycling), particles <= mean go left; a child with at most maxleaf
 // particles is a leaf (possibly empty), boxes are cut at the split planes from
 // the domain box [0, L)^3.  When no particle lies above the mean the last one goes
-// right, as the reference partition does.
+// right, and when none lies at or below it (equal coordinates whose mean rounds low)
+// the left child is empty, as the reference's two-pointer partition leaves them.
+// Only when the same particles have gone right whole on all three axes in a row (they
+// coincide, and every mean rounds low), where the reference would split forever, the
+// first particle goes left instead.
 //
 // MI355X formulation: one level of the tree per step over all nodes of that
 // level.  Each node's particle range is cut into chunks of kChunk elements, one
@@ -47,14 +51,26 @@ __device__ __forceinline__ double wave_sum(double v)
     return v;
 }
 
-// C is an upper bound of the level's chunk count (S + ceil(n / kChunk)), so the host need
-// not read the count back; the kernels take the exact count, seg_chunk[S], from the device
+// t_depth holds a node's depth in its low 16 bits and, above them, how many splits in a
+// row have passed its particles to the right child whole (the empty-left case)
+__host__ __device__ __forceinline__ int depth_of(int d) { return d & 0xffff; }
+__host__ __device__ __forceinline__ int streak_of(int d) { return d >> 16; }
+constexpr int kStuckStreak = 3;  // all three axes: the split would repeat forever
+
+// A level's nodes (segments) are the temporary nodes sz->lvl[d] .. sz->lvl[d+1] - 1, read
+// on the device.  Scap bounds the segments of any level and C the chunks (Scap +
+// ceil(n / kChunk)); the launches are sized by them, the kernels take the exact counts
+// (S and seg_chunk[S]) from the device.
 struct LevelArgs {
     const double *coord;  // the split axis of this level
-    const int *t_start, *t_count;
-    int seg0, S, C;
-    const int *seg_chunk;  // [S+1], exclusive scan of the chunks per segment
+    const int *t_start, *t_count, *t_depth;
+    const DevSizes *sz;
+    int d;                 // the level
+    int Scap, C;
+    const int *seg_chunk;  // [Scap+1], exclusive scan of the chunks per segment
     int axis;              // 0, 1, 2: which of x, y, z coord is
+    __device__ __forceinline__ int seg0() const { return sz->lvl[d]; }
+    __device__ __forceinline__ int nseg() const { return sz->lvl[d + 1] - sz->lvl[d]; }
 };
 
 __device__ __forceinline__ double axis_value(int axis, double x, double y, double z)
@@ -66,9 +82,10 @@ __global__ void __launch_bounds__(256) k_chunk_sum(LevelArgs a, double *__restri
 {
     int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     int lane = threadIdx.x % kWave;
-    if (c >= a.seg_chunk[a.S]) return;
-    int s = find_seg(a.seg_chunk, a.S, c);
-    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
+    const int S = a.nseg(), seg0 = a.seg0();
+    if (c >= a.seg_chunk[S]) return;
+    int s = find_seg(a.seg_chunk, S, c);
+    int st = a.t_start[seg0 + s], ct = a.t_count[seg0 + s];
     int b = st + (c - a.seg_chunk[s]) * kChunk;
     int e = min(b + kChunk, st + ct);
     double v = 0.0;
@@ -87,11 +104,12 @@ __global__ void __launch_bounds__(256) k_chunk_sum(LevelArgs a, double *__restri
     if (lane == 0) partial[c] = v;
 }
 
-__global__ void k_nchunks(const int *__restrict__ cnt, int seg0, int S, int *__restrict__ out)
+// chunks per segment, zeros up to the bound (the exclusive scan then ends with the total)
+__global__ void k_nchunks(LevelArgs a, int *__restrict__ out)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > S) return;
-    out[s] = (s == S) ? 0 : (cnt[seg0 + s] + kChunk - 1) / kChunk;
+    if (s > a.Scap) return;
+    out[s] = (s >= a.nseg()) ? 0 : (a.t_count[a.seg0() + s] + kChunk - 1) / kChunk;
 }
 
 // One wavefront per segment: lane l adds the chunk partials l, l + 64, ... in order and a
@@ -103,12 +121,13 @@ __global__ void __launch_bounds__(256) k_mean(LevelArgs a, const double *__restr
 {
     const int s = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     const int lane = threadIdx.x % kWave;
-    if (s >= a.S) return;
+    const int seg0 = a.seg0();
+    if (s >= a.nseg()) return;
     double v = 0.0;
     for (int c = a.seg_chunk[s] + lane; c < a.seg_chunk[s + 1]; c += kWave) v += partial[c];
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, kWave);
-    if (lane == 0) t_split[a.seg0 + s] = v / (double)a.t_count[a.seg0 + s];
+    if (lane == 0) t_split[seg0 + s] = v / (double)a.t_count[seg0 + s];
 }
 
 __global__ void __launch_bounds__(256) k_chunk_count(LevelArgs a, const double *__restrict__ t_split,
@@ -116,14 +135,15 @@ __global__ void __launch_bounds__(256) k_chunk_count(LevelArgs a, const double *
 {
     int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     int lane = threadIdx.x % kWave;
+    const int S = a.nseg(), seg0 = a.seg0();
     if (c > a.C) return;
-    if (c >= a.seg_chunk[a.S]) {  // zeros up to the bound: the exclusive scan ends with the total
+    if (c >= a.seg_chunk[S]) {  // zeros up to the bound: the exclusive scan ends with the total
         if (lane == 0) rcnt[c] = 0;
         return;
     }
-    int s = find_seg(a.seg_chunk, a.S, c);
-    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
-    double split = t_split[a.seg0 + s];
+    int s = find_seg(a.seg_chunk, S, c);
+    int st = a.t_start[seg0 + s], ct = a.t_count[seg0 + s];
+    double split = t_split[seg0 + s];
     int b = st + (c - a.seg_chunk[s]) * kChunk;
     int e = min(b + kChunk, st + ct);
     int n = 0;
@@ -144,18 +164,22 @@ __global__ void k_seg_counts(LevelArgs a, const int *__restrict__ rpre, int maxl
                              int *__restrict__ flag, uint64_t *__restrict__ kinds)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > a.S) return;
-    if (s == a.S) {  // the scan's last entry is the total
+    const int S = a.nseg(), seg0 = a.seg0();
+    if (s > a.Scap) return;
+    if (s >= S) {  // zeros up to the bound: the scan's entry S is the total
         kinds[s] = 0;
         return;
     }
-    int ct = a.t_count[a.seg0 + s];
+    int ct = a.t_count[seg0 + s];
     int nr = rpre[a.seg_chunk[s + 1]] - rpre[a.seg_chunk[s]];
+    // nothing above the mean: the last particle goes right (f = 1: the order is kept);
+    // nothing at or below it (nr = ct): the left child is empty, everything goes right,
+    // unless that has happened on all three axes before (f = 2: the first goes left)
     int nl = ct - nr, f = 0;
-    if (nr == 0) {  // nothing above the mean: the last particle goes right
+    if (nr == 0) {
         nl = ct - 1;
         f = 1;
-    } else if (nl == 0) {  // nothing at or below it (rounding): the first goes left
+    } else if (nl == 0 && streak_of(a.t_depth[seg0 + s]) >= kStuckStreak) {
         nl = 1;
         f = 2;
     }
@@ -176,13 +200,14 @@ __global__ void __launch_bounds__(256) k_scatter(LevelArgs a, const double *__re
 {
     int c = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     int lane = threadIdx.x % kWave;
-    if (c >= a.seg_chunk[a.S]) return;
-    int s = find_seg(a.seg_chunk, a.S, c);
-    int st = a.t_start[a.seg0 + s], ct = a.t_count[a.seg0 + s];
+    const int S = a.nseg(), seg0 = a.seg0();
+    if (c >= a.seg_chunk[S]) return;
+    int s = find_seg(a.seg_chunk, S, c);
+    int st = a.t_start[seg0 + s], ct = a.t_count[seg0 + s];
     int ci = c - a.seg_chunk[s];
     int b = st + ci * kChunk;
     int e = min(b + kChunk, st + ct);
-    double split = t_split[a.seg0 + s];
+    double split = t_split[seg0 + s];
     int f = flag[s];
     int nl = nleft[s];
     int rbase = rpre[c] - rpre[a.seg_chunk[s]];
@@ -236,17 +261,23 @@ struct TreeTmp {
     int tcap, lcap;
 };
 
+// children of the level's nodes: the new nodes (depth d+1) from temporary id lvl[d+1] on,
+// the new leaves from ldo[d+1] on; ids past the capacities are not written (the level
+// step below flags the overflow and stops the tree there)
 __global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, const int *__restrict__ nleft,
-                                const uint64_t *__restrict__ kind_off, int node_base, int leaf_base)
+                                const uint64_t *__restrict__ kind_off)
 {
     int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= a.S) return;
-    int v = a.seg0 + s;
+    if (s >= a.nseg()) return;
+    int v = a.seg0() + s;
+    const int node_base = a.sz->lvl[a.d + 1], leaf_base = a.sz->ldo[a.d + 1];
     int st = T.t_start[v], ct = T.t_count[v];
     int cnt[2] = {nleft[s], ct - nleft[s]};
     int beg[2] = {st, st + nleft[s]};
     double split = T.t_split[v];
-    int depth = T.t_depth[v];
+    const int depth = depth_of(T.t_depth[v]);
+    // the right child inherits the streak of whole passes, plus this one if nothing went left
+    const int streak = nleft[s] == 0 ? streak_of(T.t_depth[v]) + 1 : 0;
     const uint64_t ko = kind_off[s];
     int nextn = node_base + (int)(ko >> 32), nextl = leaf_base + (int)(uint32_t)ko;
     for (int k = 0; k < 2; k++) {
@@ -259,9 +290,10 @@ __global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, c
         else lo[axis] = split;
         if (cnt[k] > maxleaf) {
             int id = nextn++;
+            if (id >= T.tcap) continue;
             T.t_start[id] = beg[k];
             T.t_count[id] = cnt[k];
-            T.t_depth[id] = depth + 1;
+            T.t_depth[id] = (depth + 1) | ((k == 1 ? streak : 0) << 16);
             T.t_parent[id] = v;
             for (int d = 0; d < 3; d++) {
                 T.t_lo[d * T.tcap + id] = lo[d];
@@ -270,6 +302,7 @@ __global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, c
             T.t_son[2 * v + k] = id;
         } else {
             int id = nextl++;
+            if (id >= T.lcap) continue;
             T.l_start[id] = beg[k];
             T.l_count[id] = cnt[k];
             T.l_parent[id] = v;
@@ -282,15 +315,82 @@ __global__ void k_make_children(LevelArgs a, TreeTmp T, int axis, int maxleaf, c
     }
 }
 
+// After a level: the next level's node range and the leaves of depth d+2 from the scanned
+// child kinds (kind_off[S] is the total: new nodes high, new leaves low).  A level that
+// would pass the node capacities (temporary nodes, nodes per level) or the leaf capacity
+// flags the overflow and ends the tree.
+__global__ void k_level_step(DevSizes *sz, int d, const uint64_t *__restrict__ kind_off, int tcap, int lcap,
+                             int scap)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int S = sz->lvl[d + 1] - sz->lvl[d];
+    const uint64_t tot = kind_off[S];
+    const int nn = (int)(tot >> 32), nl = (int)(uint32_t)tot;
+    int ovf = 0;
+    if ((int64_t)sz->lvl[d + 1] + nn > tcap) ovf |= kOvfNodes;
+    if (nn > scap) ovf |= kOvfNodes;
+    if ((int64_t)sz->ldo[d + 1] + nl > lcap) ovf |= kOvfLeaves;
+    if (ovf) {
+        sz->overflow |= ovf;
+        sz->lvl[d + 2] = sz->lvl[d + 1];
+        sz->ldo[d + 2] = sz->ldo[d + 1];
+        return;
+    }
+    sz->lvl[d + 2] = sz->lvl[d + 1] + nn;
+    sz->ldo[d + 2] = sz->ldo[d + 1] + nl;
+}
+
+// the roots: one per box, temporary ids 0 .. nbox-1, boxes [0, L_b]^3; the sizes start over
+__global__ void k_tree_roots(int nbox, const int64_t *__restrict__ box_off, const double *__restrict__ box_len,
+                             TreeTmp T, DevSizes *sz)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) {
+        sz->overflow = 0;
+        sz->lvl[0] = 0;
+        sz->lvl[1] = nbox;
+        sz->ldo[0] = sz->ldo[1] = 0;
+    }
+    if (b >= nbox) return;
+    T.t_start[b] = (int)box_off[b];
+    T.t_count[b] = (int)(box_off[b + 1] - box_off[b]);
+    T.t_depth[b] = 0;
+    T.t_parent[b] = -1;
+    for (int d = 0; d < 3; d++) {
+        T.t_lo[d * T.tcap + b] = 0.0;
+        T.t_hi[d * T.tcap + b] = box_len[b];
+    }
+}
+
+// after `levels` levels: the cell counts, the levels that hold nodes, and the walk targets
+// of each depth (nodes, then leaves); nodes left at depth `levels` were never split
+// (a tree deeper than planned): flagged
+__global__ void k_tree_sizes(DevSizes *sz, int levels)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (sz->lvl[levels + 1] > sz->lvl[levels]) sz->overflow |= kOvfLevels;
+    const int ntmp = sz->lvl[levels], nleaf = sz->ldo[levels + 1];
+    sz->nnode = ntmp;
+    sz->nleaf = nleaf;
+    sz->ncell = ntmp + nleaf;
+    int nlevel = 0;
+    while (nlevel < levels && sz->lvl[nlevel + 1] > sz->lvl[nlevel]) nlevel++;
+    sz->nlevel = nlevel;
+    sz->tgt_off[0] = 0;
+    for (int d = 0; d <= levels; d++)
+        sz->tgt_off[d + 1] = sz->tgt_off[d] + (sz->lvl[d + 1] - sz->lvl[d]) + (sz->ldo[d + 1] - sz->ldo[d]);
+}
+
 // The levels scatter alternately into the two particle buffers and never copy back:
 // the particles of a leaf at depth D are left in buffer D % 2.  This pass moves the
 // leaves held by the second buffer into the first, one wavefront per leaf.
-__global__ void __launch_bounds__(256) k_gather_leaves(TreeTmp T, int nleaf, Particles src, Particles dst)
+__global__ void __launch_bounds__(256) k_gather_leaves(TreeTmp T, const DevSizes *__restrict__ sz, Particles src,
+                                                       Particles dst)
 {
     int l = blockIdx.x * kWavesPerBlock + threadIdx.x / kWave;
     int lane = threadIdx.x % kWave;
     // leaf depth = parent depth + 1; an even leaf depth is already in the first buffer
-    if (l >= nleaf || (T.t_depth[T.l_parent[l]] & 1) == 1) return;
+    if (sz->overflow || l >= sz->nleaf || (depth_of(T.t_depth[T.l_parent[l]]) & 1) == 1) return;
     int b = T.l_start[l], e = b + T.l_count[l];
     for (int i = b + lane; i < e; i += kWave) {
         dst.x[i] = src.x[i];
@@ -302,36 +402,37 @@ __global__ void __launch_bounds__(256) k_gather_leaves(TreeTmp T, int nleaf, Par
 
 // ---- final numbering
 
-__global__ void k_sizes(const int *__restrict__ t_son, int b, int e, int *__restrict__ size, int *__restrict__ lsize)
+__global__ void k_sizes(const int *__restrict__ t_son, const DevSizes *__restrict__ sz, int d, int *__restrict__ size,
+                        int *__restrict__ lsize)
 {
-    int v = b + blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= e) return;
-    int sz = 1, ls = 0;
+    int v = sz->lvl[d] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (sz->overflow || v >= sz->lvl[d + 1]) return;
+    int cells = 1, ls = 0;
     for (int k = 0; k < 2; k++) {
         int c = t_son[2 * v + k];
         if (c < 0) {
-            sz += 1;
+            cells += 1;
             ls += 1;
         } else {
-            sz += size[c];
+            cells += size[c];
             ls += lsize[c];
         }
     }
-    size[v] = sz;
+    size[v] = cells;
     lsize[v] = ls;
 }
 
-__global__ void k_number(const int *__restrict__ t_son, int b, int e, const int *__restrict__ size,
-                         const int *__restrict__ lsize, int *__restrict__ dfs, int *__restrict__ lfirst,
-                         int *__restrict__ l_final, int *__restrict__ l_dfs)
+__global__ void k_number(const int *__restrict__ t_son, const DevSizes *__restrict__ sz, int d,
+                         const int *__restrict__ size, const int *__restrict__ lsize, int *__restrict__ dfs,
+                         int *__restrict__ lfirst, int *__restrict__ l_final, int *__restrict__ l_dfs)
 {
-    int v = b + blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= e) return;
-    int d = dfs[v], lf = lfirst[v];
+    int v = sz->lvl[d] + blockIdx.x * blockDim.x + threadIdx.x;
+    if (sz->overflow || v >= sz->lvl[d + 1]) return;
+    int r = dfs[v], lf = lfirst[v];
     int c0 = t_son[2 * v], c1 = t_son[2 * v + 1];
     int s0 = c0 < 0 ? 1 : size[c0];
     int ls0 = c0 < 0 ? 1 : lsize[c0];
-    int dd[2] = {d + 1, d + 1 + s0};
+    int dd[2] = {r + 1, r + 1 + s0};
     int ll[2] = {lf, lf + ls0};
     int cs[2] = {c0, c1};
     for (int k = 0; k < 2; k++) {
@@ -348,10 +449,12 @@ __global__ void k_number(const int *__restrict__ t_son, int b, int e, const int 
 
 // the roots (temporary ids 0 .. nbox-1, one per box, in box order) take consecutive
 // depth-first ranges: dfs and lfirst of root b = cells and leaves of the earlier roots
-__global__ void k_root_starts(int nbox, int nleaf, const int *__restrict__ size, const int *__restrict__ lsize,
-                              int *__restrict__ dfs, int *__restrict__ lfirst, int *__restrict__ box_root)
+__global__ void k_root_starts(int nbox, const DevSizes *__restrict__ sz, const int *__restrict__ size,
+                              const int *__restrict__ lsize, int *__restrict__ dfs, int *__restrict__ lfirst,
+                              int *__restrict__ box_root)
 {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (blockIdx.x != 0 || threadIdx.x != 0 || sz->overflow) return;
+    const int nleaf = sz->nleaf;
     int d = 0, lf = 0;
     for (int b = 0; b < nbox; b++) {
         dfs[b] = d;
@@ -374,13 +477,15 @@ __device__ __forceinline__ int box_of_slot(const int64_t *__restrict__ box_off, 
     return lo;
 }
 
-__global__ void k_emit_nodes(int ntmp, int nleaf, const int *__restrict__ dfs, const int *__restrict__ lfirst,
-                             const int *__restrict__ size, TreeTmp T, const int *__restrict__ l_final,
-                             CellGeom *__restrict__ geom, CellInfo *__restrict__ info, int *__restrict__ level_nodes,
-                             const int64_t *__restrict__ box_off, int nbox, int *__restrict__ cell_box)
+__global__ void k_emit_nodes(const DevSizes *__restrict__ sz, const int *__restrict__ dfs,
+                             const int *__restrict__ lfirst, const int *__restrict__ size, TreeTmp T,
+                             const int *__restrict__ l_final, CellGeom *__restrict__ geom, CellInfo *__restrict__ info,
+                             int *__restrict__ level_nodes, const int64_t *__restrict__ box_off, int nbox,
+                             int *__restrict__ cell_box)
 {
     int v = blockIdx.x * blockDim.x + threadIdx.x;
-    if (v >= ntmp) return;
+    const int nleaf = sz->nleaf;
+    if (sz->overflow || v >= sz->nnode) return;
     int f = nleaf + dfs[v] - lfirst[v];
     CellGeom g;
     for (int d = 0; d < 3; d++) {
@@ -403,20 +508,22 @@ __global__ void k_emit_nodes(int ntmp, int nleaf, const int *__restrict__ dfs, c
     I.dsize = size[v];
     int p = T.t_parent[v];
     I.parent = p < 0 ? -1 : nleaf + dfs[p] - lfirst[p];
-    I.depth = T.t_depth[v];
+    I.depth = depth_of(T.t_depth[v]);
     info[f] = I;
     level_nodes[v] = f;
     // nodes hold more than maxleaf >= 2 particles, so their first slot names the box
     cell_box[f] = box_of_slot(box_off, nbox, I.ipart);
 }
 
-__global__ void k_emit_leaves(int nl, int nleaf, const int *__restrict__ l_final, const int *__restrict__ l_dfs,
-                              const int *__restrict__ dfs, const int *__restrict__ lfirst, TreeTmp T,
-                              CellGeom *__restrict__ geom, CellInfo *__restrict__ info,
-                              const int64_t *__restrict__ box_off, int nbox, int *__restrict__ cell_box)
+__global__ void k_emit_leaves(const DevSizes *__restrict__ sz, const int *__restrict__ l_final,
+                              const int *__restrict__ l_dfs, const int *__restrict__ dfs,
+                              const int *__restrict__ lfirst, TreeTmp T, CellGeom *__restrict__ geom,
+                              CellInfo *__restrict__ info, const int64_t *__restrict__ box_off, int nbox,
+                              int *__restrict__ cell_box)
 {
     int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= nl) return;
+    const int nleaf = sz->nleaf;
+    if (sz->overflow || l >= nleaf) return;
     int f = l_final[l];
     CellGeom g;
     for (int d = 0; d < 3; d++) {
@@ -433,7 +540,7 @@ __global__ void k_emit_leaves(int nl, int nleaf, const int *__restrict__ l_final
     I.dsize = 1;
     int p = T.l_parent[l];
     I.parent = nleaf + dfs[p] - lfirst[p];
-    I.depth = T.t_depth[p] + 1;
+    I.depth = depth_of(T.t_depth[p]) + 1;
     info[f] = I;
     // a leaf may be empty: its parent's first slot names the box (every box holds a particle)
     cell_box[f] = box_of_slot(box_off, nbox, T.t_start[p]);
@@ -447,6 +554,29 @@ __global__ void k_init_particles(int64_t n, const double *__restrict__ pos, Part
     p.y[i] = pos[3 * i + 1];
     p.z[i] = pos[3 * i + 2];
     p.idx[i] = (int)i;
+}
+
+// Targets of the dual-tree walk grouped by depth (the walk runs one depth at a time):
+// depth d holds its nodes (level order, = depth-first order within the level) and then
+// its leaves (creation order, the same).  lv[d]: first temporary node of depth d;
+// ld[d]: first temporary leaf of depth d; toff[d]: first target of depth d.
+__global__ void k_walk_targets(const DevSizes *__restrict__ sz, const int *__restrict__ t_depth,
+                               const int *__restrict__ l_parent, const int *__restrict__ level_nodes,
+                               const int *__restrict__ l_final, int *__restrict__ tgt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sz->overflow) return;
+    const int ntmp = sz->nnode, ntl = sz->nleaf;
+    const int *lv = sz->lvl, *ld = sz->ldo;
+    const int64_t *toff = sz->tgt_off;
+    if (i < ntmp) {
+        const int d = depth_of(t_depth[i]);
+        tgt[toff[d] + (i - lv[d])] = level_nodes[i];
+    } else if (i < ntmp + ntl) {
+        const int l = i - ntmp;
+        const int d = depth_of(t_depth[l_parent[l]]) + 1;
+        tgt[toff[d] + (lv[d + 1] - lv[d]) + (l - ld[d])] = l_final[l];
+    }
 }
 
 // ---------------------------------------------------------------------------- host side
@@ -477,28 +607,55 @@ static T *regrow(DevBuf &b, size_t oldcap, size_t newcap, int width, hipStream_t
     return nw;
 }
 
-static void exclusive_scan(Ctx &c, const int *in, int *out, int n)
+// exclusive scans whose temporary storage is the context's scan buffer (grown by an
+// ordinary evaluation, sized by the plan for a planned one)
+template <typename T>
+static void exclusive_scan_t(Ctx &c, const T *in, T *out, int n)
 {
     size_t bytes = 0;
     PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, c.stream));
-    void *tmp = grow(c.scan_tmp, bytes);
+    void *tmp = scan_storage(c, bytes);
     PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, n, c.stream));
+}
+static void exclusive_scan(Ctx &c, const int *in, int *out, int n) { exclusive_scan_t<int>(c, in, out, n); }
+static void exclusive_scan_u64(Ctx &c, const uint64_t *in, uint64_t *out, int n)
+{
+    exclusive_scan_t<uint64_t>(c, in, out, n);
 }
 
 void upload_positions(Ctx &c, const double *d_pos_aos)
 {
     int64_t n = c.n;
-    Particles p{ensure<double>(c.px, n), ensure<double>(c.py, n), ensure<double>(c.pz, n), ensure<int>(c.pidx, n)};
+    Particles p{static_cast<double *>(ctx_buf(c, c.px, sizeof(double) * n, "particles")),
+                static_cast<double *>(ctx_buf(c, c.py, sizeof(double) * n, "particles")),
+                static_cast<double *>(ctx_buf(c, c.pz, sizeof(double) * n, "particles")),
+                static_cast<int *>(ctx_buf(c, c.pidx, sizeof(int) * n, "particles"))};
     if (n > 0) k_init_particles<<<blocks(n, 256), 256, 0, c.stream>>>(n, d_pos_aos, p);
 }
 
+// Tree buffers: grown to the build's needs (an ordinary evaluation), or present at the
+// plan's capacities (a planned evaluation allocates nothing: `need` only checks).
+template <typename T>
+static T *tree_buf(Ctx &c, DevBuf &b, size_t count, const char *what)
+{
+    return static_cast<T *>(ctx_buf(c, b, count * sizeof(T), what));
+}
+
+// The kd tree, level by level, all counts on the device (DevSizes).  An ordinary
+// evaluation reads each level's child counts back (one copy per level, overlapped with the
+// partition) to grow the temporaries and to stop after the last level; a planned one runs
+// plan.levels levels at the plan's capacities without reading anything.
 void build_tree(Ctx &c)
 {
+    const bool planned = c.plan.active;
     const int64_t n = c.n;
     const int maxleaf = c.prm.maxleaf;
     hipStream_t st = c.stream;
+    DevSizes *sz = dptr<DevSizes>(c.sizes);
+    need(c.sizes, sizeof(DevSizes), "sizes");
     Particles A{dptr<double>(c.px), dptr<double>(c.py), dptr<double>(c.pz), dptr<int>(c.pidx)};
-    Particles B{ensure<double>(c.qx, n), ensure<double>(c.qy, n), ensure<double>(c.qz, n), ensure<int>(c.qidx, n)};
+    Particles B{tree_buf<double>(c, c.qx, n, "particles"), tree_buf<double>(c, c.qy, n, "particles"),
+                tree_buf<double>(c, c.qz, n, "particles"), tree_buf<int>(c, c.qidx, n, "particles")};
 
     TreeTmp T;
     auto alloc_nodes = [&](int newcap, int oldcap) {
@@ -520,144 +677,166 @@ void build_tree(Ctx &c)
         T.l_hi = regrow<double>(c.l_hi, oldcap, newcap, 3, st);
         T.lcap = c.lcap = newcap;
     };
-    // level-order temporaries: keep the capacity of earlier builds, start at ~4N/maxleaf
+    // level-order temporaries: keep the capacity of earlier builds, start at ~4N/maxleaf (an
+    // ordinary evaluation) or at the plan's capacities (allocated by the first planned one)
     const int want = (int)std::max<int64_t>(std::max(1024, 2 * c.nbox + 16), 4 * n / std::max(1, maxleaf) + 16);
-    if (c.tcap < want) {
-        alloc_nodes(want, 0);
+    const int want_nodes = planned ? c.plan.tcap : want, want_leaves = planned ? c.plan.lcap : want + 1;
+    const bool may_grow = !planned || c.plan.alloc;
+    if (!may_grow && (c.tcap < want_nodes || c.lcap < want_leaves))
+        throw std::runtime_error("tree temporaries are smaller than the plan");
+    if (may_grow && c.tcap < want_nodes) {
+        alloc_nodes(want_nodes, 0);
     } else {
         T.t_start = dptr<int>(c.t_start); T.t_count = dptr<int>(c.t_count); T.t_son = dptr<int>(c.t_son);
         T.t_depth = dptr<int>(c.t_depth); T.t_parent = dptr<int>(c.t_parent); T.t_split = dptr<double>(c.t_split);
         T.t_lo = dptr<double>(c.t_lo); T.t_hi = dptr<double>(c.t_hi); T.tcap = c.tcap;
     }
-    if (c.lcap < want + 1) {
-        alloc_leaves(want + 1, 0);
+    if (may_grow && c.lcap < want_leaves) {
+        alloc_leaves(want_leaves, 0);
     } else {
         T.l_start = dptr<int>(c.l_start); T.l_count = dptr<int>(c.l_count); T.l_parent = dptr<int>(c.l_parent);
         T.l_lo = dptr<double>(c.l_lo); T.l_hi = dptr<double>(c.l_hi); T.lcap = c.lcap;
     }
 
-    // roots: one per box, temporary ids 0 .. nbox-1, boxes [0, L_b]^3
     const int nbox = c.nbox;
-    {
-        std::vector<int> start(nbox), cnt(nbox), zero(nbox, 0), parent(nbox, -1);
-        std::vector<double> lo(3 * (size_t)nbox, 0.0), hi(3 * (size_t)nbox);
-        for (int b = 0; b < nbox; b++) {
-            start[b] = (int)c.box_off[b];
-            cnt[b] = (int)(c.box_off[b + 1] - c.box_off[b]);
-            for (int d = 0; d < 3; d++) hi[d * (size_t)nbox + b] = c.box_len[b];
-        }
-        const size_t ib = sizeof(int) * nbox, db = sizeof(double) * nbox;
-        PH_CHECK(hipMemcpyAsync(T.t_start, start.data(), ib, hipMemcpyHostToDevice, st));
-        PH_CHECK(hipMemcpyAsync(T.t_count, cnt.data(), ib, hipMemcpyHostToDevice, st));
-        PH_CHECK(hipMemcpyAsync(T.t_depth, zero.data(), ib, hipMemcpyHostToDevice, st));
-        PH_CHECK(hipMemcpyAsync(T.t_parent, parent.data(), ib, hipMemcpyHostToDevice, st));
-        for (int d = 0; d < 3; d++) {
-            PH_CHECK(hipMemcpyAsync(T.t_lo + d * T.tcap, lo.data() + d * (size_t)nbox, db, hipMemcpyHostToDevice, st));
-            PH_CHECK(hipMemcpyAsync(T.t_hi + d * T.tcap, hi.data() + d * (size_t)nbox, db, hipMemcpyHostToDevice, st));
-        }
-        PH_CHECK(hipStreamSynchronize(st));
-    }
+    need(c.d_box_off, sizeof(int64_t) * (size_t)(nbox + 1), "boxes");
+    need(c.d_box_len, sizeof(double) * (size_t)nbox, "boxes");
+    k_tree_roots<<<blocks(nbox, 256), 256, 0, st>>>(nbox, dptr<int64_t>(c.d_box_off), dptr<double>(c.d_box_len), T,
+                                                     sz);
 
     int *pin = reinterpret_cast<int *>(c.pinned.p);
-    std::vector<int> lvl = {0, nbox};  // temp node ids of each level
-    int ntmp = nbox, nleaf = 0;
+    std::vector<int> lvl = {0, nbox};  // host copies of the level ranges (ordinary evaluation)
+    std::vector<int> ldo = {0, 0};
+    const int cextra = (int)((n + kChunk - 1) / kChunk);
     int depth = 0;
-    std::vector<int> seg_chunk_h;
-    while (lvl[depth + 1] > lvl[depth]) {
-        int seg0 = lvl[depth], S = lvl[depth + 1] - lvl[depth];
-        int axis = depth % 3;
-        int *seg_chunk = ensure<int>(c.seg_chunk, S + 1);
-        {
-            int *nch = ensure<int>(c.seg_nn, S + 1);
-            k_nchunks<<<blocks(S + 1, 256), 256, 0, st>>>(T.t_count, seg0, S, nch);
-            exclusive_scan(c, nch, seg_chunk, S + 1);
+    for (;; depth++) {
+        int Scap;
+        if (planned) {
+            if (depth >= c.plan.levels) break;
+            Scap = c.plan.scap;
+        } else {
+            if (lvl[depth + 1] == lvl[depth]) break;
+            if (depth >= kMaxDepth) throw std::runtime_error("tree deeper than the supported number of levels");
+            Scap = lvl[depth + 1] - lvl[depth];
         }
+        const int axis = depth % 3;
         // sum over segments of ceil(count / kChunk) <= S + ceil(n / kChunk)
-        const int C = S + (int)((c.n + kChunk - 1) / kChunk);
-        const double *coord = axis == 0 ? A.x : (axis == 1 ? A.y : A.z);
-        LevelArgs a{coord, T.t_start, T.t_count, seg0, S, C, seg_chunk, axis};
-        double *partial = ensure<double>(c.chunk_part, C + 1);
-        int *rcnt = ensure<int>(c.chunk_rcnt, 2 * (size_t)(C + 1));
+        const int C = Scap + cextra;
+        int *seg_chunk = tree_buf<int>(c, c.seg_chunk, Scap + 1, "tree level");
+        int *nch = tree_buf<int>(c, c.seg_flag, 2 * (size_t)(Scap + 1), "tree level");
+        int *flag = nch + (Scap + 1);
+        double *partial = tree_buf<double>(c, c.chunk_part, C + 1, "tree level");
+        int *rcnt = tree_buf<int>(c, c.chunk_rcnt, 2 * (size_t)(C + 1), "tree level");
         int *rpre = rcnt + (C + 1);
-        int *nleft = ensure<int>(c.seg_nleft, S + 1);
-        int *flag = ensure<int>(c.seg_flag, S + 1);
-        int wb = blocks(C, kWavesPerBlock);
+        int *nleft = tree_buf<int>(c, c.seg_nleft, Scap + 1, "tree level");
+        uint64_t *kinds = tree_buf<uint64_t>(c, c.seg_nn, Scap + 1, "tree level");
+        uint64_t *kind_off = tree_buf<uint64_t>(c, c.seg_non, Scap + 1, "tree level");
+        const double *coord = axis == 0 ? A.x : (axis == 1 ? A.y : A.z);
+        LevelArgs a{coord, T.t_start, T.t_count, T.t_depth, sz, depth, Scap, C, seg_chunk, axis};
+        k_nchunks<<<blocks(Scap + 1, 256), 256, 0, st>>>(a, nch);
+        exclusive_scan(c, nch, seg_chunk, Scap + 1);
+        const int wb = blocks(C, kWavesPerBlock);
         k_chunk_sum<<<wb, 256, 0, st>>>(a, partial);
-        k_mean<<<blocks(S, kWavesPerBlock), 256, 0, st>>>(a, partial, T.t_split);
+        k_mean<<<blocks(Scap, kWavesPerBlock), 256, 0, st>>>(a, partial, T.t_split);
         k_chunk_count<<<blocks(C + 1, kWavesPerBlock), 256, 0, st>>>(a, T.t_split, rcnt);
         exclusive_scan(c, rcnt, rpre, C + 1);
-        uint64_t *kinds = ensure<uint64_t>(c.seg_nn, S + 1);
-        uint64_t *kind_off = ensure<uint64_t>(c.seg_non, S + 1);
-        k_seg_counts<<<blocks(S + 1, 256), 256, 0, st>>>(a, rpre, maxleaf, nleft, flag, kinds);
-        {
-            size_t bytes = 0;
-            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, kinds, kind_off, S + 1, st));
-            void *tmp = grow(c.scan_tmp, bytes);
-            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, kinds, kind_off, S + 1, st));
+        k_seg_counts<<<blocks(Scap + 1, 256), 256, 0, st>>>(a, rpre, maxleaf, nleft, flag, kinds);
+        exclusive_scan_u64(c, kinds, kind_off, Scap + 1);
+        if (!planned) {
+            PH_CHECK(hipMemcpyAsync(pin, kind_off + Scap, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
+            PH_CHECK(hipEventRecord(c.counts_ready, st));
         }
-        PH_CHECK(hipMemcpyAsync(pin, kind_off + S, sizeof(uint64_t), hipMemcpyDeviceToHost, st));
-        PH_CHECK(hipEventRecord(c.counts_ready, st));
         // the partition runs while the host reads the counts
         k_scatter<<<wb, 256, 0, st>>>(a, T.t_split, rpre, nleft, flag, A, B);
         std::swap(A, B);  // the children's particles are in B now
-        PH_CHECK(hipEventSynchronize(c.counts_ready));
-        const uint64_t tot = *reinterpret_cast<const uint64_t *>(pin);
-        int new_nodes = (int)(tot >> 32), new_leaves = (int)(uint32_t)tot;
-        if (ntmp + new_nodes > T.tcap) {
-            int ncap = std::max(2 * T.tcap, ntmp + new_nodes + 1024);
-            alloc_nodes(ncap, T.tcap);
-            a.t_start = T.t_start;
-            a.t_count = T.t_count;
+        if (!planned) {
+            PH_CHECK(hipEventSynchronize(c.counts_ready));
+            const uint64_t tot = *reinterpret_cast<const uint64_t *>(pin);
+            const int new_nodes = (int)(tot >> 32), new_leaves = (int)(uint32_t)tot;
+            const int ntmp = lvl[depth + 1], nleaf = ldo[depth + 1];
+            if (ntmp + new_nodes > T.tcap) {
+                alloc_nodes(std::max(2 * T.tcap, ntmp + new_nodes + 1024), T.tcap);
+                a.t_start = T.t_start;
+                a.t_count = T.t_count;
+                a.t_depth = T.t_depth;
+            }
+            if (nleaf + new_leaves > T.lcap) alloc_leaves(std::max(2 * T.lcap, nleaf + new_leaves + 1024), T.lcap);
+            lvl.push_back(ntmp + new_nodes);
+            ldo.push_back(nleaf + new_leaves);
         }
-        if (nleaf + new_leaves > T.lcap) alloc_leaves(std::max(2 * T.lcap, nleaf + new_leaves + 1024), T.lcap);
-        k_make_children<<<blocks(S, 256), 256, 0, st>>>(a, T, axis, maxleaf, nleft, kind_off, ntmp, nleaf);
-        ntmp += new_nodes;
-        nleaf += new_leaves;
-        lvl.push_back(ntmp);
-        depth++;
-        if (depth > kMaxDepth) throw std::runtime_error("tree deeper than the supported number of levels");
+        k_make_children<<<blocks(Scap, 256), 256, 0, st>>>(a, T, axis, maxleaf, nleft, kind_off);
+        // the planned capacities (the arrays may be larger, from earlier evaluations)
+        k_level_step<<<1, 1, 0, st>>>(sz, depth, kind_off, planned ? c.plan.tcap : T.tcap,
+                                      planned ? c.plan.lcap : T.lcap, planned ? c.plan.scap : INT32_MAX);
     }
-    const int nlevels = depth;  // levels holding nodes: 0 .. depth-1
+    const int levels = depth;  // levels split: nodes of depths 0 .. levels-1
+    k_tree_sizes<<<1, 1, 0, st>>>(sz, levels);
+    // launch bounds: the counts themselves (ordinary) or the plan's capacities
+    const int ntmp_b = planned ? c.plan.tcap : lvl[levels];
+    const int nleaf_b = planned ? c.plan.lcap : ldo[levels + 1];
+    auto level_bound = [&](int d) { return planned ? c.plan.scap : lvl[d + 1] - lvl[d]; };
+
     // leaves at odd depth sit in the second buffer
     {
         Particles P{dptr<double>(c.px), dptr<double>(c.py), dptr<double>(c.pz), dptr<int>(c.pidx)};
         Particles Q{dptr<double>(c.qx), dptr<double>(c.qy), dptr<double>(c.qz), dptr<int>(c.qidx)};
-        if (nleaf > 0) k_gather_leaves<<<blocks(nleaf, kWavesPerBlock), 256, 0, st>>>(T, nleaf, Q, P);
+        if (nleaf_b > 0) k_gather_leaves<<<blocks(nleaf_b, kWavesPerBlock), 256, 0, st>>>(T, sz, Q, P);
     }
 
     // subtree sizes, bottom-up
-    int *size = ensure<int>(c.t_size, ntmp);
-    int *lsize = ensure<int>(c.t_lsize, ntmp);
-    int *dfs = ensure<int>(c.t_dfs, ntmp);
-    int *lfirst = ensure<int>(c.t_lfirst, ntmp);
-    int *l_final = ensure<int>(c.l_final, 2 * (size_t)nleaf);
-    int *l_dfs = l_final + nleaf;
-    for (int d = nlevels - 1; d >= 0; d--) {
-        int b = lvl[d], e = lvl[d + 1];
-        k_sizes<<<blocks(e - b, 256), 256, 0, st>>>(T.t_son, b, e, size, lsize);
-    }
-    int *box_root = ensure<int>(c.d_box_root, nbox);
-    k_root_starts<<<1, 1, 0, st>>>(nbox, nleaf, size, lsize, dfs, lfirst, box_root);
-    for (int d = 0; d < nlevels; d++) {
-        int b = lvl[d], e = lvl[d + 1];
-        k_number<<<blocks(e - b, 256), 256, 0, st>>>(T.t_son, b, e, size, lsize, dfs, lfirst, l_final, l_dfs);
-    }
-    c.nleaf = nleaf;
-    c.nnode = ntmp;
-    c.ncell = nleaf + ntmp;
-    c.nlevel = nlevels;
-    c.maxdepth = nlevels;  // leaves sit at most at depth nlevels
-    CellGeom *geom = ensure<CellGeom>(c.geom, c.ncell);
-    CellInfo *info = ensure<CellInfo>(c.info, c.ncell);
-    int *level_nodes = ensure<int>(c.level_nodes, ntmp);
-    int *cell_box = ensure<int>(c.cell_box, c.ncell);
+    int *size = tree_buf<int>(c, c.t_size, T.tcap, "tree numbering");
+    int *lsize = tree_buf<int>(c, c.t_lsize, T.tcap, "tree numbering");
+    int *dfs = tree_buf<int>(c, c.t_dfs, T.tcap, "tree numbering");
+    int *lfirst = tree_buf<int>(c, c.t_lfirst, T.tcap, "tree numbering");
+    int *l_final = tree_buf<int>(c, c.l_final, 2 * (size_t)T.lcap, "tree numbering");
+    int *l_dfs = l_final + T.lcap;
+    for (int d = levels - 1; d >= 0; d--)
+        k_sizes<<<blocks(level_bound(d), 256), 256, 0, st>>>(T.t_son, sz, d, size, lsize);
+    int *box_root = tree_buf<int>(c, c.d_box_root, nbox, "box roots");
+    k_root_starts<<<1, 1, 0, st>>>(nbox, sz, size, lsize, dfs, lfirst, box_root);
+    for (int d = 0; d < levels; d++)
+        k_number<<<blocks(level_bound(d), 256), 256, 0, st>>>(T.t_son, sz, d, size, lsize, dfs, lfirst, l_final,
+                                                              l_dfs);
+    const int ncell_b = ntmp_b + nleaf_b;
+    CellGeom *geom = tree_buf<CellGeom>(c, c.geom, ncell_b, "cells");
+    CellInfo *info = tree_buf<CellInfo>(c, c.info, ncell_b, "cells");
+    int *level_nodes = tree_buf<int>(c, c.level_nodes, std::max(1, ntmp_b), "cells");
+    int *cell_box = tree_buf<int>(c, c.cell_box, ncell_b, "cells");
     const int64_t *box_off = dptr<int64_t>(c.d_box_off);
-    k_emit_nodes<<<blocks(ntmp, 256), 256, 0, st>>>(ntmp, nleaf, dfs, lfirst, size, T, l_final, geom, info,
-                                                     level_nodes, box_off, nbox, cell_box);
-    k_emit_leaves<<<blocks(nleaf, 256), 256, 0, st>>>(nleaf, nleaf, l_final, l_dfs, dfs, lfirst, T, geom, info,
-                                                      box_off, nbox, cell_box);
-    c.level_off.assign(lvl.begin(), lvl.end());
+    k_emit_nodes<<<blocks(ntmp_b, 256), 256, 0, st>>>(sz, dfs, lfirst, size, T, l_final, geom, info, level_nodes,
+                                                       box_off, nbox, cell_box);
+    k_emit_leaves<<<blocks(nleaf_b, 256), 256, 0, st>>>(sz, l_final, l_dfs, dfs, lfirst, T, geom, info, box_off, nbox,
+                                                        cell_box);
+    // walk targets by depth: depths 0 .. levels (the deepest holds leaves only)
+    int *tgt = tree_buf<int>(c, c.walk_tgt, ncell_b, "walk targets");
+    k_walk_targets<<<blocks(ncell_b, 256), 256, 0, st>>>(sz, T.t_depth, T.l_parent, level_nodes, l_final, tgt);
     PH_CHECK(hipGetLastError());
+
+    if (!planned) {
+        // host copies: cell counts, level ranges, walk targets per depth
+        const int nleaf = ldo[levels + 1], ntmp = lvl[levels];
+        c.nleaf = nleaf;
+        c.nnode = ntmp;
+        c.ncell = nleaf + ntmp;
+        int nlevel = 0;
+        while (nlevel < levels && lvl[nlevel + 1] > lvl[nlevel]) nlevel++;
+        c.nlevel = nlevel;
+        c.maxdepth = nlevel;  // leaves sit at most at depth nlevel
+        c.level_off.assign(lvl.begin(), lvl.end());
+        c.leaf_depth_off.assign(ldo.begin(), ldo.end());
+        const int nd = levels + 1;
+        c.tgt_off.assign(nd + 1, 0);
+        for (int d = 0; d < nd; d++)
+            c.tgt_off[d + 1] = c.tgt_off[d] + (lvl[d + 1] - lvl[d]) + (ldo[d + 1] - ldo[d]);
+        Measure &m = c.meas;
+        m.levels = levels;
+        m.ntmp = ntmp;
+        m.nleaf = nleaf;
+        m.scap = 0;
+        m.tgt = 0;
+        for (int d = 0; d < levels; d++) m.scap = std::max(m.scap, lvl[d + 1] - lvl[d]);
+        for (int d = 0; d < nd; d++) m.tgt = std::max<int64_t>(m.tgt, c.tgt_off[d + 1] - c.tgt_off[d]);
+    }
 }
 
 }  // namespace photons

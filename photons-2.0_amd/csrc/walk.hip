@@ -1,4 +1,4 @@
-// walk.hip -- device dual-tree walk emitting flat interaction lists.
+// walk.hip -- device dual-tree walk emitting interaction lists grouped by target.
 //
 // The reference walks (target, source) cell pairs depth-first from (root, root)
 // and emits P2P and M2L tasks (src/fmm.c:406-712 for the home box,
@@ -7,21 +7,22 @@
 // only on the rule applied to each pair, so any traversal that applies the same
 // rules emits the same set.
 //
-// MI355X formulation: a breadth-first walk over the whole pair space.  Each level
-// is two launches with one thread per pair (count, then write at scanned offsets):
-// the thread applies the rule and appends its children (1, 2 or 4 pairs) or its
-// emitted task.  No stack, no atomics, no per-thread divergence beyond one rule.
-// The emitted tasks (target, packed source entry) are then grouped by target with a
-// stable radix sort on the target id alone, so every list is deterministic.  Within
-// one target the entries keep the walk's emission order; the list exports restore
-// the reference's depth-first order, which is (shift, source rank in depth-first
-// order): the reference finishes a source subtree before its sibling's starts.
+// MI355X formulation: one step per depth of the target tree.  A pair (A, B) whose
+// rule opens the target A passes the source B on to both children of A, and a node
+// paired with itself passes its two children; so each node keeps one list of sources
+// for its children (its down list), and the pairs whose target is a cell X at depth d
+// are X against the down list of X's parent.  At depth d every such pair (one thread
+// each) walks the source subtree of its entry in depth-first order without a stack:
+// at source rank r the rule either descends (opens the source: r + 1) or settles the
+// whole subtree (P2P, M2L, pass down or nothing: r + subtree size).  Two passes per
+// depth (count, then write at scanned offsets) and no atomics.  The pairs of one target
+// are contiguous and its sources come in (shift, depth-first rank) order, so every
+// list is written grouped by target in the reference's order: no sort.
 //
 // Image walks: a shifted node keeps no children when its box is rc or more from
 // the domain box or when 0.95*theta*distance exceeds its largest width
 // (src/remotes.c:145-158); the rule is evaluated on the fly, no copy is built.
 #include <hipcub/hipcub.hpp>
-#include <rocprim/rocprim.hpp>
 
 #include <algorithm>
 #include <stdexcept>
@@ -30,33 +31,6 @@
 #include "fmm_math.h"
 
 namespace photons {
-
-// pair: A in bits 0-26, B in bits 27-53, shift id in bits 54-58
-__device__ __forceinline__ uint64_t pair_of(uint32_t A, uint32_t B, uint32_t sid)
-{
-    return (uint64_t)A | ((uint64_t)B << 27) | ((uint64_t)sid << 54);
-}
-
-// task: target cell in bits 32-63, packed source entry (cell, shift id) in bits 0-31
-__device__ __forceinline__ uint64_t task_of(uint32_t target, int cell, int sid)
-{
-    return ((uint64_t)target << 32) | (uint64_t)(uint32_t)pack_entry(cell, sid);
-}
-
-struct BfsArgs {
-    const CellGeom *geom;
-    const CellInfo *info;
-    int nleaf;
-    double theta, theta2, rc, rc2;
-    BoxView bx;  // box lengths: shifts and the domain box [0, L]^3 of the image prune
-    const uint64_t *in;
-    int64_t n_in;
-    uint64_t *out;
-    uint32_t *p2p_t, *m2l_t;  // task targets
-    int32_t *p2p_s, *m2l_s;   // task source entries
-    int64_t p2p_base, m2l_base;  // tasks emitted by earlier levels
-    uint8_t *act;                // [n_in] the rule's decision per pair, from the count pass
-};
 
 __host__ __device__ inline void shift_of(int sid, double L, double &sx, double &sy, double &sz)
 {
@@ -91,264 +65,340 @@ __device__ __forceinline__ bool let_pruned(const CellGeom &root, const CellGeom 
     return wmax < 0.95 * theta * dr;
 }
 
-// Outputs of the rule for one pair, in four fixed slots (static indices keep them in
-// registers): slot c holds a frontier pair when bit c of `kid` is set, a P2P task
-// (target, entry) when bit c of `p2p` is set; `m2l` holds an M2L task when has_m2l.
-struct RuleOut {
-    uint64_t slot[4];
-    unsigned kid, p2p;
-    uint64_t m2l;
-    int has_m2l;
-};
-
-// A child pair of two leaves is always a P2P task (the rule below), so it is emitted
-// as one straight away instead of passing through the next level's frontier.
-__device__ __forceinline__ void put_child(const BfsArgs &a, RuleOut &o, int c, int A, int B, int sid)
-{
-    if (A < a.nleaf && B < a.nleaf) {
-        o.slot[c] = task_of(A, B, sid);
-        o.p2p |= 1u << c;
-    } else {
-        o.slot[c] = pair_of(A, B, sid);
-        o.kid |= 1u << c;
-    }
-}
-
-// The rule applied to one pair (src/fmm.c:406-712, src/remotes.c:213-552), in two parts:
-// bfs_decide reads the geometry and returns what the pair does; bfs_emit turns that
-// into its children (up to 4, leaf-leaf children as P2P tasks), its P2P task or its M2L
-// task, reading only child ids.  The count pass stores the decision (one byte per pair),
-// so the write pass does not load the geometry or evaluate the acceptance again.
-enum : uint8_t {
+// what the rule does with one pair
+enum : int {
     kActNone = 0,  // pruned
-    kActSelfLeaf,  // a leaf with itself: P2P
-    kActSelfNode,  // a node with itself: its 4 child pairs
-    kActLeafLeaf,  // two leaves: P2P, whatever the acceptance says
+    kActP2P,       // a leaf with itself, or two leaves: P2P, whatever the acceptance says
+    kActSelfNode,  // a node with itself: its children pass down
     kActM2L,       // accepted (or a target leaf against a pruned image node): M2L
-    kActOpenA,     // open the target
+    kActOpenA,     // open the target: the source passes down
     kActOpenB      // open the source
 };
 
-__device__ __forceinline__ int bfs_decide(const BfsArgs &a, uint64_t pr)
+struct WalkArgs {
+    const CellInfo *info;
+    const int2 *rec;          // by depth-first rank: (cell, subtree size)
+    const CellGeom *rgeom;    // by depth-first rank
+    DevSizes *sz;             // cell counts, the targets of each depth, list bases, overflow
+    double theta, theta2, rc, rc2;
+    BoxView bx;
+    // this depth: its targets are walk_tgt[tgt_off[d] .. tgt_off[d+1])
+    const int *walk_tgt;
+    int d;
+    int64_t tgt_cap;          // launch bound of the targets
+    const int64_t *pair_off;  // [ntgt + 1]: first pair of each target (scan of the pending counts)
+    const int64_t *pspan;     // [2 ntgt]: each target's pending entries in `pend`
+    int64_t npair_cap;        // pairs the per-pair arrays and the launch hold
+    const int32_t *pend;      // pending entries: the previous depth's down lists (or the start list)
+    uint32_t *pcnt;           // [3 npair]: outputs of each pair (P2P, M2L, down), count pass -> write pass
+    // outputs of the write pass
+    int32_t *p2p_out, *m2l_out, *dl_out;  // list bases: sz->p2p_base, sz->m2l_base
+    int64_t *p2p_span, *m2l_span, *dl_span;  // per leaf / cell / node (index cell - nleaf)
+};
+
+// the target of a pair and what the rule needs of it
+struct Target {
+    int X;
+    bool leaf;
+    int son0, son1;
+    CellGeom g;
+    double L;         // its box length (shifts, the image prune's domain box)
+};
+
+__device__ __forceinline__ int decide(const WalkArgs &a, const Target &t, int v, int r, int sid, double sx,
+                                      double sy, double sz)
 {
-    const int A = (int)(pr & 0x7ffffff), B = (int)((pr >> 27) & 0x7ffffff), sid = (int)(pr >> 54);
-    const int nleaf = a.nleaf;
-    const bool aleaf = A < nleaf, bleaf = B < nleaf;
-    if (sid == 0 && A == B) return aleaf ? kActSelfLeaf : kActSelfNode;
-    if (aleaf && bleaf) return kActLeafLeaf;  // leaf-leaf pairs are always P2P
-    double sx = 0.0, sy = 0.0, sz = 0.0, L = 0.0;
-    if (sid != 0) {
-        L = a.bx.len_of(A);
-        shift_of(sid, L, sx, sy, sz);
-    }
-    const CellGeom gA = a.geom[A], gB = a.geom[B];
-    const double d0 = gA.c[0] - (gB.c[0] + sx);
-    const double d1 = gA.c[1] - (gB.c[1] + sy);
-    const double d2 = gA.c[2] - (gB.c[2] + sz);
-    const int flag = mac(gA.w, gB.w, d0, d1, d2, a.theta2, a.rc2);
+    const bool bleaf = v < a.sz->nleaf;
+    if (sid == 0 && v == t.X) return t.leaf ? kActP2P : kActSelfNode;
+    if (t.leaf && bleaf) return kActP2P;  // leaf-leaf pairs are always P2P
+    const CellGeom gB = a.rgeom[r];
+    const double d0 = t.g.c[0] - (gB.c[0] + sx);
+    const double d1 = t.g.c[1] - (gB.c[1] + sy);
+    const double d2 = t.g.c[2] - (gB.c[2] + sz);
+    const int flag = mac(t.g.w, gB.w, d0, d1, d2, a.theta2, a.rc2);
     if (flag < 0) return kActNone;
     bool pruned = false;
     if (sid != 0 && !bleaf) {
+        const double L = t.L;
         const CellGeom box = {{0.5 * L, 0.5 * L, 0.5 * L}, {L, L, L}};
         pruned = let_pruned(box, gB, sx, sy, sz, a.rc, a.theta);
     }
     // accepted; in the image walk a target leaf against a pruned node takes M2L whatever
     // the acceptance says (src/remotes.c:442)
-    if (flag == 1 || (aleaf && pruned)) return kActM2L;
-    if (aleaf) return kActOpenB;
+    if (flag == 1 || (t.leaf && pruned)) return kActM2L;
+    if (t.leaf) return kActOpenB;
     if (bleaf) return kActOpenA;
-    const bool bigger = (gA.w[0] + gA.w[1] + gA.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]);
+    const bool bigger = (t.g.w[0] + t.g.w[1] + t.g.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]);
     return (bigger || pruned) ? kActOpenA : kActOpenB;  // ties open the source
 }
 
-__device__ __forceinline__ void bfs_emit(const BfsArgs &a, uint64_t pr, int act, RuleOut &o)
+// The pairs of target t with the source subtree of `entry`, in depth-first order.
+// WRITE = false counts the outputs; WRITE = true stores them from the given positions,
+// never more than the count pass found (lp, lm, ld), so a store stays inside the ranges
+// the scan gave this pair.
+template <bool WRITE>
+__device__ __forceinline__ void traverse(const WalkArgs &a, const Target &t, int32_t entry, uint32_t &np,
+                                         uint32_t &nm, uint32_t &nd, int64_t qp, int64_t qm, int64_t qd,
+                                         uint32_t lp = 0, uint32_t lm = 0, uint32_t ld = 0)
 {
-    o.kid = o.p2p = 0;
-    o.has_m2l = 0;
-    const int A = (int)(pr & 0x7ffffff), B = (int)((pr >> 27) & 0x7ffffff), sid = (int)(pr >> 54);
-    if (act == kActSelfLeaf || act == kActLeafLeaf) {
-        o.slot[0] = task_of(A, B, sid);
-        o.p2p = 1;
-    } else if (act == kActSelfNode) {
-        const CellInfo IA = a.info[A];
-        put_child(a, o, 0, IA.son0, IA.son0, 0);
-        put_child(a, o, 1, IA.son0, IA.son1, 0);
-        put_child(a, o, 2, IA.son1, IA.son0, 0);
-        put_child(a, o, 3, IA.son1, IA.son1, 0);
-    } else if (act == kActM2L) {
-        o.m2l = task_of(A, B, sid);
-        o.has_m2l = 1;
-    } else if (act == kActOpenA) {
-        const CellInfo IA = a.info[A];
-        put_child(a, o, 0, IA.son0, B, sid);
-        put_child(a, o, 1, IA.son1, B, sid);
-    } else if (act == kActOpenB) {
-        const CellInfo IB = a.info[B];
-        put_child(a, o, 0, A, IB.son0, sid);
-        put_child(a, o, 1, A, IB.son1, sid);
+    const int B = entry_cell(entry), sid = entry_shift(entry);
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (sid != 0) shift_of(sid, t.L, sx, sy, sz);
+    const CellInfo IB = a.info[B];
+    int r = IB.dfs;
+    const int rend = IB.dfs + IB.dsize;
+    while (r < rend) {
+        const int2 cd = a.rec[r];
+        const int v = cd.x;
+        const int act = decide(a, t, v, r, sid, sx, sy, sz);
+        if (act == kActP2P) {
+            if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(v, sid);
+            np++;
+        } else if (act == kActM2L) {
+            if (WRITE && nm < lm) a.m2l_out[qm + nm] = pack_entry(v, sid);
+            nm++;
+        } else if (act == kActOpenA) {
+            if (WRITE && nd < ld) a.dl_out[qd + nd] = pack_entry(v, sid);
+            nd++;
+        } else if (act == kActSelfNode) {
+            if (WRITE && nd + 1 < ld) {
+                a.dl_out[qd + nd] = pack_entry(t.son0, 0);
+                a.dl_out[qd + nd + 1] = pack_entry(t.son1, 0);
+            }
+            nd += 2;
+        }
+        r += act == kActOpenB ? 1 : cd.y;
     }
 }
 
-// Each level runs in two passes without atomics: k_bfs_count sums the outputs of
-// every tile of kBfsTile pairs, one device scan turns those sums into tile offsets (and
-// the exact output sizes, which the host reads), and k_bfs_write re-applies the rule and
-// writes each output at its scanned position.
-// Within a tile the outputs keep the input order (item k * kBfsThreads + thread), so
-// neighbouring threads store to neighbouring addresses.
-constexpr int kBfsThreads = 256;
-constexpr int kBfsPerThread = 4;
-constexpr int kBfsTile = kBfsThreads * kBfsPerThread;
+constexpr int kWalkThreads = 256;  // pairs per tile
 
-// output counts of one tile (child pairs, P2P tasks, M2L tasks), scanned together
+// the target of pair q: the last j with pair_off[j] <= q
+__device__ __forceinline__ int target_of(const int64_t *__restrict__ pair_off, int ntgt, int64_t q)
+{
+    int lo = 0, hi = ntgt;  // pair_off[lo] <= q < pair_off[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (pair_off[mid] <= q) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// the depth's targets, at most the launch bound (a depth with more flags the overflow in
+// k_walk_prep, and the pair arrays hold only the bound's targets)
+__device__ __forceinline__ int ntgt_all(const WalkArgs &a) { return (int)(a.sz->tgt_off[a.d + 1] - a.sz->tgt_off[a.d]); }
+__device__ __forceinline__ int ntgt_of(const WalkArgs &a) { return (int)min((int64_t)ntgt_all(a), a.tgt_cap); }
+__device__ __forceinline__ const int *tgt_of(const WalkArgs &a) { return a.walk_tgt + a.sz->tgt_off[a.d]; }
+
+__device__ __forceinline__ Target load_target(const WalkArgs &a, int j)
+{
+    Target t;
+    t.X = tgt_of(a)[j];
+    t.leaf = t.X < a.sz->nleaf;
+    const CellInfo I = a.info[t.X];
+    t.son0 = I.son0;
+    t.son1 = I.son1;
+    t.g = a.rgeom[I.dfs];
+    t.L = a.bx.len_of(t.X);
+    return t;
+}
+
+// output counts of one tile of pairs (P2P, M2L, down entries), scanned together
 struct Cnt3 {
-    int64_t k, p, m;
+    int64_t p, m, d;
 };
 struct AddCnt3 {
-    __host__ __device__ Cnt3 operator()(const Cnt3 &x, const Cnt3 &y) const { return {x.k + y.k, x.p + y.p, x.m + y.m}; }
+    __host__ __device__ Cnt3 operator()(const Cnt3 &x, const Cnt3 &y) const { return {x.p + y.p, x.m + y.m, x.d + y.d}; }
 };
 
-__global__ void __launch_bounds__(kBfsThreads) k_bfs_count(BfsArgs a, Cnt3 *__restrict__ tile_cnt, int ntile)
+__global__ void __launch_bounds__(kWalkThreads) k_walk_count(WalkArgs a, Cnt3 *__restrict__ tile_cnt, int ntile)
 {
-    typedef hipcub::BlockReduce<int, kBfsThreads> Reduce;
+    typedef hipcub::BlockReduce<int64_t, kWalkThreads> Reduce;
     __shared__ typename Reduce::TempStorage tmp;
-    int ck = 0, cp = 0, cm = 0;
-    // all items' pairs and decisions first, the byte stores last: a byte store may alias
-    // anything, and between the loads it would keep them from overlapping
-    uint64_t pr[kBfsPerThread];
-    int act[kBfsPerThread];
-#pragma unroll
-    for (int k = 0; k < kBfsPerThread; k++) {
-        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
-        pr[k] = i < a.n_in ? a.in[i] : 0;
+    const int ntgt = ntgt_of(a);
+    const int64_t npair_all = a.pair_off[ntgt];
+    const int64_t npair = min(npair_all, a.npair_cap);
+    if (blockIdx.x == 0 && threadIdx.x == 0 && npair_all > a.npair_cap) a.sz->overflow |= kOvfPairs;
+    const int64_t q = (int64_t)blockIdx.x * kWalkThreads + threadIdx.x;
+    uint32_t np = 0, nm = 0, nd = 0;
+    if (q < npair) {
+        const int j = target_of(a.pair_off, ntgt, q);
+        const Target t = load_target(a, j);
+        const int32_t entry = a.pend[a.pspan[2 * j] + (q - a.pair_off[j])];
+        traverse<false>(a, t, entry, np, nm, nd, 0, 0, 0);
+        a.pcnt[3 * q] = np;
+        a.pcnt[3 * q + 1] = nm;
+        a.pcnt[3 * q + 2] = nd;
     }
-#pragma unroll
-    for (int k = 0; k < kBfsPerThread; k++) {
-        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
-        act[k] = i < a.n_in ? bfs_decide(a, pr[k]) : kActNone;
-    }
-#pragma unroll
-    for (int k = 0; k < kBfsPerThread; k++) {
-        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
-        if (i < a.n_in) {
-            RuleOut o;
-            bfs_emit(a, pr[k], act[k], o);
-            ck += __popc(o.kid);
-            cp += __popc(o.p2p);
-            cm += o.has_m2l;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < kBfsPerThread; k++) {
-        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
-        if (i < a.n_in) a.act[i] = (uint8_t)act[k];
-    }
-    int tk = Reduce(tmp).Sum(ck);
+    const int64_t tp = Reduce(tmp).Sum((int64_t)np);
     __syncthreads();
-    int tp = Reduce(tmp).Sum(cp);
+    const int64_t tm = Reduce(tmp).Sum((int64_t)nm);
     __syncthreads();
-    int tm = Reduce(tmp).Sum(cm);
+    const int64_t td = Reduce(tmp).Sum((int64_t)nd);
     if (threadIdx.x == 0) {
-        tile_cnt[blockIdx.x] = Cnt3{tk, tp, tm};
+        tile_cnt[blockIdx.x] = Cnt3{tp, tm, td};
         if (blockIdx.x == 0) tile_cnt[ntile] = Cnt3{0, 0, 0};  // the scan's last entry is the total
     }
 }
 
-// The counts of a thread's kBfsPerThread items sit in 16-bit fields of one 64-bit
-// word (a field's tile sum is at most 4 * kBfsThreads), so one block scan gives the
-// offset of every item within the tile's output.
-__device__ __forceinline__ int field(uint64_t v, int k) { return (int)((v >> (16 * k)) & 0xffff); }
-
-__global__ void __launch_bounds__(kBfsThreads) k_bfs_write(BfsArgs a, const Cnt3 *__restrict__ tile_off)
+__global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const Cnt3 *__restrict__ tile_off)
 {
-    static_assert(kBfsPerThread <= 4 && 4 * kBfsThreads < 65536, "16-bit count fields");
-    typedef hipcub::BlockScan<uint64_t, kBfsThreads> Scan;
+    typedef hipcub::BlockScan<int64_t, kWalkThreads> Scan;
     __shared__ typename Scan::TempStorage tmp;
-    RuleOut o[kBfsPerThread];
-    uint64_t ck = 0, cp = 0, cm = 0;
-#pragma unroll
-    for (int k = 0; k < kBfsPerThread; k++) {
-        const int64_t i = (int64_t)blockIdx.x * kBfsTile + k * kBfsThreads + threadIdx.x;
-        o[k].kid = o[k].p2p = 0;
-        o[k].has_m2l = 0;
-        if (i < a.n_in) bfs_emit(a, a.in[i], a.act[i], o[k]);
-        ck |= (uint64_t)__popc(o[k].kid) << (16 * k);
-        cp |= (uint64_t)__popc(o[k].p2p) << (16 * k);
-        cm |= (uint64_t)o[k].has_m2l << (16 * k);
+    if (a.sz->overflow) return;  // the same for every thread: no block scan is left half-entered
+    const int ntgt = ntgt_of(a);
+    const int64_t npair = min(a.pair_off[ntgt], a.npair_cap);
+    const int64_t q = (int64_t)blockIdx.x * kWalkThreads + threadIdx.x;
+    const bool live = q < npair;
+    int64_t np = 0, nm = 0, nd = 0;
+    if (live) {
+        np = a.pcnt[3 * q];
+        nm = a.pcnt[3 * q + 1];
+        nd = a.pcnt[3 * q + 2];
     }
-    uint64_t ok, op, om, tk, tp, tm;
-    Scan(tmp).ExclusiveSum(ck, ok, tk);
+    int64_t op, om, od;
+    Scan(tmp).ExclusiveSum(np, op);
     __syncthreads();
-    Scan(tmp).ExclusiveSum(cp, op, tp);
+    Scan(tmp).ExclusiveSum(nm, om);
     __syncthreads();
-    Scan(tmp).ExclusiveSum(cm, om, tm);
+    Scan(tmp).ExclusiveSum(nd, od);
+    if (!live) return;
     const Cnt3 to = tile_off[blockIdx.x];
-    const int64_t bk = to.k;
-    const int64_t bp = a.p2p_base + to.p;
-    const int64_t bm = a.m2l_base + to.m;
-    int sk = 0, sp = 0, sm = 0;  // outputs of the tile's earlier items
-#pragma unroll
-    for (int k = 0; k < kBfsPerThread; k++) {
-        const int64_t qk = bk + sk + field(ok, k), qp = bp + sp + field(op, k);
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const unsigned below = (1u << c) - 1u;
-            if ((o[k].kid >> c) & 1u) a.out[qk + __popc(o[k].kid & below)] = o[k].slot[c];
-            if ((o[k].p2p >> c) & 1u) {
-                const int64_t q = qp + __popc(o[k].p2p & below);
-                a.p2p_t[q] = (uint32_t)(o[k].slot[c] >> 32);
-                a.p2p_s[q] = (int32_t)(uint32_t)o[k].slot[c];
-            }
-        }
-        if (o[k].has_m2l) {
-            const int64_t q = bm + sm + field(om, k);
-            a.m2l_t[q] = (uint32_t)(o[k].m2l >> 32);
-            a.m2l_s[q] = (int32_t)(uint32_t)o[k].m2l;
-        }
-        sk += field(tk, k);
-        sp += field(tp, k);
-        sm += field(tm, k);
+    const int64_t qp = a.sz->p2p_base + to.p + op, qm = a.sz->m2l_base + to.m + om, qd = to.d + od;
+    const int j = target_of(a.pair_off, ntgt, q);
+    const Target t = load_target(a, j);
+    const int32_t entry = a.pend[a.pspan[2 * j] + (q - a.pair_off[j])];
+    uint32_t cp = 0, cm = 0, cd = 0;
+    traverse<true>(a, t, entry, cp, cm, cd, qp, qm, qd, (uint32_t)np, (uint32_t)nm, (uint32_t)nd);
+    // the target's first pair opens its spans, its last pair closes them
+    const bool first = q == a.pair_off[j], last = q + 1 == a.pair_off[j + 1];
+    const int nleaf = a.sz->nleaf;
+    if (first) {
+        if (t.leaf) a.p2p_span[2 * (size_t)t.X] = qp;
+        else a.dl_span[2 * (size_t)(t.X - nleaf)] = qd;
+        a.m2l_span[2 * (size_t)t.X] = qm;
+    }
+    if (last) {
+        if (t.leaf) a.p2p_span[2 * (size_t)t.X + 1] = qp + np;
+        else a.dl_span[2 * (size_t)(t.X - nleaf) + 1] = qd + nd;
+        a.m2l_span[2 * (size_t)t.X + 1] = qm + nm;
     }
 }
 
-// first frontier: (root, root, shift) of every box for the shifts [lo, hi), or one
-// given (A, B) pair per shift when box_root is null
-__global__ void k_init_frontier(uint64_t *f, const int *__restrict__ box_root, int nbox, int A, int B, int shift_lo,
-                                int shift_hi)
+// Pending entries of each target of a depth: its parent's down list, or at the start
+// depth the start list (start_target < 0: target j is the root of box j and takes
+// entries j * nstart .. ; else start_target alone takes all nstart entries).  Targets
+// without pending entries get empty spans here (the write pass sets the others).
+__global__ void k_walk_prep(WalkArgs a, int start, int start_target, int nstart, int64_t *__restrict__ cnt,
+                            int64_t *__restrict__ pspan)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ntgt = ntgt_of(a);
+    if (j == 0 && ntgt_all(a) > a.tgt_cap) a.sz->overflow |= kOvfTargets;
+    if (j > a.tgt_cap) return;
+    if (j >= ntgt || a.sz->overflow) {
+        cnt[j] = 0;  // zeros up to the bound: the scan's entry ntgt is the total
+        return;
+    }
+    const int X = tgt_of(a)[j];
+    const int nleaf = a.sz->nleaf;
+    int64_t b = 0, e = 0;
+    if (start) {
+        if (start_target < 0) {
+            b = (int64_t)j * nstart;
+            e = b + nstart;
+        } else if (X == start_target) {
+            e = nstart;
+        }
+    } else {
+        const int par = a.info[X].parent;
+        b = a.dl_span[2 * (size_t)(par - nleaf)];
+        e = a.dl_span[2 * (size_t)(par - nleaf) + 1];
+    }
+    pspan[2 * j] = b;
+    pspan[2 * j + 1] = e;
+    cnt[j] = e - b;
+    if (X < nleaf) {
+        a.p2p_span[2 * (size_t)X] = a.p2p_span[2 * (size_t)X + 1] = 0;
+    } else {
+        a.dl_span[2 * (size_t)(X - nleaf)] = a.dl_span[2 * (size_t)(X - nleaf) + 1] = 0;
+    }
+    a.m2l_span[2 * (size_t)X] = a.m2l_span[2 * (size_t)X + 1] = 0;
+}
+
+// The depth's totals (P2P, M2L and down entries: the tile scan's last entry) against the
+// capacities: the list bases move past the previous depth's entries, and what would not
+// fit flags the overflow, after which nothing more is written.
+__global__ void k_walk_step(DevSizes *sz, const Cnt3 *__restrict__ tot, int64_t p2p_cap, int64_t m2l_cap,
+                            int64_t dl_cap, int64_t pair_cap)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    sz->p2p_base = sz->n_p2p;
+    sz->m2l_base = sz->n_m2l;
+    const Cnt3 t = *tot;
+    int ovf = 0;
+    if (sz->n_p2p + t.p > p2p_cap) ovf |= kOvfP2P;
+    if (sz->n_m2l + t.m > m2l_cap) ovf |= kOvfM2L;
+    if (t.d > dl_cap) ovf |= kOvfDown;
+    if (2 * t.d > pair_cap) ovf |= kOvfPairs;
+    if (ovf || sz->overflow) {
+        sz->overflow |= ovf;
+        sz->dl_total = 0;
+        return;
+    }
+    sz->n_p2p += t.p;
+    sz->n_m2l += t.m;
+    sz->dl_total = t.d;
+}
+
+// walk start: empty lists, bases at zero
+__global__ void k_walk_begin(DevSizes *sz)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    sz->n_p2p = sz->n_m2l = sz->p2p_base = sz->m2l_base = 0;
+    sz->dl_total = 0;
+}
+
+// after the last depth of a planned walk: down entries left over mean deeper targets
+// that were never visited
+__global__ void k_walk_end(DevSizes *sz)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    if (sz->dl_total > 0) sz->overflow |= kOvfDepths;
+}
+
+// cells by depth-first rank: (cell, subtree size) and geometry, read by the traversals
+__global__ void k_walk_records(const DevSizes *__restrict__ sz, const CellInfo *__restrict__ info,
+                               const CellGeom *__restrict__ geom, int2 *__restrict__ rec, CellGeom *__restrict__ rgeom)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (sz->overflow || c >= sz->ncell) return;
+    const CellInfo I = info[c];
+    rec[I.dfs] = make_int2(c, I.dsize);
+    rgeom[I.dfs] = geom[c];
+}
+
+// every span empty (cells a walk from a given pair never reaches keep no entries); n is
+// the launch bound of the cells
+__global__ void k_clear_spans(int n, int64_t *__restrict__ span)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 2 * (int64_t)n) span[i] = 0;
+}
+
+// start list: (root of box b, shift) for every box and shift, or (source, shift)
+__global__ void k_start_list(int32_t *f, const int *__restrict__ box_root, int nbox, int B, int shift_lo,
+                             int shift_hi)
 {
     const int nshift = shift_hi - shift_lo;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (box_root ? nbox : 1) * nshift) return;
     const int s = shift_lo + i % nshift;
-    if (box_root) {
-        const int r = box_root[i / nshift];
-        f[i] = pair_of(r, r, s);
-    } else {
-        f[i] = pair_of(A, B, s);
-    }
-}
-
-// CSR offsets of the target-sorted tasks: off[t] = first task with target >= t
-__global__ void k_csr_offsets(const uint32_t *__restrict__ tgt, int64_t n, int ntarget, int64_t *__restrict__ off)
-{
-    int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t > ntarget) return;
-    int64_t lo = 0, hi = n;
-    while (lo < hi) {
-        int64_t mid = (lo + hi) >> 1;
-        if (tgt[mid] < (uint32_t)t) lo = mid + 1;
-        else hi = mid;
-    }
-    off[t] = lo;
+    f[i] = pack_entry(box_root ? box_root[i / nshift] : B, s);
 }
 
 // ---------------------------------------------------------------------------- host side
-
-static int bits_for(int64_t v)
-{
-    int b = 1;
-    while (b < 40 && ((int64_t)1 << b) <= v) b++;
-    return b;
-}
 
 // grow to `need` bytes keeping the first `keep` bytes
 static void *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
@@ -365,134 +415,158 @@ static void *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
     return nw;
 }
 
-// Stable LSD radix sort of (target, entry) tasks by target (20-21 bits at 256^3
-// particles).  7-bit digits: three passes, measured 1.4 ms faster per evaluation than
-// 8-bit digits (also three passes) at 256^3; 10-bit digits (two passes) match 8, and
-// 11-bit ones are slower because their onesweep blocks hold 139 KB of LDS.  The item
-// count is a size_t, so lists beyond 2^31 entries sort too.
-constexpr unsigned kTaskSortBits = 7;
-using TaskSortConfig = rocprim::radix_sort_config<
-    rocprim::default_config, rocprim::default_config,
-    rocprim::radix_sort_onesweep_config<rocprim::kernel_config<1024, 8>, rocprim::kernel_config<1024, 8>,
-                                        kTaskSortBits, rocprim::block_radix_rank_algorithm::match>>;
-
-// the spare buffers take the alternate halves and may be swapped with the task buffers
-static void sort_tasks(Ctx &c, DevBuf &tgt, DevBuf &src, DevBuf &spare_t, DevBuf &spare_s, int64_t n, int end_bit)
-{
-    if (n <= 1) return;
-    uint32_t *t0 = reinterpret_cast<uint32_t *>(tgt.p);
-    int32_t *s0 = reinterpret_cast<int32_t *>(src.p);
-    uint32_t *t1 = reinterpret_cast<uint32_t *>(grow(spare_t, sizeof(uint32_t) * n));
-    int32_t *s1 = reinterpret_cast<int32_t *>(grow(spare_s, sizeof(int32_t) * n));
-    rocprim::double_buffer<uint32_t> kb(t0, t1);
-    rocprim::double_buffer<int32_t> vb(s0, s1);
-    size_t bytes = 0;
-    PH_CHECK(rocprim::radix_sort_pairs<TaskSortConfig>(nullptr, bytes, kb, vb, (size_t)n, 0, end_bit, c.stream));
-    void *tmp = grow(c.scan_tmp, bytes);
-    PH_CHECK(rocprim::radix_sort_pairs<TaskSortConfig>(tmp, bytes, kb, vb, (size_t)n, 0, end_bit, c.stream));
-    if (kb.current() != t0) std::swap(tgt, spare_t);
-    if (vb.current() != s0) std::swap(src, spare_s);
-}
-
+// The walk from the pair (start_target, start_source) over the shifts [shift_lo, shift_hi),
+// or from every box's (root, root) when start_target < 0.  Per depth: pending counts and
+// their scan, the count pass, the scan of its tile sums (the host reads the depth's three
+// totals to size the outputs), the write pass.
 void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int start_source)
 {
     hipStream_t st = c.stream;
-    const int nleaf = c.nleaf, ncell = c.ncell;
-    if (ncell >= (1 << 27)) throw std::runtime_error("more than 2^27 cells");
+    const bool planned = c.plan.active;
+    const int nlb = nleaf_bound(c), ncb = ncell_bound(c), nnb = nnode_bound(c);
+    if (ncb >= (1 << 27)) throw std::runtime_error("more than 2^27 cells");
+    need(c.cell_box, sizeof(int) * (size_t)ncb, "cell boxes");
+    need(c.walk_tgt, sizeof(int) * (size_t)ncb, "walk targets");
+    const bool forest = start_target < 0;
+    if (planned && !forest) throw std::runtime_error("a planned walk starts from the roots");
+    // depths of the walk: the tree's (ordinary) or the plan's
+    const int ndepth = planned ? c.plan.depths : (int)c.tgt_off.size() - 1;
 
-    BfsArgs a{};
-    a.geom = dptr<CellGeom>(c.geom);
+    DevSizes *sz = dptr<DevSizes>(c.sizes);
+    WalkArgs a{};
     a.info = dptr<CellInfo>(c.info);
-    a.nleaf = nleaf;
+    a.sz = sz;
     a.theta = c.prm.theta;
     a.theta2 = c.prm.theta * c.prm.theta;
     a.rc = c.prm.rc;
     a.rc2 = c.prm.rc * c.prm.rc;
     a.bx = box_view(c);
-    need(c.cell_box, sizeof(int) * (size_t)ncell, "cell boxes");
+    a.walk_tgt = dptr<int>(c.walk_tgt);
+    int2 *rec = static_cast<int2 *>(ctx_buf(c, c.walk_rec, sizeof(int2) * (size_t)ncb, "walk records"));
+    CellGeom *rgeom = static_cast<CellGeom *>(ctx_buf(c, c.walk_rgeom, sizeof(CellGeom) * (size_t)ncb, "walk records"));
+    k_walk_records<<<(ncb + 255) / 256, 256, 0, st>>>(sz, dptr<CellInfo>(c.info), dptr<CellGeom>(c.geom), rec, rgeom);
+    a.rec = rec;
+    a.rgeom = rgeom;
+    a.p2p_span = static_cast<int64_t *>(ctx_buf(c, c.p2p_off, sizeof(int64_t) * 2 * (size_t)std::max(1, nlb), "P2P list"));
+    a.m2l_span = static_cast<int64_t *>(ctx_buf(c, c.m2l_off, sizeof(int64_t) * 2 * (size_t)ncb, "M2L list"));
+    a.dl_span = static_cast<int64_t *>(ctx_buf(c, c.dl_span, sizeof(int64_t) * 2 * (size_t)std::max(1, nnb), "down lists"));
+    k_clear_spans<<<(2 * nlb + 255) / 256 + 1, 256, 0, st>>>(nlb, a.p2p_span);
+    k_clear_spans<<<(2 * ncb + 255) / 256 + 1, 256, 0, st>>>(ncb, a.m2l_span);
+    k_walk_begin<<<1, 1, 0, st>>>(sz);
 
-    // first frontier: the roots of all boxes, or the given pair (start_target >= 0)
-    const bool forest = start_target < 0;
+    // start: every box's root against itself, or the given pair; shifts [lo, hi)
     const int nshift = shift_hi - shift_lo;
     const int64_t n0 = (int64_t)(forest ? c.nbox : 1) * nshift;
     if (forest) need(c.d_box_root, sizeof(int) * (size_t)c.nbox, "box roots");
-    grow_keep(c.bfs_a, sizeof(uint64_t) * std::max<int64_t>(n0, 64), 0, st);
+    int32_t *slist = static_cast<int32_t *>(ctx_buf(c, c.start_list, sizeof(int32_t) * (size_t)std::max<int64_t>(n0, 1),
+                                                    "walk start"));
+    k_start_list<<<(int)((n0 + 255) / 256), 256, 0, st>>>(slist, forest ? dptr<int>(c.d_box_root) : nullptr, c.nbox,
+                                                          start_source, shift_lo, shift_hi);
+    int d0 = 0;
+    if (!forest) {
+        CellInfo I;
+        PH_CHECK(hipMemcpyAsync(&I, dptr<CellInfo>(c.info) + start_target, sizeof I, hipMemcpyDeviceToHost, st));
+        PH_CHECK(hipStreamSynchronize(st));
+        d0 = I.depth;
+    }
+
     int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 16;
-    k_init_frontier<<<(int)((n0 + 255) / 256), 256, 0, st>>>(reinterpret_cast<uint64_t *>(c.bfs_a.p),
-                                                             forest ? dptr<int>(c.d_box_root) : nullptr, c.nbox,
-                                                             start_target, start_source, shift_lo, shift_hi);
-    int64_t n_in = n0, n_p2p = 0, n_m2l = 0;
-    int levels = 0;
-    while (n_in > 0) {
-        const int ntile = (int)((n_in + kBfsTile - 1) / kBfsTile);
-        Cnt3 *tile_cnt = reinterpret_cast<Cnt3 *>(grow(c.cnt_tmp, sizeof(Cnt3) * 2 * (size_t)(ntile + 1)));
+    int64_t npair = n0, n_p2p = 0, n_m2l = 0;  // host copies (ordinary evaluation)
+    int64_t max_pairs = n0, max_dl = 0;
+    int depths = 0;
+    const int32_t *pend = slist;
+    DevBuf *dl_out = &c.dl_b, *dl_other = &c.dl_a;
+    for (int d = d0; d < ndepth && (planned || npair > 0); d++) {
+        // launch bounds of this depth: targets and pairs
+        const int64_t tcap = planned ? c.plan.tgt_cap : c.tgt_off[d + 1] - c.tgt_off[d];
+        const int64_t pcap = planned ? c.plan.pair_cap : npair;
+        a.d = d;
+        a.tgt_cap = tcap;
+        a.pend = pend;
+        int64_t *poff = static_cast<int64_t *>(ctx_buf(c, c.pair_off, sizeof(int64_t) * (size_t)(tcap + 1), "walk"));
+        int64_t *cnt = static_cast<int64_t *>(ctx_buf(c, c.pend_span, sizeof(int64_t) * 3 * (size_t)(tcap + 1), "walk"));
+        int64_t *pspan = cnt + (tcap + 1);
+        a.pair_off = poff;
+        a.pspan = pspan;
+        k_walk_prep<<<(int)((tcap + 1 + 255) / 256), 256, 0, st>>>(a, d == d0, forest ? -1 : start_target, nshift, cnt,
+                                                                   pspan);
+        {
+            size_t bytes = 0;
+            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, poff, (int)(tcap + 1), st));
+            void *tmp = scan_storage(c, bytes);
+            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, poff, (int)(tcap + 1), st));
+        }
+        const int ntile = (int)((pcap + kWalkThreads - 1) / kWalkThreads);  // >= 1: pcap >= 1
+        a.pcnt = static_cast<uint32_t *>(ctx_buf(c, c.pair_cnt, sizeof(uint32_t) * 3 * (size_t)std::max<int64_t>(pcap, 1),
+                                                 "walk pairs"));
+        a.npair_cap = pcap;
+        Cnt3 *tile_cnt = static_cast<Cnt3 *>(ctx_buf(c, c.cnt_tmp, sizeof(Cnt3) * 2 * (size_t)(ntile + 1), "walk tiles"));
         Cnt3 *tile_off = tile_cnt + (ntile + 1);
-        a.in = reinterpret_cast<uint64_t *>(c.bfs_a.p);
-        a.n_in = n_in;
-        a.act = static_cast<uint8_t *>(grow(c.bfs_act, (size_t)n_in));
-        // one count pass, one scan of the three counts, one copy of the three totals
-        k_bfs_count<<<ntile, kBfsThreads, 0, st>>>(a, tile_cnt, ntile);
+        k_walk_count<<<ntile, kWalkThreads, 0, st>>>(a, tile_cnt, ntile);
         {
             size_t bytes = 0;
             PH_CHECK(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, tile_cnt, tile_off, AddCnt3(), Cnt3{0, 0, 0},
                                                        ntile + 1, st));
-            void *tmp = grow(c.scan_tmp, bytes);
+            void *tmp = scan_storage(c, bytes);
             PH_CHECK(hipcub::DeviceScan::ExclusiveScan(tmp, bytes, tile_cnt, tile_off, AddCnt3(), Cnt3{0, 0, 0},
                                                        ntile + 1, st));
         }
-        PH_CHECK(hipMemcpyAsync(pin, tile_off + ntile, sizeof(Cnt3), hipMemcpyDeviceToHost, st));
-        PH_CHECK(hipEventRecord(c.counts_ready, st));
-        a.p2p_base = n_p2p;
-        a.m2l_base = n_m2l;
-        // A pair has at most 4 children, 4 P2P tasks and 1 M2L task.  When the buffers
-        // already hold that bound (repeated evaluations keep their capacity) the write is
-        // queued at once and the host reads the counts while it runs; otherwise the
-        // buffers grow to the exact counts first.
-        const bool fits = c.bfs_b.cap >= sizeof(uint64_t) * 4 * (size_t)n_in &&
-                          c.p2p_keys.cap >= 4 * (size_t)(n_p2p + 4 * n_in) &&
-                          c.p2p_src.cap >= 4 * (size_t)(n_p2p + 4 * n_in) &&
-                          c.m2l_keys.cap >= 4 * (size_t)(n_m2l + n_in) && c.m2l_src.cap >= 4 * (size_t)(n_m2l + n_in);
-        if (fits) {
-            a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
-            a.p2p_t = static_cast<uint32_t *>(c.p2p_keys.p);
-            a.p2p_s = static_cast<int32_t *>(c.p2p_src.p);
-            a.m2l_t = static_cast<uint32_t *>(c.m2l_keys.p);
-            a.m2l_s = static_cast<int32_t *>(c.m2l_src.p);
-            k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off);
+        int64_t p2p_cap, m2l_cap, dl_cap, next_pairs;
+        if (planned) {
+            p2p_cap = c.plan.p2p_cap;
+            m2l_cap = c.plan.m2l_cap;
+            dl_cap = c.plan.dl_cap;
+            next_pairs = c.plan.pair_cap;
+        } else {
+            // the depth's totals size its outputs
+            PH_CHECK(hipMemcpyAsync(pin, tile_off + ntile, sizeof(Cnt3), hipMemcpyDeviceToHost, st));
+            PH_CHECK(hipEventRecord(c.counts_ready, st));
+            PH_CHECK(hipEventSynchronize(c.counts_ready));
+            const int64_t add_p2p = pin[0], add_m2l = pin[1], add_dl = pin[2];
+            grow_keep(c.p2p_src, 4 * (size_t)std::max<int64_t>(n_p2p + add_p2p, 1), 4 * (size_t)n_p2p, st);
+            grow_keep(c.m2l_src, 4 * (size_t)std::max<int64_t>(n_m2l + add_m2l, 1), 4 * (size_t)n_m2l, st);
+            // the pending list (the other ping-pong buffer) is read by this write: only the
+            // output buffer may move
+            grow_keep(*dl_out, 4 * (size_t)std::max<int64_t>(add_dl, 1), 0, st);
+            n_p2p += add_p2p;
+            n_m2l += add_m2l;
+            npair = 2 * add_dl;  // the children of this depth's nodes read their parents' lists
+            max_pairs = std::max(max_pairs, npair);
+            max_dl = std::max(max_dl, add_dl);
+            depths = d + 1;
+            p2p_cap = c.p2p_src.cap / 4;
+            m2l_cap = c.m2l_src.cap / 4;
+            dl_cap = dl_out->cap / 4;
+            next_pairs = npair;
         }
-        PH_CHECK(hipEventSynchronize(c.counts_ready));
-        const int64_t n_out = pin[0], add_p2p = pin[1], add_m2l = pin[2];
-        if (!fits) {
-            grow_keep(c.bfs_b, sizeof(uint64_t) * std::max<int64_t>(n_out, 1), 0, st);
-            const size_t np = std::max<int64_t>(n_p2p + add_p2p, 1), nm = std::max<int64_t>(n_m2l + add_m2l, 1);
-            a.p2p_t = static_cast<uint32_t *>(grow_keep(c.p2p_keys, 4 * np, 4 * n_p2p, st));
-            a.p2p_s = static_cast<int32_t *>(grow_keep(c.p2p_src, 4 * np, 4 * n_p2p, st));
-            a.m2l_t = static_cast<uint32_t *>(grow_keep(c.m2l_keys, 4 * nm, 4 * n_m2l, st));
-            a.m2l_s = static_cast<int32_t *>(grow_keep(c.m2l_src, 4 * nm, 4 * n_m2l, st));
-            a.out = reinterpret_cast<uint64_t *>(c.bfs_b.p);
-            k_bfs_write<<<ntile, kBfsThreads, 0, st>>>(a, tile_off);
-        }
-        n_in = n_out;
-        n_p2p += add_p2p;
-        n_m2l += add_m2l;
-        std::swap(c.bfs_a, c.bfs_b);
-        if (++levels > 4 * kMaxDepth + 8) throw std::runtime_error("dual-tree walk does not terminate");
+        // at the plan's capacities (allocated by the first planned evaluation, checked by the
+        // others); an ordinary evaluation grew them above
+        ctx_buf(c, c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(p2p_cap, 1), "P2P list");
+        ctx_buf(c, c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(m2l_cap, 1), "M2L list");
+        ctx_buf(c, *dl_out, sizeof(int32_t) * (size_t)std::max<int64_t>(dl_cap, 1), "down lists");
+        a.p2p_out = dptr<int32_t>(c.p2p_src);
+        a.m2l_out = dptr<int32_t>(c.m2l_src);
+        a.dl_out = dptr<int32_t>(*dl_out);
+        k_walk_step<<<1, 1, 0, st>>>(sz, tile_off + ntile, p2p_cap, m2l_cap, dl_cap, next_pairs);
+        k_walk_write<<<ntile, kWalkThreads, 0, st>>>(a, tile_off);
+        pend = a.dl_out;
+        std::swap(dl_out, dl_other);
     }
-    c.n_p2p = n_p2p;
-    c.n_m2l = n_m2l;
+    if (planned) {
+        k_walk_end<<<1, 1, 0, st>>>(sz);
+    } else {
+        c.n_p2p = n_p2p;
+        c.n_m2l = n_m2l;
+        if (forest) {
+            Measure &m = c.meas;
+            m.depths = depths;
+            m.pairs = max_pairs;
+            m.dl = max_dl;
+            m.p2p = n_p2p;
+            m.m2l = n_m2l;
+        }
+    }
     c.lists_valid = true;
-
-    // group by target, then CSR offsets
-    sort_tasks(c, c.p2p_keys, c.p2p_src, c.bfs_a, c.bfs_b, n_p2p, bits_for(nleaf));
-    sort_tasks(c, c.m2l_keys, c.m2l_src, c.bfs_a, c.bfs_b, n_m2l, bits_for(ncell));
-    int64_t *p2p_off = reinterpret_cast<int64_t *>(grow(c.p2p_off, sizeof(int64_t) * (size_t)(nleaf + 1)));
-    int64_t *m2l_off = reinterpret_cast<int64_t *>(grow(c.m2l_off, sizeof(int64_t) * (size_t)(ncell + 1)));
-    grow(c.p2p_src, sizeof(int32_t));
-    grow(c.m2l_src, sizeof(int32_t));
-    k_csr_offsets<<<(nleaf + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint32_t *>(c.p2p_keys.p), n_p2p, nleaf,
-                                                           p2p_off);
-    k_csr_offsets<<<(ncell + 1 + 255) / 256, 256, 0, st>>>(reinterpret_cast<uint32_t *>(c.m2l_keys.p), n_m2l, ncell,
-                                                           m2l_off);
     PH_CHECK(hipGetLastError());
 }
 

@@ -10,7 +10,11 @@
  *   IN   int32 n, maxleaf, 0, 0; double boxsize, mass, soft, rs, rc, theta;
  *        n x 3 double positions
  *   OUT  n x 3 double accelerations in input order, then the int64 counters
- *        idxP2P, idxM2L, walk_m2l_count, p2p_count_remote, leaves, nodes        */
+ *        idxP2P, idxM2L, walk_m2l_count, p2p_count_remote, leaves, nodes, then the
+ *        double timers dtime_prep, dtime_task, dtime_ext, dtime_fmm, dtime_p2p,
+ *        dtime_m2l [s] and the library's phase times of the call [ms]
+ *        (PHOTONS_NPHASE doubles).  The timers are printed as src/photoNs.c:122-127
+ *        prints them.                                                            */
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -22,6 +26,7 @@ extern photons_body *part;
 extern int NPART, MAXLEAF, first_leaf, last_leaf, first_node, last_node;
 extern double MASSPART, SoftenScale, splitRadius, cutoffRadius, open_angle, BOXSIZE;
 extern unsigned long idxP2P, idxM2L, walk_m2l_count, p2p_count_remote;
+extern double dtime_prep, dtime_task, dtime_ext, dtime_fmm, dtime_p2p, dtime_m2l;
 
 /* the drop-in layer exits on a fatal error, as the reference does; report its message */
 static void report(void)
@@ -80,11 +85,20 @@ int main(int argc, char **argv)
     }
     int64_t cnt[6] = {(int64_t)idxP2P, (int64_t)idxM2L, (int64_t)walk_m2l_count, (int64_t)p2p_count_remote,
                       (int64_t)(last_leaf - first_leaf), (int64_t)(last_node - first_node + 1)};
+    /* src/photoNs.c:122-127 */
+    printf(" dtime_m2l = %lf, dtime_p2p = %lf [sec]\n", dtime_m2l, dtime_p2p);
+    printf(" dtime_fmm/task = %lf, task = %lf [sec]\n", dtime_fmm, dtime_task);
+    printf(" dtime_prep = %lf, ext = %lf [sec]\n", dtime_prep, dtime_ext);
+    double tm[6] = {dtime_prep, dtime_task, dtime_ext, dtime_fmm, dtime_p2p, dtime_m2l};
+    double ph[PHOTONS_NPHASE];
+    if (photons_get_phase_ms(photons_compat_context(), ph) != 0) die(photons_last_error());
     fmm_deconstruct();
     f = fopen(argv[2], "wb");
     if (!f) die("cannot open the output");
     fwrite(acc, sizeof(double) * 3, (size_t)n, f);
     fwrite(cnt, sizeof cnt, 1, f);
+    fwrite(tm, sizeof tm, 1, f);
+    fwrite(ph, sizeof ph, 1, f);
     fclose(f);
     free(acc);
     free(pos);

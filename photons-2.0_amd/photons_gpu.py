@@ -66,6 +66,10 @@ def lib():
         L.photons_get_p2p_stage.argtypes = [vp]
         L.photons_autotune_boxes.argtypes = [vp, C.c_int, i64p, dp, vp, vp, C.c_int, C.POINTER(C.c_int),
                                              C.POINTER(C.c_int), dp]
+        L.photons_graph_create.argtypes = [vp, C.c_int, i64p, dp, vp, vp, C.c_double, C.c_int]
+        L.photons_graph_launch.argtypes = [vp]
+        L.photons_graph_check.argtypes = [vp, C.POINTER(C.c_int), dp]
+        L.photons_graph_destroy.argtypes = [vp]
         L.photons_prepare_host.argtypes = [vp, C.c_int64, dp]
         L.photons_evaluate_host.argtypes = [vp, dp]
         L.photons_tree_perm.argtypes = [vp, ip]
@@ -73,6 +77,7 @@ def lib():
         L.photons_p2p_list.argtypes = [vp, C.c_int64, ip, ip, bp, i64p]
         L.photons_m2l_list.argtypes = [vp, C.c_int64, ip, ip, bp, i64p]
         L.photons_expansions.argtypes = [vp, dp, dp]
+        L.photons_list_lengths.argtypes = [vp, C.c_int, i64p]
         for name in ("photons_p2m_batch",):
             getattr(L, name).argtypes = [vp, C.c_int, C.c_int, vp, vp, vp, vp, C.c_double, vp]
         L.photons_m2m_batch.argtypes = [vp, C.c_int, C.c_int, vp, vp, vp]
@@ -177,6 +182,28 @@ class Context:
                                                 _dp(lengths), C.c_void_p(d_pos), C.c_void_p(d_acc)),
                "photons_force_boxes_device")
 
+    def graph_create(self, counts, lengths, d_pos, d_acc, margin=0.25, capture=True):
+        """plan the evaluation of these boxes at device arrays d_pos -> d_acc (pointers as
+        ints) and capture it into a hipGraph; d_acc holds the force of d_pos afterwards"""
+        counts = np.ascontiguousarray(counts, dtype=np.int64)
+        lengths = np.ascontiguousarray(lengths, dtype=np.float64)
+        _check(lib().photons_graph_create(self.h, len(counts), counts.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          _dp(lengths), C.c_void_p(d_pos), C.c_void_p(d_acc), float(margin),
+                                          1 if capture else 0), "photons_graph_create")
+
+    def graph_launch(self):
+        """queue one replay of the planned evaluation (returns at once)"""
+        _check(lib().photons_graph_launch(self.h), "photons_graph_launch")
+
+    def graph_check(self):
+        """wait; returns (overflow bits of the last launch, its GPU ms or -1)"""
+        ovf, ms = C.c_int(), C.c_double()
+        _check(lib().photons_graph_check(self.h, C.byref(ovf), C.byref(ms)), "photons_graph_check")
+        return ovf.value, ms.value
+
+    def graph_destroy(self):
+        _check(lib().photons_graph_destroy(self.h), "photons_graph_destroy")
+
     def step_device(self, pm, n, d_pos, d_vel, d_acc, d_acc_pm, dkh, dd):
         """one TreePM step of [n][3] device arrays (pointers as ints): half kicks with
         the previous acc_pm and acc, drift, wrap, new acc and acc_pm, half kicks; pm may
@@ -224,6 +251,15 @@ class Context:
         st = Stats()
         _check(lib().photons_get_stats(self.h, C.byref(st)), "photons_get_stats")
         return st.as_dict()
+
+    def list_lengths(self, which):
+        """entries per target of the last force call: 'p2p' (per leaf) or 'm2l' (per cell)"""
+        st = self.stats()
+        nt = st["nleaf"] if which == "p2p" else st["nleaf"] + st["nnode"]
+        out = np.zeros(nt, np.int64)
+        _check(lib().photons_list_lengths(self.h, 0 if which == "p2p" else 1,
+                                          out.ctypes.data_as(C.POINTER(C.c_int64))), "photons_list_lengths")
+        return out
 
     def perm(self, n):
         p = np.zeros(n, np.int32)

@@ -61,3 +61,20 @@ LARGE_CASES = {"clus64_pm64", "clus48_pm48_th0.3"}
 
 def case_inputs(name):
     return FORCE_CASES[name]()
+
+
+# The forest case: three clustered boxes of 16^3..24^3 particles (log-uniform, seeded) at
+# the density of the 256^3 workload, evaluated together; one explicit parameter set for
+# every box (mass, softening, rs and rc of 256^3 particles in 500 Mpc/h, p = 3), each
+# box with its own length.  The golden vectors hold each box's reference force alone.
+def forest_inputs():
+    counts, lengths, pos, prm = ic.workload_ensemble(3, 16 ** 3, 24 ** 3, seed=31, order=3)
+    return counts, lengths, pos, prm
+
+
+def forest_box_params(prm, length):
+    return ic.ForceParams(**{**prm.as_dict(), "boxsize": float(length)})
+
+
+# the multipoles of every cell after fmm_prepare, for walk_m2m
+TREE_CASES = ("clus16_pm32",)

@@ -11,6 +11,8 @@ Outputs, all committed as small fixtures next to this script:
                     counters; written only when the oracle agrees with the reference.
   lcdm_pos.npy      positions of demo/ic_lcdm.gdt2 (float32, exactly as stored there)
   ops.npz           single-operator calls: inputs and outputs
+  tree_<case>.npz   the multipoles of every node and leaf after fmm_prepare (walk_m2m)
+  forest.npz        three boxes of one parameter set, each box's force alone
 
     python tests/golden/make_golden.py [case ...]
 """
@@ -33,7 +35,7 @@ sys.path.insert(0, HERE)
 
 import ic  # noqa: E402
 import oracle  # noqa: E402
-from cases import FORCE_CASES, STRIDE, case_inputs  # noqa: E402
+from cases import FORCE_CASES, STRIDE, TREE_CASES, case_inputs, forest_box_params, forest_inputs  # noqa: E402
 
 REFERENCE = "/root/reference"
 
@@ -171,14 +173,53 @@ def write_ops(tmp: str):
     print("ops", {k: v.shape for k, v in out.items()})
 
 
+def write_tree_cases(tmp: str):
+    for name in TREE_CASES:
+        pos, prm = case_inputs(name)
+        node_M, leaf_M = oracle.ref_tree_multipoles(pos, prm, tmp)
+        np.savez_compressed(os.path.join(HERE, f"tree_{name}.npz"), pos_sha256=np.array(pos_digest(pos)),
+                            node_M=node_M, leaf_M=leaf_M)
+        print("tree", name, node_M.shape, leaf_M.shape, flush=True)
+
+
+def write_forest(tmp: str):
+    counts, lengths, pos, prm = forest_inputs()
+    off = np.concatenate([[0], np.cumsum(counts)])
+    idx, acc, cnts = [], [], []
+    for b in range(len(counts)):
+        pb, p = pos[off[b]:off[b + 1]], forest_box_params(prm, lengths[b])
+        if not oracle.Tree(pb, p).fits_reference_buffers():
+            raise SystemExit(f"forest box {b}: exceeds the reference's fixed buffers")
+        a, cnt = oracle.ref_force(pb, p, tmp)
+        oa, _ = oracle.force(pb, p)
+        agree = float(np.max(np.linalg.norm(oa - a, axis=1) / np.linalg.norm(a, axis=1)))
+        if agree > 1e-12:
+            raise SystemExit(f"forest box {b}: the oracle and the reference differ by {agree:.2e}")
+        k = np.arange(0, len(pb), 8)
+        idx.append(off[b] + k)
+        acc.append(a[k])
+        cnts.append([cnt[x] for x in ("idxP2P", "idxM2L", "walk_m2l_count", "p2p_count_remote", "nleaf", "nnode")])
+        print("forest box", b, len(pb), cnt, f"oracle vs reference {agree:.1e}", flush=True)
+    np.savez_compressed(os.path.join(HERE, "forest.npz"), pos_sha256=np.array(pos_digest(pos)),
+                        counts=counts, lengths=lengths, idx=np.concatenate(idx).astype(np.int64),
+                        acc=np.concatenate(acc), counters=np.array(cnts, np.int64))
+
+
 def main():
-    """no arguments: every fixture; otherwise only the named force cases"""
+    """no arguments: every fixture; `tree` or `forest`: that fixture; otherwise only the
+    named force cases"""
     if not oracle.ref_available():
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref"], check=True)
     names = sys.argv[1:] or list(FORCE_CASES)
     with tempfile.TemporaryDirectory() as tmp:
         if not sys.argv[1:]:
             write_ops(tmp)
+        if sys.argv[1:] in (["tree"], ["forest"]):
+            (write_tree_cases if sys.argv[1] == "tree" else write_forest)(tmp)
+            return
+        if not sys.argv[1:]:
+            write_tree_cases(tmp)
+            write_forest(tmp)
         write_force_cases(tmp, names)
 
 

@@ -122,3 +122,15 @@ def test_run_program_fails_loudly_without_a_gpu(tmp_path):
     prm.write_text(f"InputPath {ic_path}\nSnapFormat -2\nNumMeshSide 16\n")
     r = subprocess.run([exe, str(prm)], capture_output=True, text=True, timeout=60)
     assert r.returncode != 0 and "SnapFormat" in r.stderr
+
+
+def test_pm_mesh_index_bound(lib):
+    """the PM's CIC corner keys are 32-bit mesh indices: a mesh with N^3 >= 2^32 cells
+    (N >= 1626) is refused before anything touches the GPU"""
+    h = C.c_void_p()
+    lib.photons_pm_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_double, C.c_double, C.c_double,
+                                      C.c_int, C.c_void_p]
+    lib.photons_pm_last_error.restype = C.c_char_p
+    for nside in (1626, 1700, 2048):
+        assert lib.photons_pm_create(C.byref(h), nside, 1000.0, 10.0, 1.0, 0, None) != 0
+        assert b"1624" in lib.photons_pm_last_error()

@@ -67,6 +67,31 @@ def test_forest_fp64_matches_oracle(ensemble):
         assert rel(acc[sl], want).max() <= 1e-10
 
 
+def test_forest_matches_reference_golden():
+    """three boxes of one explicit parameter set evaluated as one forest: each box's
+    force equals the reference's force of that box alone (tests/golden/forest.npz, written
+    by the reference FMM subset per box) to 1e-10, and the list counters are the sums of
+    the reference's per-box counters"""
+    import os
+
+    from cases import forest_inputs
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "forest.npz"),
+                allow_pickle=False)
+    counts, lengths, pos, prm = forest_inputs()
+    np.testing.assert_array_equal(counts, g["counts"])
+    np.testing.assert_array_equal(lengths, g["lengths"])
+    ctx = pg.Context(prm, pg.FP64)
+    acc = ctx.force_boxes(counts, lengths, pos)
+    e = rel(acc[g["idx"]], g["acc"])
+    print(f"forest vs reference: max {e.max():.2e}")
+    assert e.max() <= 1e-10
+    st = ctx.stats()
+    c = g["counters"].sum(axis=0)
+    assert st["p2p_local_pairs"] == c[0] and st["m2l_local_pairs"] == c[1]
+    assert st["m2l_pairs"] == c[2] and st["p2p_image_interactions"] == c[3]
+    assert st["nleaf"] == c[4] and st["nnode"] == c[5]
+
+
 def test_fp32_forest_accuracy(ensemble):
     counts, lengths, pos, prm, off = ensemble
     a64 = pg.Context(prm, pg.FP64).force_boxes(counts, lengths, pos)

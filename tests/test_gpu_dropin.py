@@ -175,6 +175,47 @@ def test_single_operators_match_reference_golden():
         assert r.lib.acceptance(dp(wi), dp(wj), dp(di)) == flag
 
 
+class RefNode(C.Structure):
+    """Node of inc/typesdef.h:25-36 (p = 3), as the library mirrors btree[]"""
+    _fields_ = [("updated", C.c_int), ("npart", C.c_int), ("son", C.c_int * 2), ("split", C.c_double),
+                ("width", C.c_double * 3), ("center", C.c_double * 3), ("M", C.c_double * 20),
+                ("L", C.c_double * 20)]
+
+
+def test_walk_m2m_matches_reference_multipoles():
+    """walk_m2m(first_node) (src/operator.c:165-194) through the drop-in layer on node
+    mirrors whose multipoles were zeroed: every node's M equals the one the reference's
+    own fmm_prepare computed (tests/golden/tree_clus16_pm32.npz), and so does the M the
+    GPU's M2M pass left there before"""
+    g = np.load(os.path.join(GOLDEN, "tree_clus16_pm32.npz"), allow_pickle=False)
+    r = Ref()
+    pos, prm = case_inputs("clus16_pm32")
+    r.set_params(prm, len(pos))
+    body = bodies(pos)
+    r.set_part(body)
+    r.lib.fmm_construct()
+    r.lib.fmm_prepare()
+    fn, ln = r.i("first_node").value, r.i("last_node").value
+    nnode = ln - fn + 1
+    assert nnode == len(g["node_M"])
+    base = C.c_void_p.in_dll(r.lib, "btree").value + fn * C.sizeof(RefNode)
+    nodes = (RefNode * nnode).from_address(base)
+    ref = g["node_M"]
+
+    def check(M):
+        for lo, hi in ((0, 1), (1, 4), (4, 10), (10, 20)):
+            scale = np.abs(ref[:, lo:hi]).max()
+            assert np.abs(M[:, lo:hi] - ref[:, lo:hi]).max() <= 1e-12 * scale, (lo, hi)
+
+    check(np.array([list(nd.M) for nd in nodes]))
+    for nd in nodes:
+        for q in range(20):
+            nd.M[q] = 0.0
+    r.lib.walk_m2m(fn)
+    check(np.array([list(nd.M) for nd in nodes]))
+    r.lib.fmm_deconstruct()
+
+
 def test_pair_kernels_of_kernels_h():
     r = Ref()
     pos, prm = case_inputs("clus16_pm32")
@@ -224,18 +265,20 @@ def run_driver(pos, prm, tmp, fp32=False):
     raw = open(fout, "rb").read()
     n = len(pos)
     acc = np.frombuffer(raw[: 24 * n], dtype=np.float64).reshape(n, 3)
-    cnt = np.frombuffer(raw[24 * n:], dtype=np.int64)
-    return acc, cnt
+    cnt = np.frombuffer(raw[24 * n: 24 * n + 48], dtype=np.int64)
+    tail = np.frombuffer(raw[24 * n + 48:], dtype=np.float64)
+    timers = dict(zip(("prep", "task", "ext", "fmm", "p2p", "m2l"), tail[:6]))
+    return acc, cnt, timers, tail[6:]
 
 
 @pytest.mark.parametrize("name", ["clus16_pm32", "lattice32_rc6"])
 def test_compiled_c_driver_matches_reference_golden(name, tmp_path):
     pos, prm = case_inputs(name)
-    acc, cnt = run_driver(pos, prm, str(tmp_path))
+    acc, cnt, _, _ = run_driver(pos, prm, str(tmp_path))
     g = check_golden(name, acc)
     assert list(cnt[:4]) == list(g["counters"][:4])
     assert cnt[4] == g["counters"][4] and cnt[5] == g["counters"][5]
-    a32, _ = run_driver(pos, prm, str(tmp_path), fp32=True)
+    a32, _, _, _ = run_driver(pos, prm, str(tmp_path), fp32=True)
     e = np.linalg.norm(a32[g["idx"]] - g["acc"], axis=1) / np.linalg.norm(g["acc"], axis=1)
     assert np.percentile(e, 99) <= 1e-4
 
@@ -257,3 +300,22 @@ def test_partmesh_drop_in_matches_the_pm_oracle():
     assert np.abs(got - want).max() <= 1e-11 * np.abs(want).max()
     # the FMM globals and part[].pos are untouched
     np.testing.assert_array_equal(body[:, 0:3], pos)
+
+
+def test_reference_timers_are_written(tmp_path):
+    """dtime_prep/task/ext/fmm/p2p/m2l (inc/photoNs.h:95-111) as the reference driver
+    prints them (src/photoNs.c:122-127): non-zero, the kernel timers equal the library's
+    phase times of the same call, and the phases of fmm_task fit inside its wall time"""
+    pos, prm = case_inputs("clus24_pm48")
+    _, _, t, ph = run_driver(pos, prm, str(tmp_path))
+    print(t, ph)
+    for k in ("prep", "task", "ext", "fmm", "p2p", "m2l"):
+        assert t[k] > 0.0, k
+    P = pg.PHASES
+    assert t["p2p"] == 1e-3 * ph[P.index("p2p")]
+    assert t["m2l"] == 1e-3 * ph[P.index("m2l")]
+    gpu_task = 1e-3 * sum(ph[P.index(k)] for k in ("walk", "m2l", "l2l", "p2p"))
+    assert gpu_task <= t["task"]
+    gpu_prep = 1e-3 * sum(ph[P.index(k)] for k in ("upload", "tree", "upward"))
+    assert gpu_prep <= t["prep"]
+    assert abs(t["fmm"] - (t["prep"] + 1e-3 * ph[P.index("l2l")])) <= 1e-12 * t["fmm"]

@@ -143,6 +143,66 @@ def test_context_reuse_across_sizes():
         assert rel_err(ctx.force(pos), ref).max() <= 1e-10
 
 
+def _plane_value(n, L):
+    """a coordinate v whose mean over n copies rounds below v, both in the reference's
+    sequential sum and in the device's chunked sum (64 lanes, then a butterfly)"""
+    for k in range(1, 2000):
+        v = L * k / 2003.0
+        seq = 0.0
+        for _ in range(n):
+            seq += v
+        lanes = np.zeros(64)
+        for i in range(n):
+            lanes[i % 64] += v
+        o = 32
+        while o >= 1:
+            lanes = lanes + lanes[np.arange(64) ^ o]
+            o //= 2
+        if seq / n < v and lanes[0] / n < v:
+            return v
+    raise AssertionError("no such coordinate")
+
+
+def test_plane_of_equal_coordinates_matches_oracle():
+    """100 particles sharing one x coordinate whose mean rounds below it: the reference's
+    two-pointer partition (src/fmm.c:56-77) leaves the root's left child empty and passes
+    every particle right; tree, lists and force are the oracle's"""
+    _, prm = case_inputs("clus16_pm32")
+    n = 100
+    rng = np.random.default_rng(17)
+    pos = rng.uniform(0.0, prm.boxsize, (n, 3))
+    pos[:, 0] = _plane_value(n, prm.boxsize)
+    ctx = pg.Context(prm, pg.FP64)
+    acc = ctx.force(pos)
+    t = oracle.Tree(pos, prm)
+    ex = t.export()
+    cells = ctx.cells()
+    nl = t.nleaf
+    assert cells["nleaf"] == nl and cells["nnode"] == t.nnode
+    root = nl  # nodes follow the leaves, the root first
+    left = cells["son"][root][0]
+    assert left < nl and cells["npart"][left] == 0
+    np.testing.assert_array_equal(cells["npart"][:nl], ex["leaf_npart"])
+    np.testing.assert_array_equal(cells["npart"][nl:], ex["node_npart"])
+    ref, _ = oracle.force(pos, prm)
+    assert rel_err(acc, ref).max() <= 1e-10
+
+
+def test_zero_softening_matches_oracle():
+    """SOFTENING 0 is a valid reference setting (src/initial.c:123 refuses only < 0): a
+    particle with itself then adds nothing, as the reference's skip of that pair"""
+    pos, prm = case_inputs("clus16_pm32")
+    prm.soft = 0.0
+    ref, _ = oracle.force(pos, prm)
+    assert np.isfinite(ref).all()
+    a64 = pg.Context(prm, pg.FP64).force(pos)
+    assert np.isfinite(a64).all()
+    assert rel_err(a64, ref).max() <= 1e-10
+    a32 = pg.Context(prm, pg.FP32_MIXED).force(pos)
+    assert np.isfinite(a32).all()
+    assert np.percentile(rel_err(a32, ref), 99) <= 1e-4
+
+
 def test_edge_inputs():
     _, prm = case_inputs("clus16_pm32")
     ctx = pg.Context(prm, pg.FP64)
@@ -158,8 +218,8 @@ def test_edge_inputs():
         acc = pg.Context(p, pg.FP64).force(pos)
         scale = np.abs(ref).max() + 1e-300
         assert np.abs(acc - ref).max() <= 1e-10 * scale, n
-    # more than maxleaf particles at one position: the reference's partition would
-    # recurse forever; here the node splits off one particle per level
+    # more than maxleaf particles at one position: where the reference's partition would
+    # recurse forever (every mean rounding low), the node splits off one particle
     rng = np.random.default_rng(5)
     pos = np.concatenate([rng.uniform(0, prm.boxsize, (300, 3)), np.repeat(rng.uniform(0, prm.boxsize, (1, 3)), 50, 0)])
     ctx = pg.Context(prm, pg.FP64)

@@ -92,3 +92,25 @@ def test_acceptance_matches_reference():
         assert oracle.acceptance(wi, wj, dist, theta, rc) == flag
     # every outcome is represented
     assert set(np.unique(g["acc_flag"])) == {-1, 0, 1}
+
+
+def test_forest_boxes_match_reference():
+    """the forest fixture: each box's force from the reference alone, reproduced by the
+    oracle on that box with the shared parameter set"""
+    from cases import forest_box_params, forest_inputs
+    g = load("forest.npz")
+    counts, lengths, pos, prm = forest_inputs()
+    digest = hashlib.sha256(np.ascontiguousarray(pos, np.float64).tobytes()).hexdigest()
+    assert digest == str(g["pos_sha256"]), "input generator drifted from the golden inputs"
+    off = np.concatenate([[0], np.cumsum(counts)])
+    idx = g["idx"]
+    for b in range(len(counts)):
+        acc, st = oracle.force(pos[off[b]:off[b + 1]], forest_box_params(prm, lengths[b]))
+        sel = (idx >= off[b]) & (idx < off[b + 1])
+        ref = g["acc"][sel]
+        rel = np.linalg.norm(acc[idx[sel] - off[b]] - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        assert rel.max() < 1e-12, (b, rel.max())
+        c = g["counters"][b]
+        assert st["p2p_local_pairs"] == c[0] and st["m2l_local_pairs"] == c[1]
+        assert st["m2l_calls"] == c[2] and st["p2p_ext_inter"] == c[3]
+        assert st["nleaf"] == c[4] and st["nnode"] == c[5]

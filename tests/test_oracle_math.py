@@ -290,3 +290,34 @@ def test_force_call_against_direct_sum():
     mag = np.linalg.norm(ref, axis=1)
     assert np.percentile(err / mag, 99) < 1e-5
     assert err.max() < 1e-3 * np.median(mag)
+
+
+def test_force_on_selected_leaves_is_the_whole_force_there():
+    """orc_force_leaves (the headline-size parity checker) against orc_force: the same
+    walks entered only above the selected leaves give those leaves' particles the same
+    force bit for bit, and the same list lengths as the full lists"""
+    from cases import case_inputs
+    pos, prm = case_inputs("clus24_pm48")
+    prm.order = 4
+    full, _ = oracle.force(pos, prm)
+    t = oracle.Tree(pos, prm)
+    sel = np.random.default_rng(4).choice(t.nleaf, 25, replace=False).astype(np.int32)
+    acc, ids, plen, mlen = oracle.force_leaves(pos, prm, sel)
+    ex = t.export()
+    assert len(ids) == ex["leaf_npart"][sel].sum()
+    np.testing.assert_array_equal(acc, full[ids])
+    tg, _, _ = t.p2p_list()
+    np.testing.assert_array_equal(plen, np.bincount(tg, minlength=t.nleaf)[sel])
+    # M2L entries of each leaf and its ancestors, from the full list
+    mt, _, _ = t.m2l_list()
+    per_cell = np.bincount(mt, minlength=t.nleaf + t.nnode)
+    parent = np.full(t.nleaf + t.nnode, -1)
+    for v, (a, b) in enumerate(ex["node_son"]):
+        for c in (a, b):
+            parent[c + t.nleaf if c >= 0 else -c - 1] = t.nleaf + v
+    for k, leaf in enumerate(sel):
+        tot, c = 0, int(leaf)
+        while c >= 0:
+            tot += per_cell[c]
+            c = parent[c]
+        assert mlen[k] == tot
