@@ -75,6 +75,7 @@ struct DevSizes {
     int64_t npair;                    // walk pairs of the current depth
     int64_t p2p_base, m2l_base;       // list entries written by the walk's earlier depths
     int64_t dl_total;                 // down-list entries written by the walk's current depth
+    int64_t n_big;                    // walk traversals of the current depth continued by a wavefront
     int lvl[kMaxLevels + 1];          // first temporary node of depth d
     int ldo[kMaxLevels + 1];          // first temporary leaf of depth d
     int64_t tgt_off[kMaxLevels + 1];  // first walk target of depth d (walk_tgt)
@@ -138,20 +139,19 @@ struct Ctx {
     DevBuf px, py, pz, pidx;       // tree-ordered particles
     DevBuf qx, qy, qz, qidx;       // build double buffer
     DevBuf ox, oy, oz;             // FP32 offsets
-    DevBuf acc;                    // tree-order acceleration [3][n] doubles
     DevBuf geom, info;             // CellGeom / CellInfo [ncell]
     DevBuf level_nodes;            // node cell ids by depth
     DevBuf M, L;                   // expansions
     // lists: entries grouped by target; p2p_off / m2l_off hold one span (begin, end) of
     // entries per target leaf / cell: off[2 t] .. off[2 t + 1]
     DevBuf p2p_off, p2p_src, m2l_off, m2l_src, cnt_tmp;
-    // walk: the cells by depth-first rank (id and subtree size; geometry), the targets
-    // grouped by depth, each node's list of sources passed down to its two children
-    // (ping-pong by depth) with its span, and per-depth work arrays
-    DevBuf walk_rec, walk_rgeom, walk_tgt, walk_tgt_off, dl_a, dl_b, dl_span;
-    DevBuf pair_off, pend_span, pair_cnt, start_list;
+    // walk: the cells by depth-first rank (geometry, id, subtree size, children), the
+    // targets grouped by depth, each node's list of sources passed down to its two children
+    // (ping-pong by depth) with its span, and per-depth work arrays (target records, the
+    // target of each tile's first pair, pair counts, continued traversals)
+    DevBuf walk_rec, walk_trec, walk_tiles, walk_tgt, dl_a, dl_b, dl_span;
+    DevBuf pair_off, pend_span, pair_cnt, start_list, big_pairs;
     std::vector<int64_t> tgt_off;             // walk targets of depth d: walk_tgt[tgt_off[d] .. tgt_off[d+1])
-    std::vector<int> leaf_depth_off;          // tree temporaries: leaves of depth d are [off[d], off[d+1])
     DevBuf p2p_order, p2p_order_tmp;          // P2P launch order (heaviest target leaves first)
     DevBuf m2l_item_off, m2l_item_tgt, m2l_part;  // balanced M2L work items
     DevBuf scan_tmp;
@@ -160,7 +160,7 @@ struct Ctx {
     DevBuf t_start, t_count, t_split, t_lo, t_hi, t_son, t_depth, t_parent;
     DevBuf l_start, l_count, l_lo, l_hi, l_parent;
     DevBuf seg_chunk, chunk_part, chunk_rcnt, seg_nleft, seg_flag, seg_nn, seg_non;
-    DevBuf t_size, t_lsize, t_dfs, t_lfirst, t_final, l_final;
+    DevBuf t_size, t_lsize, t_dfs, t_lfirst, l_final;
     DevBuf err;                    // device error flag (int)
     DevBuf sizes;                  // DevSizes of the current evaluation
     DevBuf stats;                  // device counters (int64)
@@ -197,6 +197,10 @@ struct Ctx {
     } while (0)
 
 void *grow(DevBuf &b, size_t bytes);
+// device memory of the evaluations
+void *dev_alloc(size_t bytes);
+void dev_free(void *p);
+bool arena_active();  // a bump arena serves dev_alloc on this thread (exact sizes)
 // a buffer of the current evaluation: grown (ordinary) or checked against the plan
 // (a planned evaluation allocates nothing)
 void *ctx_buf(Ctx &c, DevBuf &b, size_t bytes, const char *what);

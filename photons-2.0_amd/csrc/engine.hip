@@ -31,15 +31,29 @@ void op_m2l(hipStream_t st, int order, int count, double rs, const double *R, co
 void op_l2p(hipStream_t st, int order, int n, const double *pos, const int32_t *owner, const double *center,
             const double *L, double *acc);
 
+bool arena_active() { return false; }
+
+void *dev_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    PH_CHECK(hipMalloc(&p, bytes));
+    return p;
+}
+
+void dev_free(void *p)
+{
+    if (p) PH_CHECK(hipFree(p));
+}
+
 void *grow(DevBuf &b, size_t bytes)
 {
     if (bytes == 0) bytes = 1;
     if (bytes <= b.cap) return b.p;
-    if (b.p) PH_CHECK(hipFree(b.p));
+    dev_free(b.p);
     b.p = nullptr;
     b.cap = 0;
-    size_t want = bytes + bytes / 8;
-    PH_CHECK(hipMalloc(&b.p, want));
+    const size_t want = arena_active() ? bytes : bytes + bytes / 8;
+    b.p = dev_alloc(want);
     b.cap = want;
     return b.p;
 }
@@ -63,6 +77,19 @@ __global__ void k_zero_words(uint32_t *__restrict__ p, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
+}
+
+// the buffers of one force evaluation
+static std::vector<DevBuf *> eval_bufs(Ctx &c)
+{
+    return {&c.px, &c.py, &c.pz, &c.pidx, &c.qx, &c.qy, &c.qz, &c.qidx, &c.ox, &c.oy, &c.oz, &c.geom, &c.info,
+            &c.level_nodes, &c.M, &c.L, &c.p2p_off, &c.p2p_src, &c.m2l_off, &c.m2l_src, &c.cnt_tmp, &c.scan_tmp,
+            &c.t_start, &c.t_count, &c.t_split, &c.t_lo, &c.t_hi, &c.t_son, &c.t_depth, &c.t_parent, &c.l_start,
+            &c.l_count, &c.l_lo, &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
+            &c.seg_flag, &c.seg_nn, &c.seg_non, &c.t_size, &c.t_lsize, &c.t_dfs, &c.t_lfirst, &c.l_final,
+            &c.walk_rec, &c.walk_trec, &c.walk_tiles, &c.walk_tgt, &c.dl_a, &c.dl_b, &c.dl_span, &c.pair_off, &c.pend_span,
+            &c.pair_cnt, &c.start_list, &c.big_pairs, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off, &c.d_box_len,
+            &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp};
 }
 
 static void release(DevBuf &b)
@@ -176,17 +203,9 @@ int photons_destroy(photons_ctx *x)
         PH_CHECK(hipStreamSynchronize(c.stream));
         if (c.graph_exec) (void)hipGraphExecDestroy(c.graph_exec);
         if (c.graph) (void)hipGraphDestroy(c.graph);
-        DevBuf *bufs[] = {&c.sizes, &c.px, &c.py, &c.pz, &c.pidx, &c.qx, &c.qy, &c.qz, &c.qidx, &c.ox, &c.oy, &c.oz,
-                          &c.acc, &c.geom, &c.info, &c.level_nodes, &c.M, &c.L, &c.p2p_off, &c.p2p_src,
-                          &c.m2l_off, &c.m2l_src, &c.cnt_tmp, &c.scan_tmp, &c.t_start, &c.t_count, &c.t_split,
-                          &c.t_lo, &c.t_hi, &c.t_son, &c.t_depth, &c.t_parent, &c.l_start, &c.l_count, &c.l_lo,
-                          &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
-                          &c.seg_flag, &c.seg_nn, &c.seg_non, &c.t_size, &c.t_lsize,
-                          &c.t_dfs, &c.t_lfirst, &c.t_final, &c.l_final, &c.err, &c.stats, &c.walk_rec,
-                          &c.walk_rgeom, &c.walk_tgt, &c.walk_tgt_off, &c.dl_a, &c.dl_b, &c.dl_span, &c.pair_off,
-                          &c.pend_span, &c.pair_cnt, &c.start_list, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part,
-                          &c.d_box_off, &c.d_box_len, &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp,
-                          &x->io_pos, &x->io_acc, &x->op_a, &x->op_b, &x->op_c, &x->op_d, &x->op_e};
+        for (DevBuf *b : eval_bufs(c)) release(*b);
+        DevBuf *bufs[] = {&c.sizes, &c.err, &c.stats, &x->io_pos, &x->io_acc, &x->op_a, &x->op_b, &x->op_c,
+                          &x->op_d, &x->op_e};
         for (DevBuf *b : bufs) release(*b);
         if (c.pinned.p) (void)hipHostFree(c.pinned.p);
         for (int i = 0; i < 16; i++) (void)hipEventDestroy(c.timing.ev[i]);

@@ -594,13 +594,12 @@ template <typename T>
 static T *regrow(DevBuf &b, size_t oldcap, size_t newcap, int width, hipStream_t s)
 {
     T *old = reinterpret_cast<T *>(b.p);
-    T *nw = nullptr;
-    PH_CHECK(hipMalloc(&nw, newcap * width * sizeof(T)));
+    T *nw = static_cast<T *>(dev_alloc(newcap * width * sizeof(T)));
     if (old) {
         for (int w = 0; w < width; w++)
             PH_CHECK(hipMemcpyAsync(nw + w * newcap, old + w * oldcap, oldcap * sizeof(T), hipMemcpyDeviceToDevice, s));
         PH_CHECK(hipStreamSynchronize(s));
-        PH_CHECK(hipFree(old));
+        dev_free(old);
     }
     b.p = nw;
     b.cap = newcap * width * sizeof(T);
@@ -823,7 +822,6 @@ void build_tree(Ctx &c)
         c.nlevel = nlevel;
         c.maxdepth = nlevel;  // leaves sit at most at depth nlevel
         c.level_off.assign(lvl.begin(), lvl.end());
-        c.leaf_depth_off.assign(ldo.begin(), ldo.end());
         const int nd = levels + 1;
         c.tgt_off.assign(nd + 1, 0);
         for (int d = 0; d < nd; d++)

@@ -75,10 +75,34 @@ enum : int {
     kActOpenB      // open the source
 };
 
+// by depth-first rank: what a traversal reads of a source cell, in one 64-byte record
+struct alignas(16) WalkRec {
+    CellGeom g;
+    int32_t cell, dsize;  // cell id; subtree size (1: a leaf)
+    int32_t kid[2];       // a node's children, first and second in depth-first order
+};
+static_assert(sizeof(WalkRec) == 64, "one 64-byte record per rank");
+
+// a target of the current depth and what the rule needs of it, written by k_walk_prep
+struct Target {
+    CellGeom g;
+    double L;              // its box length (shifts, the image prune's domain box)
+    int64_t pbeg;          // its pending entries: pend[pbeg ..], one per pair
+    int32_t X, rank;       // cell id, depth-first rank
+    int32_t son0r, son1r;  // ranks of the children (nodes)
+    int32_t leaf, pad;
+};
+
+// a traversal left to a wavefront: pair q of target j goes on at rank r (< rend, the end
+// of its source subtree) after np, nm, nd outputs
+struct Cont {
+    int32_t q, j, r, rend;
+    uint32_t np, nm, nd, pad;
+};
+
 struct WalkArgs {
     const CellInfo *info;
-    const int2 *rec;          // by depth-first rank: (cell, subtree size)
-    const CellGeom *rgeom;    // by depth-first rank
+    const WalkRec *rec;       // by depth-first rank
     DevSizes *sz;             // cell counts, the targets of each depth, list bases, overflow
     double theta, theta2, rc, rc2;
     BoxView bx;
@@ -86,97 +110,167 @@ struct WalkArgs {
     const int *walk_tgt;
     int d;
     int64_t tgt_cap;          // launch bound of the targets
+    Target *tgt;              // [tgt_cap]: the depth's targets (k_walk_prep)
     const int64_t *pair_off;  // [ntgt + 1]: first pair of each target (scan of the pending counts)
-    const int64_t *pspan;     // [2 ntgt]: each target's pending entries in `pend`
+    const int32_t *tile_tgt;  // per tile of kWalkThreads pairs: the target of its first pair
     int64_t npair_cap;        // pairs the per-pair arrays and the launch hold
     const int32_t *pend;      // pending entries: the previous depth's down lists (or the start list)
     uint32_t *pcnt;           // [3 npair]: outputs of each pair (P2P, M2L, down), count pass -> write pass
+    Cont *big;                // [npair]: the traversals continued by a wavefront each
     // outputs of the write pass
     int32_t *p2p_out, *m2l_out, *dl_out;  // list bases: sz->p2p_base, sz->m2l_base
     int64_t *p2p_span, *m2l_span, *dl_span;  // per leaf / cell / node (index cell - nleaf)
 };
 
-// the target of a pair and what the rule needs of it
-struct Target {
-    int X;
-    bool leaf;
-    int son0, son1;
-    CellGeom g;
-    double L;         // its box length (shifts, the image prune's domain box)
-};
+// Entries of the pending and down lists name the source by its depth-first rank (the
+// traversal starts there without a cell lookup); the P2P and M2L lists name cells.
+__device__ __forceinline__ int entry_rank(int32_t e) { return entry_cell(e); }
 
-__device__ __forceinline__ int decide(const WalkArgs &a, const Target &t, int v, int r, int sid, double sx,
-                                      double sy, double sz)
+__device__ __forceinline__ int decide(const WalkArgs &a, const Target &t, const WalkRec &R, int r, int sid,
+                                      double sx, double sy, double sz)
 {
-    const bool bleaf = v < a.sz->nleaf;
-    if (sid == 0 && v == t.X) return t.leaf ? kActP2P : kActSelfNode;
+    const bool bleaf = R.dsize == 1;
+    if (sid == 0 && r == t.rank) return t.leaf ? kActP2P : kActSelfNode;
     if (t.leaf && bleaf) return kActP2P;  // leaf-leaf pairs are always P2P
-    const CellGeom gB = a.rgeom[r];
-    const double d0 = t.g.c[0] - (gB.c[0] + sx);
-    const double d1 = t.g.c[1] - (gB.c[1] + sy);
-    const double d2 = t.g.c[2] - (gB.c[2] + sz);
-    const int flag = mac(t.g.w, gB.w, d0, d1, d2, a.theta2, a.rc2);
+    const double d0 = t.g.c[0] - (R.g.c[0] + sx);
+    const double d1 = t.g.c[1] - (R.g.c[1] + sy);
+    const double d2 = t.g.c[2] - (R.g.c[2] + sz);
+    const int flag = mac(t.g.w, R.g.w, d0, d1, d2, a.theta2, a.rc2);
     if (flag < 0) return kActNone;
     bool pruned = false;
     if (sid != 0 && !bleaf) {
         const double L = t.L;
         const CellGeom box = {{0.5 * L, 0.5 * L, 0.5 * L}, {L, L, L}};
-        pruned = let_pruned(box, gB, sx, sy, sz, a.rc, a.theta);
+        pruned = let_pruned(box, R.g, sx, sy, sz, a.rc, a.theta);
     }
     // accepted; in the image walk a target leaf against a pruned node takes M2L whatever
     // the acceptance says (src/remotes.c:442)
     if (flag == 1 || (t.leaf && pruned)) return kActM2L;
     if (t.leaf) return kActOpenB;
     if (bleaf) return kActOpenA;
-    const bool bigger = (t.g.w[0] + t.g.w[1] + t.g.w[2]) > (gB.w[0] + gB.w[1] + gB.w[2]);
+    const bool bigger = (t.g.w[0] + t.g.w[1] + t.g.w[2]) > (R.g.w[0] + R.g.w[1] + R.g.w[2]);
     return (bigger || pruned) ? kActOpenA : kActOpenB;  // ties open the source
-}
-
-// The pairs of target t with the source subtree of `entry`, in depth-first order.
-// WRITE = false counts the outputs; WRITE = true stores them from the given positions,
-// never more than the count pass found (lp, lm, ld), so a store stays inside the ranges
-// the scan gave this pair.
-template <bool WRITE>
-__device__ __forceinline__ void traverse(const WalkArgs &a, const Target &t, int32_t entry, uint32_t &np,
-                                         uint32_t &nm, uint32_t &nd, int64_t qp, int64_t qm, int64_t qd,
-                                         uint32_t lp = 0, uint32_t lm = 0, uint32_t ld = 0)
-{
-    const int B = entry_cell(entry), sid = entry_shift(entry);
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    if (sid != 0) shift_of(sid, t.L, sx, sy, sz);
-    const CellInfo IB = a.info[B];
-    int r = IB.dfs;
-    const int rend = IB.dfs + IB.dsize;
-    while (r < rend) {
-        const int2 cd = a.rec[r];
-        const int v = cd.x;
-        const int act = decide(a, t, v, r, sid, sx, sy, sz);
-        if (act == kActP2P) {
-            if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(v, sid);
-            np++;
-        } else if (act == kActM2L) {
-            if (WRITE && nm < lm) a.m2l_out[qm + nm] = pack_entry(v, sid);
-            nm++;
-        } else if (act == kActOpenA) {
-            if (WRITE && nd < ld) a.dl_out[qd + nd] = pack_entry(v, sid);
-            nd++;
-        } else if (act == kActSelfNode) {
-            if (WRITE && nd + 1 < ld) {
-                a.dl_out[qd + nd] = pack_entry(t.son0, 0);
-                a.dl_out[qd + nd + 1] = pack_entry(t.son1, 0);
-            }
-            nd += 2;
-        }
-        r += act == kActOpenB ? 1 : cd.y;
-    }
 }
 
 constexpr int kWalkThreads = 256;  // pairs per tile
 
-// the target of pair q: the last j with pair_off[j] <= q
-__device__ __forceinline__ int target_of(const int64_t *__restrict__ pair_off, int ntgt, int64_t q)
+// A thread applies the rule to at most kThreadVisits ranks of its pair; a traversal that
+// goes on (a target leaf beside a dense cluster opens one source node after another: one
+// pair in a few hundred, but a wavefront waits for its slowest lane) is continued from the
+// rank it reached by a whole wavefront, 64 ranks at a time (wave_traverse).
+constexpr int kThreadVisits = 16;
+
+// The pairs of target t with the source subtree at rank r (record R, already loaded), in
+// depth-first order, for at most kThreadVisits ranks; returns the rank it stopped at (>=
+// the subtree's end when it finished).  WRITE = false counts the outputs; WRITE = true
+// stores them from the given positions, never more than the count pass found (lp, lm,
+// ld), so a store stays inside the ranges the scan gave this pair.
+template <bool WRITE>
+__device__ __forceinline__ int traverse(const WalkArgs &a, const Target &t, int r, int sid, WalkRec R, int &rend,
+                                        uint32_t &np, uint32_t &nm, uint32_t &nd, int64_t qp, int64_t qm, int64_t qd,
+                                        uint32_t lp = 0, uint32_t lm = 0, uint32_t ld = 0)
 {
-    int lo = 0, hi = ntgt;  // pair_off[lo] <= q < pair_off[hi]
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (sid != 0) shift_of(sid, t.L, sx, sy, sz);
+    rend = r + R.dsize;
+    for (int visits = 1;; visits++) {
+        const int act = decide(a, t, R, r, sid, sx, sy, sz);
+        if (act == kActP2P) {
+            if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.cell, sid);
+            np++;
+        } else if (act == kActM2L) {
+            if (WRITE && nm < lm) a.m2l_out[qm + nm] = pack_entry(R.cell, sid);
+            nm++;
+        } else if (act == kActOpenA) {
+            if (WRITE && nd < ld) a.dl_out[qd + nd] = pack_entry(r, sid);
+            nd++;
+        } else if (act == kActSelfNode) {
+            if (WRITE && nd + 1 < ld) {
+                a.dl_out[qd + nd] = pack_entry(t.son0r, 0);
+                a.dl_out[qd + nd + 1] = pack_entry(t.son1r, 0);
+            }
+            nd += 2;
+        }
+        if (act == kActOpenB && t.leaf) {
+            // a target leaf against a leaf takes P2P whatever the geometry: the opened
+            // node's leaf children are emitted here, in rank order, without a visit
+            const int nleaf = a.sz->nleaf;
+            r += 1;
+            if (R.kid[0] < nleaf) {
+                if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.kid[0], sid);
+                np++;
+                r += 1;
+                if (R.kid[1] < nleaf) {
+                    if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.kid[1], sid);
+                    np++;
+                    r += 1;
+                }
+            }
+        } else {
+            r += act == kActOpenB ? 1 : R.dsize;
+        }
+        if (r >= rend || visits == kThreadVisits) return r;
+        R = a.rec[r];
+    }
+}
+
+// The rest of a traversal, ranks [r, rend), by a wavefront: lane l takes rank carry + l
+// and applies the rule; a rank is visited unless an earlier rank of the chunk settled a
+// subtree containing it (an exclusive prefix max of the ranks each lane accounts for: a
+// settled rank its whole subtree, an opened one itself; a rank inside a settled subtree
+// accounts for no more than that subtree, and every subtree settled before r ends by r);
+// the visited ranks' outputs are emitted in rank order, as the single-thread walk emits
+// them.  A pair's own target is only ever met at its first rank (self pair), which the
+// thread has taken, so no rank here passes two children down.  np, nm, nd: wave-uniform
+// counts; WRITE stores from qp, qm, qd, never past lp, lm, ld.
+template <bool WRITE>
+__device__ __forceinline__ void wave_traverse(const WalkArgs &a, const Target &t, int r, int rend, int sid, int lane,
+                                              uint32_t &np, uint32_t &nm, uint32_t &nd, int64_t qp = 0,
+                                              int64_t qm = 0, int64_t qd = 0, uint32_t lp = 0, uint32_t lm = 0,
+                                              uint32_t ld = 0)
+{
+    double sx = 0.0, sy = 0.0, sz = 0.0;
+    if (sid != 0) shift_of(sid, t.L, sx, sy, sz);
+    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    int carry = r;
+    while (carry < rend) {
+        const int s = carry + lane;
+        int act = kActNone, cov = s + 1, v = 0;
+        if (s < rend) {
+            const WalkRec R = a.rec[s];
+            v = R.cell;
+            act = decide(a, t, R, s, sid, sx, sy, sz);
+            cov = act == kActOpenB ? s + 1 : s + R.dsize;
+        }
+        int incl = cov;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int o = __shfl_up(incl, d, 64);
+            if (lane >= d) incl = max(incl, o);
+        }
+        int excl = __shfl_up(incl, 1, 64);
+        excl = lane == 0 ? carry : max(excl, carry);
+        const bool visited = s < rend && s >= excl;
+        const bool isp = visited && act == kActP2P, ism = visited && act == kActM2L;
+        const bool isd = visited && act == kActOpenA;
+        const uint64_t bp = __ballot(isp), bm = __ballot(ism), bd = __ballot(isd);
+        if (WRITE) {
+            const uint32_t ip = np + __popcll(bp & below), im = nm + __popcll(bm & below);
+            const uint32_t id = nd + __popcll(bd & below);
+            if (isp && ip < lp) a.p2p_out[qp + ip] = pack_entry(v, sid);
+            if (ism && im < lm) a.m2l_out[qm + im] = pack_entry(v, sid);
+            if (isd && id < ld) a.dl_out[qd + id] = pack_entry(s, sid);
+        }
+        np += __popcll(bp);
+        nm += __popcll(bm);
+        nd += __popcll(bd);
+        carry = max(carry + 64, __shfl(incl, 63, 64));
+    }
+}
+
+// the target of pair q: the last j in [lo, hi) with pair_off[j] <= q
+__device__ __forceinline__ int target_of(const int64_t *__restrict__ pair_off, int lo, int hi, int64_t q)
+{
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
         if (pair_off[mid] <= q) lo = mid;
@@ -190,21 +284,49 @@ __device__ __forceinline__ int target_of(const int64_t *__restrict__ pair_off, i
 __device__ __forceinline__ int ntgt_all(const WalkArgs &a) { return (int)(a.sz->tgt_off[a.d + 1] - a.sz->tgt_off[a.d]); }
 __device__ __forceinline__ int ntgt_of(const WalkArgs &a) { return (int)min((int64_t)ntgt_all(a), a.tgt_cap); }
 __device__ __forceinline__ const int *tgt_of(const WalkArgs &a) { return a.walk_tgt + a.sz->tgt_off[a.d]; }
+__device__ __forceinline__ int64_t npair_of(const WalkArgs &a, int ntgt) { return min(a.pair_off[ntgt], a.npair_cap); }
 
-__device__ __forceinline__ Target load_target(const WalkArgs &a, int j)
+// The first pair offsets of the targets from the tile's first target on, in LDS: a tile
+// of pairs rarely spans more than kWalkThreads targets.  Called by every thread of the
+// block (it synchronises); `live` says whether the tile holds pairs (block-uniform).
+__device__ __forceinline__ int tile_window(const WalkArgs &a, int ntgt, bool live, int64_t *s_off)
 {
-    Target t;
-    t.X = tgt_of(a)[j];
-    t.leaf = t.X < a.sz->nleaf;
-    const CellInfo I = a.info[t.X];
-    t.son0 = I.son0;
-    t.son1 = I.son1;
-    t.g = a.rgeom[I.dfs];
-    t.L = a.bx.len_of(t.X);
-    return t;
+    int j0 = 0;
+    if (live) {
+        j0 = a.tile_tgt[blockIdx.x];
+        for (int i = threadIdx.x; i <= kWalkThreads; i += kWalkThreads) s_off[i] = a.pair_off[min(j0 + i, ntgt)];
+    }
+    __syncthreads();
+    return j0;
 }
 
-// output counts of one tile of pairs (P2P, M2L, down entries), scanned together
+// the target of pair q of this tile: a search of the LDS window, else of the rest
+__device__ __forceinline__ int tile_target(const WalkArgs &a, const int64_t *s_off, int j0, int ntgt, int64_t q)
+{
+    if (q < s_off[kWalkThreads]) {
+        int lo = 0, hi = kWalkThreads;  // s_off[lo] <= q < s_off[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[mid] <= q) lo = mid;
+            else hi = mid;
+        }
+        return j0 + lo;
+    }
+    // pair_off[j0 + kWalkThreads] <= q < pair_off[ntgt]
+    return target_of(a.pair_off, j0 + kWalkThreads, ntgt, q);
+}
+
+// per tile, the target of its first pair: each target marks the tile starts in its range
+__global__ void k_tile_targets(WalkArgs a, int32_t *__restrict__ tile_tgt)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int ntgt = ntgt_of(a);
+    if (j >= ntgt) return;
+    const int64_t npair = npair_of(a, ntgt);
+    const int64_t lo = a.pair_off[j], hi = min(a.pair_off[j + 1], npair);
+    for (int64_t k = (lo + kWalkThreads - 1) / kWalkThreads; k * kWalkThreads < hi; k++) tile_tgt[k] = j;
+}
+
 struct Cnt3 {
     int64_t p, m, d;
 };
@@ -216,20 +338,41 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_count(WalkArgs a, Cnt3 *_
 {
     typedef hipcub::BlockReduce<int64_t, kWalkThreads> Reduce;
     __shared__ typename Reduce::TempStorage tmp;
+    __shared__ int64_t s_off[kWalkThreads + 1];
     const int ntgt = ntgt_of(a);
     const int64_t npair_all = a.pair_off[ntgt];
     const int64_t npair = min(npair_all, a.npair_cap);
     if (blockIdx.x == 0 && threadIdx.x == 0 && npair_all > a.npair_cap) a.sz->overflow |= kOvfPairs;
-    const int64_t q = (int64_t)blockIdx.x * kWalkThreads + threadIdx.x;
+    const int64_t q0 = (int64_t)blockIdx.x * kWalkThreads, q = q0 + threadIdx.x;
+    const int j0 = tile_window(a, ntgt, q0 < npair, s_off);
     uint32_t np = 0, nm = 0, nd = 0;
+    bool more = false;
+    Cont cont;
     if (q < npair) {
-        const int j = target_of(a.pair_off, ntgt, q);
-        const Target t = load_target(a, j);
-        const int32_t entry = a.pend[a.pspan[2 * j] + (q - a.pair_off[j])];
-        traverse<false>(a, t, entry, np, nm, nd, 0, 0, 0);
+        const int j = tile_target(a, s_off, j0, ntgt, q);
+        const Target t = a.tgt[j];
+        const int32_t entry = a.pend[t.pbeg + (q - a.pair_off[j])];
+        const int r = entry_rank(entry);
+        int rend;
+        const int stop = traverse<false>(a, t, r, entry_shift(entry), a.rec[r], rend, np, nm, nd, 0, 0, 0);
+        more = stop < rend;
+        if (more) cont = Cont{(int32_t)q, j, stop, rend, np, nm, nd, 0};
         a.pcnt[3 * q] = np;
         a.pcnt[3 * q + 1] = nm;
         a.pcnt[3 * q + 2] = nd;
+    }
+    {
+        // the continued traversals, counted by k_walk_deep_count (which adds them to this
+        // tile's sums): one append per wavefront
+        const uint64_t bal = __ballot(more);
+        if (bal) {
+            const int lane = threadIdx.x % 64;
+            unsigned long long base = 0;
+            if (lane == __ffsll((unsigned long long)bal) - 1)
+                base = atomicAdd(reinterpret_cast<unsigned long long *>(&a.sz->n_big), (unsigned long long)__popcll(bal));
+            base = __shfl(base, __ffsll((unsigned long long)bal) - 1, 64);
+            if (more) a.big[base + __popcll(bal & ((1ull << lane) - 1ull))] = cont;
+        }
     }
     const int64_t tp = Reduce(tmp).Sum((int64_t)np);
     __syncthreads();
@@ -246,10 +389,11 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const C
 {
     typedef hipcub::BlockScan<int64_t, kWalkThreads> Scan;
     __shared__ typename Scan::TempStorage tmp;
+    __shared__ int64_t s_off[kWalkThreads + 1];
     if (a.sz->overflow) return;  // the same for every thread: no block scan is left half-entered
     const int ntgt = ntgt_of(a);
-    const int64_t npair = min(a.pair_off[ntgt], a.npair_cap);
-    const int64_t q = (int64_t)blockIdx.x * kWalkThreads + threadIdx.x;
+    const int64_t npair = npair_of(a, ntgt);
+    const int64_t q0 = (int64_t)blockIdx.x * kWalkThreads, q = q0 + threadIdx.x;
     const bool live = q < npair;
     int64_t np = 0, nm = 0, nd = 0;
     if (live) {
@@ -263,16 +407,24 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const C
     Scan(tmp).ExclusiveSum(nm, om);
     __syncthreads();
     Scan(tmp).ExclusiveSum(nd, od);
+    const int j0 = tile_window(a, ntgt, q0 < npair, s_off);
     if (!live) return;
     const Cnt3 to = tile_off[blockIdx.x];
     const int64_t qp = a.sz->p2p_base + to.p + op, qm = a.sz->m2l_base + to.m + om, qd = to.d + od;
-    const int j = target_of(a.pair_off, ntgt, q);
-    const Target t = load_target(a, j);
-    const int32_t entry = a.pend[a.pspan[2 * j] + (q - a.pair_off[j])];
-    uint32_t cp = 0, cm = 0, cd = 0;
-    traverse<true>(a, t, entry, cp, cm, cd, qp, qm, qd, (uint32_t)np, (uint32_t)nm, (uint32_t)nd);
+    const int j = tile_target(a, s_off, j0, ntgt, q);
+    const int64_t pj = a.pair_off[j];
+    const Target t = a.tgt[j];
+    if (np + nm + nd > 0) {
+        // the same ranks as the count pass; a continued traversal's rest is k_walk_deep_write's
+        const int32_t entry = a.pend[t.pbeg + (q - pj)];
+        const int r = entry_rank(entry);
+        int rend;
+        uint32_t cp = 0, cm = 0, cd = 0;
+        traverse<true>(a, t, r, entry_shift(entry), a.rec[r], rend, cp, cm, cd, qp, qm, qd, (uint32_t)np,
+                       (uint32_t)nm, (uint32_t)nd);
+    }
     // the target's first pair opens its spans, its last pair closes them
-    const bool first = q == a.pair_off[j], last = q + 1 == a.pair_off[j + 1];
+    const bool first = q == pj, last = q + 1 == a.pair_off[j + 1];
     const int nleaf = a.sz->nleaf;
     if (first) {
         if (t.leaf) a.p2p_span[2 * (size_t)t.X] = qp;
@@ -286,16 +438,82 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const C
     }
 }
 
+// The continued traversals, one wavefront each (a fixed grid walking the list of them):
+// their counts into pcnt and, by atomic adds of integers (order-free), into their tiles'
+// sums.
+constexpr int kDeepWaves = 8192;
+
+__global__ void __launch_bounds__(256) k_walk_deep_count(WalkArgs a, Cnt3 *__restrict__ tile_cnt)
+{
+    const int lane = threadIdx.x % 64;
+    const int64_t nbig = a.sz->n_big;
+    for (int64_t k = (int64_t)blockIdx.x * 4 + threadIdx.x / 64; k < nbig; k += (int64_t)gridDim.x * 4) {
+        const Cont ct = a.big[k];
+        const int64_t q = ct.q;
+        const Target t = a.tgt[ct.j];
+        const int32_t entry = a.pend[t.pbeg + (q - a.pair_off[ct.j])];
+        uint32_t np = 0, nm = 0, nd = 0;
+        wave_traverse<false>(a, t, ct.r, ct.rend, entry_shift(entry), lane, np, nm, nd);
+        if (lane == 0) {
+            a.pcnt[3 * q] = ct.np + np;
+            a.pcnt[3 * q + 1] = ct.nm + nm;
+            a.pcnt[3 * q + 2] = ct.nd + nd;
+            unsigned long long *tc = reinterpret_cast<unsigned long long *>(tile_cnt + q / kWalkThreads);
+            atomicAdd(tc, (unsigned long long)np);
+            atomicAdd(tc + 1, (unsigned long long)nm);
+            atomicAdd(tc + 2, (unsigned long long)nd);
+        }
+    }
+}
+
+// The continued traversals' outputs: the pair's offsets (the tile's offset plus the
+// outputs of the tile's earlier pairs, summed by the wavefront) past what its thread
+// wrote.
+__global__ void __launch_bounds__(256) k_walk_deep_write(WalkArgs a, const Cnt3 *__restrict__ tile_off)
+{
+    if (a.sz->overflow) return;
+    const int lane = threadIdx.x % 64;
+    const int64_t nbig = a.sz->n_big;
+    for (int64_t k = (int64_t)blockIdx.x * 4 + threadIdx.x / 64; k < nbig; k += (int64_t)gridDim.x * 4) {
+        const Cont ct = a.big[k];
+        const int64_t q = ct.q;
+        const int64_t t0 = q / kWalkThreads * kWalkThreads;
+        int64_t sp = 0, sm = 0, sd = 0;
+        for (int64_t i = t0 + lane; i < q; i += 64) {
+            sp += a.pcnt[3 * i];
+            sm += a.pcnt[3 * i + 1];
+            sd += a.pcnt[3 * i + 2];
+        }
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            sp += __shfl_xor(sp, o, 64);
+            sm += __shfl_xor(sm, o, 64);
+            sd += __shfl_xor(sd, o, 64);
+        }
+        const Cnt3 to = tile_off[q / kWalkThreads];
+        const int64_t qp = a.sz->p2p_base + to.p + sp + ct.np, qm = a.sz->m2l_base + to.m + sm + ct.nm;
+        const int64_t qd = to.d + sd + ct.nd;
+        const Target t = a.tgt[ct.j];
+        const int32_t entry = a.pend[t.pbeg + (q - a.pair_off[ct.j])];
+        uint32_t np = 0, nm = 0, nd = 0;
+        wave_traverse<true>(a, t, ct.r, ct.rend, entry_shift(entry), lane, np, nm, nd, qp, qm, qd,
+                            a.pcnt[3 * q] - ct.np, a.pcnt[3 * q + 1] - ct.nm, a.pcnt[3 * q + 2] - ct.nd);
+    }
+}
+
 // Pending entries of each target of a depth: its parent's down list, or at the start
 // depth the start list (start_target < 0: target j is the root of box j and takes
-// entries j * nstart .. ; else start_target alone takes all nstart entries).  Targets
-// without pending entries get empty spans here (the write pass sets the others).
-__global__ void k_walk_prep(WalkArgs a, int start, int start_target, int nstart, int64_t *__restrict__ cnt,
-                            int64_t *__restrict__ pspan)
+// entries j * nstart .. ; else start_target alone takes all nstart entries); and the
+// target's record.  Targets without pending entries get empty spans here (the write
+// pass sets the others).
+__global__ void k_walk_prep(WalkArgs a, int start, int start_target, int nstart, int64_t *__restrict__ cnt)
 {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     const int ntgt = ntgt_of(a);
-    if (j == 0 && ntgt_all(a) > a.tgt_cap) a.sz->overflow |= kOvfTargets;
+    if (j == 0) {
+        a.sz->n_big = 0;  // the count pass lists this depth's continued traversals
+        if (ntgt_all(a) > a.tgt_cap) a.sz->overflow |= kOvfTargets;
+    }
     if (j > a.tgt_cap) return;
     if (j >= ntgt || a.sz->overflow) {
         cnt[j] = 0;  // zeros up to the bound: the scan's entry ntgt is the total
@@ -303,6 +521,7 @@ __global__ void k_walk_prep(WalkArgs a, int start, int start_target, int nstart,
     }
     const int X = tgt_of(a)[j];
     const int nleaf = a.sz->nleaf;
+    const CellInfo I = a.info[X];
     int64_t b = 0, e = 0;
     if (start) {
         if (start_target < 0) {
@@ -312,12 +531,20 @@ __global__ void k_walk_prep(WalkArgs a, int start, int start_target, int nstart,
             e = nstart;
         }
     } else {
-        const int par = a.info[X].parent;
-        b = a.dl_span[2 * (size_t)(par - nleaf)];
-        e = a.dl_span[2 * (size_t)(par - nleaf) + 1];
+        b = a.dl_span[2 * (size_t)(I.parent - nleaf)];
+        e = a.dl_span[2 * (size_t)(I.parent - nleaf) + 1];
     }
-    pspan[2 * j] = b;
-    pspan[2 * j + 1] = e;
+    Target t;
+    t.g = a.rec[I.dfs].g;
+    t.L = a.bx.len_of(X);
+    t.pbeg = b;
+    t.X = X;
+    t.rank = I.dfs;
+    t.leaf = X < nleaf;
+    t.son0r = t.leaf ? -1 : a.info[I.son0].dfs;
+    t.son1r = t.leaf ? -1 : a.info[I.son1].dfs;
+    t.pad = 0;
+    a.tgt[j] = t;
     cnt[j] = e - b;
     if (X < nleaf) {
         a.p2p_span[2 * (size_t)X] = a.p2p_span[2 * (size_t)X + 1] = 0;
@@ -368,15 +595,24 @@ __global__ void k_walk_end(DevSizes *sz)
     if (sz->dl_total > 0) sz->overflow |= kOvfDepths;
 }
 
-// cells by depth-first rank: (cell, subtree size) and geometry, read by the traversals
+// cells by depth-first rank: geometry, cell id and subtree size, read by the traversals
 __global__ void k_walk_records(const DevSizes *__restrict__ sz, const CellInfo *__restrict__ info,
-                               const CellGeom *__restrict__ geom, int2 *__restrict__ rec, CellGeom *__restrict__ rgeom)
+                               const CellGeom *__restrict__ geom, WalkRec *__restrict__ rec)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (sz->overflow || c >= sz->ncell) return;
     const CellInfo I = info[c];
-    rec[I.dfs] = make_int2(c, I.dsize);
-    rgeom[I.dfs] = geom[c];
+    WalkRec R;
+    R.g = geom[c];
+    R.cell = c;
+    R.dsize = I.dsize;
+    R.kid[0] = R.kid[1] = -1;
+    if (I.son0 >= 0) {
+        const bool first0 = info[I.son0].dfs == I.dfs + 1;
+        R.kid[0] = first0 ? I.son0 : I.son1;
+        R.kid[1] = first0 ? I.son1 : I.son0;
+    }
+    rec[I.dfs] = R;
 }
 
 // every span empty (cells a walk from a given pair never reaches keep no entries); n is
@@ -388,14 +624,14 @@ __global__ void k_clear_spans(int n, int64_t *__restrict__ span)
 }
 
 // start list: (root of box b, shift) for every box and shift, or (source, shift)
-__global__ void k_start_list(int32_t *f, const int *__restrict__ box_root, int nbox, int B, int shift_lo,
-                             int shift_hi)
+__global__ void k_start_list(int32_t *f, const CellInfo *__restrict__ info, const int *__restrict__ box_root, int nbox,
+                             int B, int shift_lo, int shift_hi)
 {
     const int nshift = shift_hi - shift_lo;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (box_root ? nbox : 1) * nshift) return;
     const int s = shift_lo + i % nshift;
-    f[i] = pack_entry(box_root ? box_root[i / nshift] : B, s);
+    f[i] = pack_entry(info[box_root ? box_root[i / nshift] : B].dfs, s);  // pending entries name ranks
 }
 
 // ---------------------------------------------------------------------------- host side
@@ -404,12 +640,11 @@ __global__ void k_start_list(int32_t *f, const int *__restrict__ box_root, int n
 static void *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
 {
     if (need <= b.cap) return b.p;
-    size_t want = need + need / 2;
-    void *nw = nullptr;
-    PH_CHECK(hipMalloc(&nw, want));
+    const size_t want = arena_active() ? need : need + need / 2;
+    void *nw = dev_alloc(want);
     if (b.p && keep) PH_CHECK(hipMemcpyAsync(nw, b.p, keep, hipMemcpyDeviceToDevice, st));
     PH_CHECK(hipStreamSynchronize(st));
-    if (b.p) PH_CHECK(hipFree(b.p));
+    dev_free(b.p);
     b.p = nw;
     b.cap = want;
     return nw;
@@ -442,11 +677,9 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     a.rc2 = c.prm.rc * c.prm.rc;
     a.bx = box_view(c);
     a.walk_tgt = dptr<int>(c.walk_tgt);
-    int2 *rec = static_cast<int2 *>(ctx_buf(c, c.walk_rec, sizeof(int2) * (size_t)ncb, "walk records"));
-    CellGeom *rgeom = static_cast<CellGeom *>(ctx_buf(c, c.walk_rgeom, sizeof(CellGeom) * (size_t)ncb, "walk records"));
-    k_walk_records<<<(ncb + 255) / 256, 256, 0, st>>>(sz, dptr<CellInfo>(c.info), dptr<CellGeom>(c.geom), rec, rgeom);
+    WalkRec *rec = static_cast<WalkRec *>(ctx_buf(c, c.walk_rec, sizeof(WalkRec) * (size_t)ncb, "walk records"));
+    k_walk_records<<<(ncb + 255) / 256, 256, 0, st>>>(sz, dptr<CellInfo>(c.info), dptr<CellGeom>(c.geom), rec);
     a.rec = rec;
-    a.rgeom = rgeom;
     a.p2p_span = static_cast<int64_t *>(ctx_buf(c, c.p2p_off, sizeof(int64_t) * 2 * (size_t)std::max(1, nlb), "P2P list"));
     a.m2l_span = static_cast<int64_t *>(ctx_buf(c, c.m2l_off, sizeof(int64_t) * 2 * (size_t)ncb, "M2L list"));
     a.dl_span = static_cast<int64_t *>(ctx_buf(c, c.dl_span, sizeof(int64_t) * 2 * (size_t)std::max(1, nnb), "down lists"));
@@ -460,7 +693,8 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     if (forest) need(c.d_box_root, sizeof(int) * (size_t)c.nbox, "box roots");
     int32_t *slist = static_cast<int32_t *>(ctx_buf(c, c.start_list, sizeof(int32_t) * (size_t)std::max<int64_t>(n0, 1),
                                                     "walk start"));
-    k_start_list<<<(int)((n0 + 255) / 256), 256, 0, st>>>(slist, forest ? dptr<int>(c.d_box_root) : nullptr, c.nbox,
+    k_start_list<<<(int)((n0 + 255) / 256), 256, 0, st>>>(slist, dptr<CellInfo>(c.info),
+                                                          forest ? dptr<int>(c.d_box_root) : nullptr, c.nbox,
                                                           start_source, shift_lo, shift_hi);
     int d0 = 0;
     if (!forest) {
@@ -484,12 +718,11 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         a.tgt_cap = tcap;
         a.pend = pend;
         int64_t *poff = static_cast<int64_t *>(ctx_buf(c, c.pair_off, sizeof(int64_t) * (size_t)(tcap + 1), "walk"));
-        int64_t *cnt = static_cast<int64_t *>(ctx_buf(c, c.pend_span, sizeof(int64_t) * 3 * (size_t)(tcap + 1), "walk"));
-        int64_t *pspan = cnt + (tcap + 1);
+        if (pcap >= ((int64_t)1 << 31)) throw std::runtime_error("more than 2^31 walk pairs at one depth");
+        int64_t *cnt = static_cast<int64_t *>(ctx_buf(c, c.pend_span, sizeof(int64_t) * (size_t)(tcap + 1), "walk"));
         a.pair_off = poff;
-        a.pspan = pspan;
-        k_walk_prep<<<(int)((tcap + 1 + 255) / 256), 256, 0, st>>>(a, d == d0, forest ? -1 : start_target, nshift, cnt,
-                                                                   pspan);
+        a.tgt = static_cast<Target *>(ctx_buf(c, c.walk_trec, sizeof(Target) * (size_t)std::max<int64_t>(tcap, 1), "walk"));
+        k_walk_prep<<<(int)((tcap + 1 + 255) / 256), 256, 0, st>>>(a, d == d0, forest ? -1 : start_target, nshift, cnt);
         {
             size_t bytes = 0;
             PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, poff, (int)(tcap + 1), st));
@@ -500,9 +733,16 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         a.pcnt = static_cast<uint32_t *>(ctx_buf(c, c.pair_cnt, sizeof(uint32_t) * 3 * (size_t)std::max<int64_t>(pcap, 1),
                                                  "walk pairs"));
         a.npair_cap = pcap;
+        a.big = static_cast<Cont *>(ctx_buf(c, c.big_pairs, sizeof(Cont) * (size_t)std::max<int64_t>(pcap, 1),
+                                            "walk pairs"));
+        int32_t *tile_tgt = static_cast<int32_t *>(ctx_buf(c, c.walk_tiles, sizeof(int32_t) * (size_t)ntile, "walk tiles"));
+        a.tile_tgt = tile_tgt;
+        k_tile_targets<<<(int)((std::max<int64_t>(tcap, 1) + 255) / 256), 256, 0, st>>>(a, tile_tgt);
         Cnt3 *tile_cnt = static_cast<Cnt3 *>(ctx_buf(c, c.cnt_tmp, sizeof(Cnt3) * 2 * (size_t)(ntile + 1), "walk tiles"));
         Cnt3 *tile_off = tile_cnt + (ntile + 1);
         k_walk_count<<<ntile, kWalkThreads, 0, st>>>(a, tile_cnt, ntile);
+        const int deep_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(kDeepWaves / 4, (pcap + 3) / 4));
+        k_walk_deep_count<<<deep_blocks, 256, 0, st>>>(a, tile_cnt);
         {
             size_t bytes = 0;
             PH_CHECK(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, tile_cnt, tile_off, AddCnt3(), Cnt3{0, 0, 0},
@@ -549,6 +789,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         a.dl_out = dptr<int32_t>(*dl_out);
         k_walk_step<<<1, 1, 0, st>>>(sz, tile_off + ntile, p2p_cap, m2l_cap, dl_cap, next_pairs);
         k_walk_write<<<ntile, kWalkThreads, 0, st>>>(a, tile_off);
+        k_walk_deep_write<<<deep_blocks, 256, 0, st>>>(a, tile_off);
         pend = a.dl_out;
         std::swap(dl_out, dl_other);
     }
