@@ -101,6 +101,15 @@ int photons_force_boxes_host(photons_ctx *ctx, int nbox, const int64_t *counts, 
                              const double *pos, double *acc);
 int photons_force_boxes_device(photons_ctx *ctx, int nbox, const int64_t *counts, const double *lengths,
                                const double *d_pos, double *d_acc);
+/* photons_force_device without any allocation, for callers that cannot allocate
+ * (tight serving loops, code that must not call hipMalloc).  No counterpart in the
+ * reference.  One box of n <= 32768 particles (else an error): every buffer of the
+ * evaluation comes from a static 96 MiB device array that the library holds on each
+ * GPU it runs on until the process ends; the context's own buffers and its last
+ * ordinary evaluation are left as they were.  Returns when d_acc is written.  All
+ * calls on one GPU share the array: calls that overlap in time on one GPU (from other
+ * threads) race.                                                              */
+int photons_force_device_static(photons_ctx *ctx, int64_t n, const double *d_pos, double *d_acc);
 
 /* Tuning (no counterpart in the reference, whose MAXLEAF is a compile-time
  * constant, inc/photoNs.h).  The FP32-mixed P2P kernel stages 256 or 512 source

@@ -197,10 +197,11 @@ struct Ctx {
     } while (0)
 
 void *grow(DevBuf &b, size_t bytes);
-// device memory of the evaluations
+// Device memory of the evaluations: hipMalloc, or, inside photons_force_device_static, a
+// piece of the library's static scratch array (dev_free of such a piece does nothing).
 void *dev_alloc(size_t bytes);
 void dev_free(void *p);
-bool arena_active();  // a bump arena serves dev_alloc on this thread (exact sizes)
+bool arena_active();  // the static scratch serves dev_alloc on this thread
 // a buffer of the current evaluation: grown (ordinary) or checked against the plan
 // (a planned evaluation allocates nothing)
 void *ctx_buf(Ctx &c, DevBuf &b, size_t bytes, const char *what);

@@ -14,6 +14,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 
@@ -31,10 +32,53 @@ void op_m2l(hipStream_t st, int order, int count, double rs, const double *R, co
 void op_l2p(hipStream_t st, int order, int n, const double *pos, const int32_t *owner, const double *center,
             const double *L, double *acc);
 
-bool arena_active() { return false; }
+// ---- the static scratch of photons_force_device_static: a file-scope device array of
+// kScratchBytes on every GPU the library runs on, held until the process ends.  Its
+// address on a device comes from hipGetSymbolAddress (per device, at context creation);
+// the array's name is not a device address in host code.
+constexpr size_t kScratchBytes = (size_t)96 << 20;
+constexpr int64_t kStaticMaxParticles = 32768;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static char *g_scratch_ptr[kMaxDevices];
+
+static char *scratch_on(int device)
+{
+    if (device < 0 || device >= kMaxDevices) throw std::runtime_error("device id out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        int cur = 0;
+        PH_CHECK(hipGetDevice(&cur));
+        PH_CHECK(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        PH_CHECK(hipSetDevice(cur));
+        PH_CHECK(e);
+        g_scratch_ptr[device] = static_cast<char *>(p);
+    }
+    return g_scratch_ptr[device];
+}
+
+// bump allocation over the scratch during one static evaluation (this thread only)
+struct Arena {
+    char *base;
+    size_t cap, used;
+};
+static thread_local Arena *g_arena = nullptr;
+
+bool arena_active() { return g_arena != nullptr; }
 
 void *dev_alloc(size_t bytes)
 {
+    if (g_arena) {
+        const size_t off = (g_arena->used + 255) & ~(size_t)255;
+        if (off + bytes > g_arena->cap)
+            throw std::runtime_error("the evaluation needs more than the " + std::to_string(kScratchBytes >> 20) +
+                                     " MiB static scratch");
+        g_arena->used = off + bytes;
+        return g_arena->base + off;
+    }
     void *p = nullptr;
     PH_CHECK(hipMalloc(&p, bytes));
     return p;
@@ -42,7 +86,7 @@ void *dev_alloc(size_t bytes)
 
 void dev_free(void *p)
 {
-    if (p) PH_CHECK(hipFree(p));
+    if (p && !g_arena) PH_CHECK(hipFree(p));
 }
 
 void *grow(DevBuf &b, size_t bytes)
@@ -79,7 +123,8 @@ __global__ void k_zero_words(uint32_t *__restrict__ p, size_t n)
     if (i < n) p[i] = 0u;
 }
 
-// the buffers of one force evaluation
+// the buffers of one force evaluation (what photons_force_device_static takes from the
+// static scratch instead)
 static std::vector<DevBuf *> eval_bufs(Ctx &c)
 {
     return {&c.px, &c.py, &c.pz, &c.pidx, &c.qx, &c.qy, &c.qz, &c.qidx, &c.ox, &c.oy, &c.oz, &c.geom, &c.info,
@@ -181,6 +226,7 @@ int photons_create(photons_ctx **out, const photons_params *prm, int device, voi
             x->c.pinned.cap = 4096;
             grow(x->c.err, sizeof(int));
             grow(x->c.sizes, sizeof(DevSizes));
+            scratch_on(device);  // the static scratch's address on this device, once
             for (int i = 0; i < 16; i++) PH_CHECK(hipEventCreate(&x->c.timing.ev[i]));
             PH_CHECK(hipEventCreateWithFlags(&x->c.counts_ready, hipEventDisableTiming));
         } catch (...) {
@@ -377,6 +423,53 @@ int photons_force_boxes_device(photons_ctx *x, int nbox, const int64_t *counts, 
         return fail("photons_force_boxes_device", e);
     }
 }
+// The force of one box of n <= kStaticMaxParticles particles without allocating: every
+// buffer of the evaluation comes from the static scratch, the context's own buffers stand
+// aside meanwhile (and keep their contents).  Synchronous; two calls that overlap on one
+// GPU (other threads) share the scratch and race.
+int photons_force_device_static(photons_ctx *x, int64_t n, const double *d_pos, double *d_acc)
+{
+    try {
+        Ctx &c = x->c;
+        if (c.graph_ready) return fail_msg(kGraphHeld);
+        std::string m = check_n(c, n);
+        if (!m.empty()) return fail_msg(m);
+        if (n > kStaticMaxParticles)
+            return fail_msg("photons_force_device_static: at most " + std::to_string(kStaticMaxParticles) +
+                            " particles (the " + std::to_string(kScratchBytes >> 20) + " MiB static scratch)");
+        if (n == 0) return 0;  // nothing to compute, nothing written
+        PH_CHECK(hipSetDevice(c.device));
+        Arena arena{scratch_on(c.device), kScratchBytes, 0};
+        // the context as it was (buffers, counts, lists): restored afterwards, so its last
+        // ordinary evaluation stays inspectable
+        const Ctx kept = c;
+        c.n = n;
+        for (DevBuf *b : eval_bufs(c)) *b = DevBuf();
+        c.tcap = c.lcap = 0;
+        g_arena = &arena;
+        try {
+            single_box(c, n);
+            prepare(c, d_pos);
+            evaluate(c, d_acc, true);
+            PH_CHECK(hipStreamSynchronize(c.stream));
+            collect_timing(c);
+        } catch (...) {
+            g_arena = nullptr;
+            (void)hipStreamSynchronize(c.stream);
+            c = kept;
+            throw;
+        }
+        g_arena = nullptr;
+        double ph[16];
+        std::memcpy(ph, c.phase_ms, sizeof ph);
+        c = kept;
+        std::memcpy(c.phase_ms, ph, sizeof ph);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail("photons_force_device_static", e);
+    }
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------- planned evaluation, hipGraph

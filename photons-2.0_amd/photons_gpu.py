@@ -66,6 +66,7 @@ def lib():
         L.photons_get_p2p_stage.argtypes = [vp]
         L.photons_autotune_boxes.argtypes = [vp, C.c_int, i64p, dp, vp, vp, C.c_int, C.POINTER(C.c_int),
                                              C.POINTER(C.c_int), dp]
+        L.photons_force_device_static.argtypes = [vp, C.c_int64, vp, vp]
         L.photons_graph_create.argtypes = [vp, C.c_int, i64p, dp, vp, vp, C.c_double, C.c_int]
         L.photons_graph_launch.argtypes = [vp]
         L.photons_graph_check.argtypes = [vp, C.POINTER(C.c_int), dp]
