@@ -72,13 +72,11 @@ struct DevSizes {
     int n_items;                      // M2L work items
     int pad[2];
     int64_t n_p2p, n_m2l;             // list lengths
-    int64_t npair;                    // walk pairs of the current depth
+    int64_t nrec;                     // walk records of the current depth (its pairs: twice as many)
     int64_t p2p_base, m2l_base;       // list entries written by the walk's earlier depths
-    int64_t dl_total;                 // down-list entries written by the walk's current depth
-    int64_t n_big;                    // walk traversals of the current depth continued by a wavefront
+    int64_t dl_total;                 // down records written by the walk's current depth
     int lvl[kMaxLevels + 1];          // first temporary node of depth d
     int ldo[kMaxLevels + 1];          // first temporary leaf of depth d
-    int64_t tgt_off[kMaxLevels + 1];  // first walk target of depth d (walk_tgt)
 };
 
 // per-box data the kernels read: box length by cell
@@ -99,9 +97,7 @@ struct Plan {
     int depths = 0;        // walk depths (0 .. depths-1)
     int scap = 0;          // nodes of one level
     int tcap = 0, lcap = 0;  // temporary nodes, leaves
-    int64_t tgt_cap = 0;   // walk targets of one depth
-    int64_t pair_cap = 0;  // walk pairs of one depth
-    int64_t dl_cap = 0;    // down-list entries of one depth
+    int64_t dl_cap = 0;    // walk records (down records) of one depth
     int64_t p2p_cap = 0, m2l_cap = 0;  // list entries
     int item_cap = 0;      // M2L work items
 };
@@ -109,7 +105,7 @@ struct Plan {
 // What an ordinary evaluation measured, for planning: the largest count of each kind
 struct Measure {
     int levels = 0, scap = 0, ntmp = 0, nleaf = 0, depths = 0, items = 0;
-    int64_t tgt = 0, pairs = 0, dl = 0, p2p = 0, m2l = 0;
+    int64_t dl = 0, p2p = 0, m2l = 0;
 };
 
 struct Timing {
@@ -145,13 +141,10 @@ struct Ctx {
     // lists: entries grouped by target; p2p_off / m2l_off hold one span (begin, end) of
     // entries per target leaf / cell: off[2 t] .. off[2 t + 1]
     DevBuf p2p_off, p2p_src, m2l_off, m2l_src, cnt_tmp;
-    // walk: the cells by depth-first rank (geometry, id, subtree size, children), the
-    // targets grouped by depth, each node's list of sources passed down to its two children
-    // (ping-pong by depth) with its span, and per-depth work arrays (target records, the
-    // target of each tile's first pair, pair counts, continued traversals)
-    DevBuf walk_rec, walk_trec, walk_tiles, walk_tgt, dl_a, dl_b, dl_span;
-    DevBuf pair_off, pend_span, pair_cnt, start_list, big_pairs;
-    std::vector<int64_t> tgt_off;             // walk targets of depth d: walk_tgt[tgt_off[d] .. tgt_off[d+1])
+    // walk: the cells by depth-first rank as sources and by cell as targets, the down
+    // records of a depth (ping-pong), each pair's outputs, the continued traversals
+    DevBuf walk_rec, walk_trec, dl_a, dl_b, pair_cnt, big_pairs, walk_tlist, walk_tcount, walk_qoff, walk_ccount,
+        walk_cidx;
     DevBuf p2p_order, p2p_order_tmp;          // P2P launch order (heaviest target leaves first)
     DevBuf m2l_item_off, m2l_item_tgt, m2l_part;  // balanced M2L work items
     DevBuf scan_tmp;

@@ -132,8 +132,7 @@ static std::vector<DevBuf *> eval_bufs(Ctx &c)
             &c.t_start, &c.t_count, &c.t_split, &c.t_lo, &c.t_hi, &c.t_son, &c.t_depth, &c.t_parent, &c.l_start,
             &c.l_count, &c.l_lo, &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
             &c.seg_flag, &c.seg_nn, &c.seg_non, &c.t_size, &c.t_lsize, &c.t_dfs, &c.t_lfirst, &c.l_final,
-            &c.walk_rec, &c.walk_trec, &c.walk_tiles, &c.walk_tgt, &c.dl_a, &c.dl_b, &c.dl_span, &c.pair_off, &c.pend_span,
-            &c.pair_cnt, &c.start_list, &c.big_pairs, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off, &c.d_box_len,
+            &c.walk_rec, &c.walk_trec, &c.dl_a, &c.dl_b, &c.pair_cnt, &c.big_pairs, &c.walk_tlist, &c.walk_tcount, &c.walk_qoff, &c.walk_ccount, &c.walk_cidx, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off, &c.d_box_len,
             &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp};
 }
 
@@ -487,8 +486,6 @@ static void plan_from_measure(Ctx &c, double margin)
     p.scap = (int)up(m.scap, 64);
     p.tcap = (int)up(m.ntmp, 1024);
     p.lcap = (int)up(m.nleaf, 1024);
-    p.tgt_cap = up(m.tgt, 64);
-    p.pair_cap = up(m.pairs, 1024);
     p.dl_cap = up(m.dl, 1024);
     p.p2p_cap = up(m.p2p, 4096);
     p.m2l_cap = up(m.m2l, 4096);

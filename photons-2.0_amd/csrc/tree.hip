@@ -376,9 +376,6 @@ __global__ void k_tree_sizes(DevSizes *sz, int levels)
     int nlevel = 0;
     while (nlevel < levels && sz->lvl[nlevel + 1] > sz->lvl[nlevel]) nlevel++;
     sz->nlevel = nlevel;
-    sz->tgt_off[0] = 0;
-    for (int d = 0; d <= levels; d++)
-        sz->tgt_off[d + 1] = sz->tgt_off[d] + (sz->lvl[d + 1] - sz->lvl[d]) + (sz->ldo[d + 1] - sz->ldo[d]);
 }
 
 // The levels scatter alternately into the two particle buffers and never copy back:
@@ -554,29 +551,6 @@ __global__ void k_init_particles(int64_t n, const double *__restrict__ pos, Part
     p.y[i] = pos[3 * i + 1];
     p.z[i] = pos[3 * i + 2];
     p.idx[i] = (int)i;
-}
-
-// Targets of the dual-tree walk grouped by depth (the walk runs one depth at a time):
-// depth d holds its nodes (level order, = depth-first order within the level) and then
-// its leaves (creation order, the same).  lv[d]: first temporary node of depth d;
-// ld[d]: first temporary leaf of depth d; toff[d]: first target of depth d.
-__global__ void k_walk_targets(const DevSizes *__restrict__ sz, const int *__restrict__ t_depth,
-                               const int *__restrict__ l_parent, const int *__restrict__ level_nodes,
-                               const int *__restrict__ l_final, int *__restrict__ tgt)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (sz->overflow) return;
-    const int ntmp = sz->nnode, ntl = sz->nleaf;
-    const int *lv = sz->lvl, *ld = sz->ldo;
-    const int64_t *toff = sz->tgt_off;
-    if (i < ntmp) {
-        const int d = depth_of(t_depth[i]);
-        tgt[toff[d] + (i - lv[d])] = level_nodes[i];
-    } else if (i < ntmp + ntl) {
-        const int l = i - ntmp;
-        const int d = depth_of(t_depth[l_parent[l]]) + 1;
-        tgt[toff[d] + (lv[d + 1] - lv[d]) + (l - ld[d])] = l_final[l];
-    }
 }
 
 // ---------------------------------------------------------------------------- host side
@@ -806,13 +780,10 @@ void build_tree(Ctx &c)
                                                        box_off, nbox, cell_box);
     k_emit_leaves<<<blocks(nleaf_b, 256), 256, 0, st>>>(sz, l_final, l_dfs, dfs, lfirst, T, geom, info, box_off, nbox,
                                                         cell_box);
-    // walk targets by depth: depths 0 .. levels (the deepest holds leaves only)
-    int *tgt = tree_buf<int>(c, c.walk_tgt, ncell_b, "walk targets");
-    k_walk_targets<<<blocks(ncell_b, 256), 256, 0, st>>>(sz, T.t_depth, T.l_parent, level_nodes, l_final, tgt);
     PH_CHECK(hipGetLastError());
 
     if (!planned) {
-        // host copies: cell counts, level ranges, walk targets per depth
+        // host copies: cell counts, level ranges
         const int nleaf = ldo[levels + 1], ntmp = lvl[levels];
         c.nleaf = nleaf;
         c.nnode = ntmp;
@@ -822,18 +793,12 @@ void build_tree(Ctx &c)
         c.nlevel = nlevel;
         c.maxdepth = nlevel;  // leaves sit at most at depth nlevel
         c.level_off.assign(lvl.begin(), lvl.end());
-        const int nd = levels + 1;
-        c.tgt_off.assign(nd + 1, 0);
-        for (int d = 0; d < nd; d++)
-            c.tgt_off[d + 1] = c.tgt_off[d] + (lvl[d + 1] - lvl[d]) + (ldo[d + 1] - ldo[d]);
         Measure &m = c.meas;
         m.levels = levels;
         m.ntmp = ntmp;
         m.nleaf = nleaf;
         m.scap = 0;
-        m.tgt = 0;
         for (int d = 0; d < levels; d++) m.scap = std::max(m.scap, lvl[d + 1] - lvl[d]);
-        for (int d = 0; d < nd; d++) m.tgt = std::max<int64_t>(m.tgt, c.tgt_off[d + 1] - c.tgt_off[d]);
     }
 }
 

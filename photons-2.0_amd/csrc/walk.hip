@@ -9,16 +9,17 @@
 //
 // MI355X formulation: one step per depth of the target tree.  A pair (A, B) whose
 // rule opens the target A passes the source B on to both children of A, and a node
-// paired with itself passes its two children; so each node keeps one list of sources
-// for its children (its down list), and the pairs whose target is a cell X at depth d
-// are X against the down list of X's parent.  At depth d every such pair (one thread
-// each) walks the source subtree of its entry in depth-first order without a stack:
-// at source rank r the rule either descends (opens the source: r + 1) or settles the
-// whole subtree (P2P, M2L, pass down or nothing: r + subtree size).  Two passes per
-// depth (count, then write at scanned offsets) and no atomics.  The pairs of one target
-// are contiguous and its sources come in (shift, depth-first rank) order, so every
-// list is written grouped by target in the reference's order: no sort.
-//
+// paired with itself passes its two children; so each node writes one down record per
+// source for its children, and the pairs of a depth are the previous depth's down
+// records, each against both children it names (pair g the first, pair n + g the
+// second: one target's pairs are contiguous and in its parent's order).  At each depth
+// every pair (one thread) walks the source subtree of its entry in depth-first order
+// without a stack: at source rank r the rule either descends (opens the source: r + 1)
+// or settles the whole subtree (P2P, M2L, pass down or nothing: r + subtree size); a
+// traversal longer than a few ranks is continued by a group of lanes.  Two passes per
+// depth (count, then write at scanned offsets).  Every list is written grouped by target
+// in the reference's order (shift, depth-first rank): no sort.
+
 // Image walks: a shifted node keeps no children when its box is rc or more from
 // the domain box or when 0.95*theta*distance exceeds its largest width
 // (src/remotes.c:145-158); the rule is evaluated on the fly, no copy is built.
@@ -83,50 +84,73 @@ struct alignas(16) WalkRec {
 };
 static_assert(sizeof(WalkRec) == 64, "one 64-byte record per rank");
 
-// a target of the current depth and what the rule needs of it, written by k_walk_prep
-struct Target {
+// by cell: what the rule needs of a target, in one 80-byte record
+struct alignas(16) TgtRec {
     CellGeom g;
-    double L;              // its box length (shifts, the image prune's domain box)
-    int64_t pbeg;          // its pending entries: pend[pbeg ..], one per pair
-    int32_t X, rank;       // cell id, depth-first rank
-    int32_t son0r, son1r;  // ranks of the children (nodes)
-    int32_t leaf, pad;
+    double L;         // its box length (shifts, the image prune's domain box)
+    int32_t rank;     // depth-first rank
+    int32_t son[2];   // children (nodes)
+    int32_t sonr[2];  // their ranks
+    int32_t leaf;
 };
+static_assert(sizeof(TgtRec) == 80, "one 80-byte record per cell");
 
-// a traversal left to a wavefront: pair q of target j goes on at rank r (< rend, the end
-// of its source subtree) after np, nm, nd outputs
-struct Cont {
-    int32_t q, j, r, rend;
-    uint32_t np, nm, nd, pad;
+// A down record: a source passed on to the two children t0, t1 of a node (at the start
+// depth t0 is the target itself and t1 = -1).  The entry holds the source's shift and its
+// depth-first rank, where a traversal of its subtree starts; a leaf source, which needs
+// no traversal, is named by its cell instead and flagged in t0.
+struct DownRec {
+    int32_t entry, t0, t1;
+};
+constexpr int32_t kSrcLeaf = 1 << 30;
+__device__ __forceinline__ bool src_leaf(const DownRec &D) { return (D.t0 & kSrcLeaf) != 0; }
+__device__ __forceinline__ int owner_of(const DownRec &D) { return D.t0 & kCellMask; }  // names the parent
+__device__ __forceinline__ int target_of(const DownRec &D, int c) { return c ? D.t1 : (D.t0 & kCellMask); }
+
+// A traversal continued by a group of lanes: pair q goes on at rank r (< rend, the end of
+// its source subtree) after the outputs its thread found (`part`, packed); the count
+// passes add up the pair's outputs (tp, tm, td), the write pass leaves where the rest
+// goes (qp, qm, qd).
+struct alignas(16) Cont {
+    int32_t q, r, rend, pad;
+    uint64_t part;
+    uint32_t tp, tm, td, pad2;
+    int64_t qp, qm, qd;
+};
+static_assert(sizeof(Cont) == 64, "one 64-byte record per continued traversal");
+
+// a listed pair (a node source): the pair, its target and its source entry
+struct TravRec {
+    int32_t q, T, entry;
 };
 
 struct WalkArgs {
-    const CellInfo *info;
     const WalkRec *rec;       // by depth-first rank
-    DevSizes *sz;             // cell counts, the targets of each depth, list bases, overflow
+    const TgtRec *trec;       // by cell
+    DevSizes *sz;             // cell counts, records of the depth, list bases, overflow
     double theta, theta2, rc, rc2;
-    BoxView bx;
-    // this depth: its targets are walk_tgt[tgt_off[d] .. tgt_off[d+1])
-    const int *walk_tgt;
-    int d;
-    int64_t tgt_cap;          // launch bound of the targets
-    Target *tgt;              // [tgt_cap]: the depth's targets (k_walk_prep)
-    const int64_t *pair_off;  // [ntgt + 1]: first pair of each target (scan of the pending counts)
-    const int32_t *tile_tgt;  // per tile of kWalkThreads pairs: the target of its first pair
-    int64_t npair_cap;        // pairs the per-pair arrays and the launch hold
-    const int32_t *pend;      // pending entries: the previous depth's down lists (or the start list)
-    uint32_t *pcnt;           // [3 npair]: outputs of each pair (P2P, M2L, down), count pass -> write pass
-    Cont *big;                // [npair]: the traversals continued by a wavefront each
+    const DownRec *din;       // this depth's records (sz->nrec of them, at most rec_cap)
+    int64_t rec_cap;          // launch bound of the records; the pairs are twice as many
+    uint64_t *pcnt;           // per pair: its outputs (packed) or its continuation
+    TravRec *tlist;           // per tile, its pairs with a node source (a traversal): tlist[tile * 256 + i]
+    int32_t *tcount;          // per tile, how many
+    uint32_t *qoff;           // [3 per pair]: a listed pair's output offsets from its tile's base
+    int trav_waves;           // wavefronts of the traversal passes (a multiple of kTileWaves)
+    Cont *cont;               // continued traversals: kContPerWave slots per traversal wavefront
+    int32_t *ccount;          // per traversal wavefront, its slots in use
+    const int32_t *coff;      // their exclusive scan
+    int32_t *cidx;            // the slots in use, in order
     // outputs of the write pass
-    int32_t *p2p_out, *m2l_out, *dl_out;  // list bases: sz->p2p_base, sz->m2l_base
-    int64_t *p2p_span, *m2l_span, *dl_span;  // per leaf / cell / node (index cell - nleaf)
+    int32_t *p2p_out, *m2l_out;  // list bases: sz->p2p_base, sz->m2l_base
+    DownRec *dl_out;             // the next depth's records
+    int64_t *p2p_span, *m2l_span;  // per leaf / cell
 };
 
-// Entries of the pending and down lists name the source by its depth-first rank (the
+// Entries of the walk's own records name the source by its depth-first rank (the
 // traversal starts there without a cell lookup); the P2P and M2L lists name cells.
 __device__ __forceinline__ int entry_rank(int32_t e) { return entry_cell(e); }
 
-__device__ __forceinline__ int decide(const WalkArgs &a, const Target &t, const WalkRec &R, int r, int sid,
+__device__ __forceinline__ int decide(const WalkArgs &a, const TgtRec &t, const WalkRec &R, int r, int sid,
                                       double sx, double sy, double sz)
 {
     const bool bleaf = R.dsize == 1;
@@ -152,179 +176,200 @@ __device__ __forceinline__ int decide(const WalkArgs &a, const Target &t, const 
     return (bigger || pruned) ? kActOpenA : kActOpenB;  // ties open the source
 }
 
-constexpr int kWalkThreads = 256;  // pairs per tile
+constexpr int kTile = 256;       // pairs per tile, one wavefront each
+constexpr int kTileWaves = 4;    // tiles (wavefronts) per block
+constexpr int kWalkThreads = 64 * kTileWaves;
 
-// A thread applies the rule to at most kThreadVisits ranks of its pair; a traversal that
-// goes on (a target leaf beside a dense cluster opens one source node after another: one
-// pair in a few hundred, but a wavefront waits for its slowest lane) is continued from the
-// rank it reached by a whole wavefront, 64 ranks at a time (wave_traverse).
-constexpr int kThreadVisits = 16;
+// A lane applies the rule to at most kThreadVisits ranks of a pair; a traversal that goes
+// on (a target leaf beside a dense cluster opens one source node after another) is
+// continued from the rank it reached by a group of kGroupLanes lanes, that many ranks at
+// a time.
+constexpr int kThreadVisits = 32;
+constexpr int kGroupLanes = 16;
 
-// The pairs of target t with the source subtree at rank r (record R, already loaded), in
-// depth-first order, for at most kThreadVisits ranks; returns the rank it stopped at (>=
-// the subtree's end when it finished).  WRITE = false counts the outputs; WRITE = true
-// stores them from the given positions, never more than the count pass found (lp, lm,
-// ld), so a store stays inside the ranges the scan gave this pair.
-template <bool WRITE>
-__device__ __forceinline__ int traverse(const WalkArgs &a, const Target &t, int r, int sid, WalkRec R, int &rend,
-                                        uint32_t &np, uint32_t &nm, uint32_t &nd, int64_t qp, int64_t qm, int64_t qd,
-                                        uint32_t lp = 0, uint32_t lm = 0, uint32_t ld = 0)
+// A pair's outputs as one 64-bit word: bit 0 clear, then down (19 bits), M2L (20), P2P
+// (24); bit 0 set: the pair was continued, bits 1.. hold its continuation's slot.  A
+// count past its field (a single pair's source subtree with 2^24 leaves) flags the
+// evaluation (kOvfPairs) instead of wrapping.
+__device__ __forceinline__ uint64_t pack_cnt(const WalkArgs &a, uint32_t p, uint32_t m, uint32_t d)
 {
-    double sx = 0.0, sy = 0.0, sz = 0.0;
-    if (sid != 0) shift_of(sid, t.L, sx, sy, sz);
-    rend = r + R.dsize;
-    for (int visits = 1;; visits++) {
-        const int act = decide(a, t, R, r, sid, sx, sy, sz);
-        if (act == kActP2P) {
-            if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.cell, sid);
-            np++;
-        } else if (act == kActM2L) {
-            if (WRITE && nm < lm) a.m2l_out[qm + nm] = pack_entry(R.cell, sid);
-            nm++;
-        } else if (act == kActOpenA) {
-            if (WRITE && nd < ld) a.dl_out[qd + nd] = pack_entry(r, sid);
-            nd++;
-        } else if (act == kActSelfNode) {
-            if (WRITE && nd + 1 < ld) {
-                a.dl_out[qd + nd] = pack_entry(t.son0r, 0);
-                a.dl_out[qd + nd + 1] = pack_entry(t.son1r, 0);
-            }
-            nd += 2;
-        }
-        if (act == kActOpenB && t.leaf) {
-            // a target leaf against a leaf takes P2P whatever the geometry: the opened
-            // node's leaf children are emitted here, in rank order, without a visit
-            const int nleaf = a.sz->nleaf;
-            r += 1;
-            if (R.kid[0] < nleaf) {
-                if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.kid[0], sid);
-                np++;
-                r += 1;
-                if (R.kid[1] < nleaf) {
-                    if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.kid[1], sid);
-                    np++;
-                    r += 1;
-                }
-            }
-        } else {
-            r += act == kActOpenB ? 1 : R.dsize;
-        }
-        if (r >= rend || visits == kThreadVisits) return r;
-        R = a.rec[r];
-    }
+    if (p >= (1u << 24) || m >= (1u << 20) || d >= (1u << 19)) atomicOr(&a.sz->overflow, (int)kOvfPairs);
+    return ((uint64_t)(p & 0xffffffu) << 40) | ((uint64_t)(m & 0xfffffu) << 20) | ((uint64_t)(d & 0x7ffffu) << 1);
+}
+__device__ __forceinline__ void unpack_cnt(uint64_t w, uint32_t &p, uint32_t &m, uint32_t &d)
+{
+    p = (uint32_t)(w >> 40);
+    m = (uint32_t)(w >> 20) & 0xfffffu;
+    d = (uint32_t)(w >> 1) & 0x7ffffu;
 }
 
-// The rest of a traversal, ranks [r, rend), by a wavefront: lane l takes rank carry + l
-// and applies the rule; a rank is visited unless an earlier rank of the chunk settled a
-// subtree containing it (an exclusive prefix max of the ranks each lane accounts for: a
-// settled rank its whole subtree, an opened one itself; a rank inside a settled subtree
-// accounts for no more than that subtree, and every subtree settled before r ends by r);
-// the visited ranks' outputs are emitted in rank order, as the single-thread walk emits
-// them.  A pair's own target is only ever met at its first rank (self pair), which the
-// thread has taken, so no rank here passes two children down.  np, nm, nd: wave-uniform
-// counts; WRITE stores from qp, qm, qd, never past lp, lm, ld.
-template <bool WRITE>
-__device__ __forceinline__ void wave_traverse(const WalkArgs &a, const Target &t, int r, int rend, int sid, int lane,
-                                              uint32_t &np, uint32_t &nm, uint32_t &nd, int64_t qp = 0,
-                                              int64_t qm = 0, int64_t qd = 0, uint32_t lp = 0, uint32_t lm = 0,
-                                              uint32_t ld = 0)
+// the down record of source `src` (its cell if a leaf, else its rank) for t's children
+__device__ __forceinline__ void write_down(const WalkArgs &a, int64_t at, int src, bool leaf, int sid, const TgtRec &t)
 {
+    a.dl_out[at] = DownRec{pack_entry(src, sid), t.son[0] | (leaf ? kSrcLeaf : 0), t.son[1]};
+}
+
+// One step of a pair's depth-first traversal of its source subtree: the rule at rank r
+// (record R) of target t; returns the next rank (>= the subtree's end when done).
+// WRITE = false counts the outputs; WRITE = true stores them from the given positions,
+// never more than the count pass found (lp, lm, ld), so a store stays inside the ranges
+// the scan gave this pair.
+template <bool WRITE>
+__device__ __forceinline__ int visit(const WalkArgs &a, const TgtRec &t, const WalkRec &R, int r, int sid, double sx,
+                                     double sy, double sz, uint32_t &np, uint32_t &nm, uint32_t &nd, int64_t qp,
+                                     int64_t qm, int64_t qd, uint32_t lp, uint32_t lm, uint32_t ld)
+{
+    const int act = decide(a, t, R, r, sid, sx, sy, sz);
+    if (act == kActP2P) {
+        if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.cell, sid);
+        np++;
+    } else if (act == kActM2L) {
+        if (WRITE && nm < lm) a.m2l_out[qm + nm] = pack_entry(R.cell, sid);
+        nm++;
+    } else if (act == kActOpenA) {
+        if (WRITE && nd < ld) {
+            const bool lf = R.dsize == 1;
+            write_down(a, qd + nd, lf ? R.cell : r, lf, sid, t);
+        }
+        nd++;
+    } else if (act == kActSelfNode) {
+        if (WRITE && nd + 1 < ld) {
+            const int nleaf = a.sz->nleaf;
+#pragma unroll
+            for (int k = 0; k < 2; k++) {
+                const bool lf = t.son[k] < nleaf;
+                write_down(a, qd + nd + k, lf ? t.son[k] : t.sonr[k], lf, 0, t);
+            }
+        }
+        nd += 2;
+    }
+    if (act != kActOpenB) return r + R.dsize;
+    if (!t.leaf) return r + 1;
+    // a target leaf against a leaf takes P2P whatever the geometry: the opened node's leaf
+    // children are emitted here, in rank order, without a step of their own
+    const int nleaf = a.sz->nleaf;
+    r += 1;
+    if (R.kid[0] < nleaf) {
+        if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.kid[0], sid);
+        np++;
+        r += 1;
+        if (R.kid[1] < nleaf) {
+            if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.kid[1], sid);
+            np++;
+            r += 1;
+        }
+    }
+    return r;
+}
+
+// The rest of a traversal, ranks [r, rend), by a group of G lanes (gl: the lane in the
+// group): lane gl takes rank carry + gl and applies the rule; a rank is visited unless an
+// earlier rank of the chunk settled a subtree containing it (an exclusive prefix max of
+// the ranks each lane accounts for: a settled rank its whole subtree, an opened one
+// itself; a rank inside a settled subtree accounts for no more than that subtree, and
+// every subtree settled before r ends by r); the visited ranks' outputs are emitted in
+// rank order, as the single-thread walk emits them.  A pair's own target is only ever met
+// at its first rank (self pair), which the thread has taken, so no rank here passes two
+// children down.  np, nm, nd: group-uniform counts; WRITE stores from qp, qm, qd, never
+// past lp, lm, ld.  Every lane of the group runs this together.
+template <bool WRITE, int G>
+__device__ __forceinline__ void group_traverse(const WalkArgs &a, const TgtRec &t, int r, int rend, int sid, int gl,
+                                               uint32_t &np, uint32_t &nm, uint32_t &nd, int64_t qp = 0,
+                                               int64_t qm = 0, int64_t qd = 0, uint32_t lp = 0, uint32_t lm = 0,
+                                               uint32_t ld = 0)
+{
+    static_assert(G <= 32 && (G & (G - 1)) == 0, "a power-of-two group of at most 32 lanes");
     double sx = 0.0, sy = 0.0, sz = 0.0;
     if (sid != 0) shift_of(sid, t.L, sx, sy, sz);
-    const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    const int gshift = (int)(threadIdx.x % 64) / G * G;  // the group's first lane in the wavefront
+    const uint32_t gmask = G == 32 ? 0xffffffffu : ((1u << G) - 1u);
+    const uint32_t below = (1u << gl) - 1u;
     int carry = r;
     while (carry < rend) {
-        const int s = carry + lane;
+        const int s = carry + gl;
         int act = kActNone, cov = s + 1, v = 0;
+        bool lf = false;
         if (s < rend) {
             const WalkRec R = a.rec[s];
             v = R.cell;
+            lf = R.dsize == 1;
             act = decide(a, t, R, s, sid, sx, sy, sz);
             cov = act == kActOpenB ? s + 1 : s + R.dsize;
         }
         int incl = cov;
 #pragma unroll
-        for (int d = 1; d < 64; d <<= 1) {
-            const int o = __shfl_up(incl, d, 64);
-            if (lane >= d) incl = max(incl, o);
+        for (int d = 1; d < G; d <<= 1) {
+            const int o = __shfl_up(incl, d, G);
+            if (gl >= d) incl = max(incl, o);
         }
-        int excl = __shfl_up(incl, 1, 64);
-        excl = lane == 0 ? carry : max(excl, carry);
+        int excl = __shfl_up(incl, 1, G);
+        excl = gl == 0 ? carry : max(excl, carry);
         const bool visited = s < rend && s >= excl;
         const bool isp = visited && act == kActP2P, ism = visited && act == kActM2L;
         const bool isd = visited && act == kActOpenA;
-        const uint64_t bp = __ballot(isp), bm = __ballot(ism), bd = __ballot(isd);
+        const uint32_t bp = (uint32_t)(__ballot(isp) >> gshift) & gmask;
+        const uint32_t bm = (uint32_t)(__ballot(ism) >> gshift) & gmask;
+        const uint32_t bd = (uint32_t)(__ballot(isd) >> gshift) & gmask;
         if (WRITE) {
-            const uint32_t ip = np + __popcll(bp & below), im = nm + __popcll(bm & below);
-            const uint32_t id = nd + __popcll(bd & below);
+            const uint32_t ip = np + __popc(bp & below), im = nm + __popc(bm & below);
+            const uint32_t id = nd + __popc(bd & below);
             if (isp && ip < lp) a.p2p_out[qp + ip] = pack_entry(v, sid);
             if (ism && im < lm) a.m2l_out[qm + im] = pack_entry(v, sid);
-            if (isd && id < ld) a.dl_out[qd + id] = pack_entry(s, sid);
+            if (isd && id < ld) write_down(a, qd + id, lf ? v : s, lf, sid, t);
         }
-        np += __popcll(bp);
-        nm += __popcll(bm);
-        nd += __popcll(bd);
-        carry = max(carry + 64, __shfl(incl, 63, 64));
+        np += __popc(bp);
+        nm += __popc(bm);
+        nd += __popc(bd);
+        carry = max(carry + G, __shfl(incl, G - 1, G));
     }
 }
 
-// the target of pair q: the last j in [lo, hi) with pair_off[j] <= q
-__device__ __forceinline__ int target_of(const int64_t *__restrict__ pair_off, int lo, int hi, int64_t q)
+// The pairs of a depth: record g yields pair g (target t0) and pair npad + g (target t1;
+// npad: the records rounded up to whole tiles), so the pairs of one target are contiguous
+// and in the order of its parent's down list.  Output positions follow this pair order
+// (tile sums scanned in tile order); which wavefront runs which tile is free, and is
+// chosen for the caches: workgroups go to the 8 XCDs in turn (block b to XCD b % 8), each
+// with its own L2, so XCD x takes one contiguous run of tiles, and within it the tiles of
+// both children of the same records one after the other (their sources are the same).
+__device__ __forceinline__ int64_t nrec_of(const WalkArgs &a) { return min(a.sz->nrec, a.rec_cap); }
+constexpr int kXcds = 8;
+
+struct TileMap {
+    int64_t n, npad, H;  // records, records rounded up to tiles, tiles per child stream
+    int64_t tile;        // this wavefront's tile (pair order), or -1: no tile
+    int64_t g0;          // the first record of the tile
+    int c;               // the child stream
+    int npair;           // pairs of the tile with a record
+};
+
+__device__ __forceinline__ TileMap tile_map(const WalkArgs &a)
 {
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (pair_off[mid] <= q) lo = mid;
-        else hi = mid;
+    TileMap m;
+    m.n = nrec_of(a);
+    m.H = (m.n + kTile - 1) / kTile;
+    m.npad = m.H * kTile;
+    const int64_t per = (2 * m.H + kXcds * kTileWaves - 1) / (kXcds * kTileWaves);  // blocks per XCD
+    const int x = blockIdx.x % kXcds;
+    const int64_t i = blockIdx.x / kXcds;
+    const int64_t p = ((int64_t)x * per + i) * kTileWaves + threadIdx.x / 64;
+    m.tile = -1;
+    m.g0 = 0;
+    m.c = 0;
+    m.npair = 0;
+    if (i < per && p < 2 * m.H) {
+        m.c = (int)(p & 1);
+        m.g0 = (p >> 1) * kTile;
+        m.tile = m.c * m.H + (p >> 1);
+        m.npair = (int)min<int64_t>(kTile, m.n - m.g0);
     }
-    return lo;
+    return m;
 }
 
-// the depth's targets, at most the launch bound (a depth with more flags the overflow in
-// k_walk_prep, and the pair arrays hold only the bound's targets)
-__device__ __forceinline__ int ntgt_all(const WalkArgs &a) { return (int)(a.sz->tgt_off[a.d + 1] - a.sz->tgt_off[a.d]); }
-__device__ __forceinline__ int ntgt_of(const WalkArgs &a) { return (int)min((int64_t)ntgt_all(a), a.tgt_cap); }
-__device__ __forceinline__ const int *tgt_of(const WalkArgs &a) { return a.walk_tgt + a.sz->tgt_off[a.d]; }
-__device__ __forceinline__ int64_t npair_of(const WalkArgs &a, int ntgt) { return min(a.pair_off[ntgt], a.npair_cap); }
-
-// The first pair offsets of the targets from the tile's first target on, in LDS: a tile
-// of pairs rarely spans more than kWalkThreads targets.  Called by every thread of the
-// block (it synchronises); `live` says whether the tile holds pairs (block-uniform).
-__device__ __forceinline__ int tile_window(const WalkArgs &a, int ntgt, bool live, int64_t *s_off)
+// blocks for ntile tiles of pairs: a whole number per XCD
+static inline int walk_blocks(int64_t ntile)
 {
-    int j0 = 0;
-    if (live) {
-        j0 = a.tile_tgt[blockIdx.x];
-        for (int i = threadIdx.x; i <= kWalkThreads; i += kWalkThreads) s_off[i] = a.pair_off[min(j0 + i, ntgt)];
-    }
-    __syncthreads();
-    return j0;
-}
-
-// the target of pair q of this tile: a search of the LDS window, else of the rest
-__device__ __forceinline__ int tile_target(const WalkArgs &a, const int64_t *s_off, int j0, int ntgt, int64_t q)
-{
-    if (q < s_off[kWalkThreads]) {
-        int lo = 0, hi = kWalkThreads;  // s_off[lo] <= q < s_off[hi]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (s_off[mid] <= q) lo = mid;
-            else hi = mid;
-        }
-        return j0 + lo;
-    }
-    // pair_off[j0 + kWalkThreads] <= q < pair_off[ntgt]
-    return target_of(a.pair_off, j0 + kWalkThreads, ntgt, q);
-}
-
-// per tile, the target of its first pair: each target marks the tile starts in its range
-__global__ void k_tile_targets(WalkArgs a, int32_t *__restrict__ tile_tgt)
-{
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ntgt = ntgt_of(a);
-    if (j >= ntgt) return;
-    const int64_t npair = npair_of(a, ntgt);
-    const int64_t lo = a.pair_off[j], hi = min(a.pair_off[j + 1], npair);
-    for (int64_t k = (lo + kWalkThreads - 1) / kWalkThreads; k * kWalkThreads < hi; k++) tile_tgt[k] = j;
+    const int64_t per = (ntile + kXcds * kTileWaves - 1) / (kXcds * kTileWaves);
+    return (int)(per * kXcds);
 }
 
 struct Cnt3 {
@@ -334,231 +379,562 @@ struct AddCnt3 {
     __host__ __device__ Cnt3 operator()(const Cnt3 &x, const Cnt3 &y) const { return {x.p + y.p, x.m + y.m, x.d + y.d}; }
 };
 
-__global__ void __launch_bounds__(kWalkThreads) k_walk_count(WalkArgs a, Cnt3 *__restrict__ tile_cnt, int ntile)
+__device__ __forceinline__ uint64_t wave_sum(uint64_t v)
 {
-    typedef hipcub::BlockReduce<int64_t, kWalkThreads> Reduce;
-    __shared__ typename Reduce::TempStorage tmp;
-    __shared__ int64_t s_off[kWalkThreads + 1];
-    const int ntgt = ntgt_of(a);
-    const int64_t npair_all = a.pair_off[ntgt];
-    const int64_t npair = min(npair_all, a.npair_cap);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && npair_all > a.npair_cap) a.sz->overflow |= kOvfPairs;
-    const int64_t q0 = (int64_t)blockIdx.x * kWalkThreads, q = q0 + threadIdx.x;
-    const int j0 = tile_window(a, ntgt, q0 < npair, s_off);
-    uint32_t np = 0, nm = 0, nd = 0;
-    bool more = false;
-    Cont cont;
-    if (q < npair) {
-        const int j = tile_target(a, s_off, j0, ntgt, q);
-        const Target t = a.tgt[j];
-        const int32_t entry = a.pend[t.pbeg + (q - a.pair_off[j])];
-        const int r = entry_rank(entry);
-        int rend;
-        const int stop = traverse<false>(a, t, r, entry_shift(entry), a.rec[r], rend, np, nm, nd, 0, 0, 0);
-        more = stop < rend;
-        if (more) cont = Cont{(int32_t)q, j, stop, rend, np, nm, nd, 0};
-        a.pcnt[3 * q] = np;
-        a.pcnt[3 * q + 1] = nm;
-        a.pcnt[3 * q + 2] = nd;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// A lane's current pair: target record, source traversal state and outputs so far.
+struct LanePair {
+    int64_t q;           // the pair, -1: none
+    TgtRec t;
+    WalkRec R;           // the record at rank r
+    int r, rend, sid, visits;
+    double sx, sy, sz;   // the source's shift
+    uint32_t np, nm, nd;
+};
+
+// start pair q on this lane (target T >= 0, a node source: entry names its rank)
+__device__ __forceinline__ void lane_start(const WalkArgs &a, LanePair &L, int64_t q, int T, int32_t entry)
+{
+    L.q = q;
+    L.t = a.trec[T];
+    L.r = entry_rank(entry);
+    L.sid = entry_shift(entry);
+    L.R = a.rec[L.r];
+    L.rend = L.r + L.R.dsize;
+    L.sx = L.sy = L.sz = 0.0;
+    if (L.sid != 0) shift_of(L.sid, L.t.L, L.sx, L.sy, L.sz);
+    L.np = L.nm = L.nd = 0;
+    L.visits = 0;
+}
+
+// Two pairs whose source is a leaf (use[i]; target T[i] >= 0), one rule each, no
+// traversal: a target leaf takes P2P (leaf-leaf pairs always do); a node target an M2L
+// entry, a down record or nothing, by the rule on the two cells.  The loads of both pairs
+// go out before either is decided.  WRITE = false counts; WRITE = true stores at qp, qm,
+// qd within the counts found (lp, lm, ld).
+template <bool WRITE>
+__device__ __forceinline__ void leaf_source_pairs(const WalkArgs &a, int nleaf, const bool (&use)[2],
+                                                  const int (&T)[2], const DownRec (&D)[2], uint32_t (&np)[2],
+                                                  uint32_t (&nm)[2], uint32_t (&nd)[2], const int64_t (&qp)[2],
+                                                  const int64_t (&qm)[2], const int64_t (&qd)[2],
+                                                  const uint32_t (&lp)[2], const uint32_t (&lm)[2],
+                                                  const uint32_t (&ld)[2])
+{
+    TgtRec t[2];
+    CellGeom g[2];
+    bool node[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        node[i] = use[i] && T[i] >= nleaf;
+        t[i] = a.trec[node[i] ? T[i] : 0];
+        g[i] = a.trec[node[i] ? entry_cell(D[i].entry) : 0].g;
     }
-    {
-        // the continued traversals, counted by k_walk_deep_count (which adds them to this
-        // tile's sums): one append per wavefront
-        const uint64_t bal = __ballot(more);
-        if (bal) {
-            const int lane = threadIdx.x % 64;
-            unsigned long long base = 0;
-            if (lane == __ffsll((unsigned long long)bal) - 1)
-                base = atomicAdd(reinterpret_cast<unsigned long long *>(&a.sz->n_big), (unsigned long long)__popcll(bal));
-            base = __shfl(base, __ffsll((unsigned long long)bal) - 1, 64);
-            if (more) a.big[base + __popcll(bal & ((1ull << lane) - 1ull))] = cont;
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        np[i] = nm[i] = nd[i] = 0;
+        if (!use[i]) continue;
+        const int cell = entry_cell(D[i].entry), sid = entry_shift(D[i].entry);
+        if (!node[i]) {
+            if (WRITE && lp[i] > 0) a.p2p_out[qp[i]] = pack_entry(cell, sid);
+            np[i] = 1;
+            continue;
+        }
+        WalkRec R;
+        R.g = g[i];
+        R.cell = cell;
+        R.dsize = 1;
+        R.kid[0] = R.kid[1] = -1;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        if (sid != 0) shift_of(sid, t[i].L, sx, sy, sz);
+        const int act = decide(a, t[i], R, -1, sid, sx, sy, sz);  // rank -1: not the target itself
+        if (act == kActM2L) {
+            if (WRITE && lm[i] > 0) a.m2l_out[qm[i]] = pack_entry(cell, sid);
+            nm[i] = 1;
+        } else if (act == kActOpenA) {
+            if (WRITE && ld[i] > 0) write_down(a, qd[i], cell, true, sid, t[i]);
+            nd[i] = 1;
         }
     }
-    const int64_t tp = Reduce(tmp).Sum((int64_t)np);
-    __syncthreads();
-    const int64_t tm = Reduce(tmp).Sum((int64_t)nm);
-    __syncthreads();
-    const int64_t td = Reduce(tmp).Sum((int64_t)nd);
-    if (threadIdx.x == 0) {
-        tile_cnt[blockIdx.x] = Cnt3{tp, tm, td};
-        if (blockIdx.x == 0) tile_cnt[ntile] = Cnt3{0, 0, 0};  // the scan's last entry is the total
-    }
 }
 
-__global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const Cnt3 *__restrict__ tile_off)
+// the lanes of one wavefront see each other's LDS stores from here on
+__device__ __forceinline__ void wave_sync()
 {
-    typedef hipcub::BlockScan<int64_t, kWalkThreads> Scan;
-    __shared__ typename Scan::TempStorage tmp;
-    __shared__ int64_t s_off[kWalkThreads + 1];
-    if (a.sz->overflow) return;  // the same for every thread: no block scan is left half-entered
-    const int ntgt = ntgt_of(a);
-    const int64_t npair = npair_of(a, ntgt);
-    const int64_t q0 = (int64_t)blockIdx.x * kWalkThreads, q = q0 + threadIdx.x;
-    const bool live = q < npair;
-    int64_t np = 0, nm = 0, nd = 0;
-    if (live) {
-        np = a.pcnt[3 * q];
-        nm = a.pcnt[3 * q + 1];
-        nd = a.pcnt[3 * q + 2];
-    }
-    int64_t op, om, od;
-    Scan(tmp).ExclusiveSum(np, op);
-    __syncthreads();
-    Scan(tmp).ExclusiveSum(nm, om);
-    __syncthreads();
-    Scan(tmp).ExclusiveSum(nd, od);
-    const int j0 = tile_window(a, ntgt, q0 < npair, s_off);
-    if (!live) return;
-    const Cnt3 to = tile_off[blockIdx.x];
-    const int64_t qp = a.sz->p2p_base + to.p + op, qm = a.sz->m2l_base + to.m + om, qd = to.d + od;
-    const int j = tile_target(a, s_off, j0, ntgt, q);
-    const int64_t pj = a.pair_off[j];
-    const Target t = a.tgt[j];
-    if (np + nm + nd > 0) {
-        // the same ranks as the count pass; a continued traversal's rest is k_walk_deep_write's
-        const int32_t entry = a.pend[t.pbeg + (q - pj)];
-        const int r = entry_rank(entry);
-        int rend;
-        uint32_t cp = 0, cm = 0, cd = 0;
-        traverse<true>(a, t, r, entry_shift(entry), a.rec[r], rend, cp, cm, cd, qp, qm, qd, (uint32_t)np,
-                       (uint32_t)nm, (uint32_t)nd);
-    }
-    // the target's first pair opens its spans, its last pair closes them
-    const bool first = q == pj, last = q + 1 == a.pair_off[j + 1];
-    const int nleaf = a.sz->nleaf;
-    if (first) {
-        if (t.leaf) a.p2p_span[2 * (size_t)t.X] = qp;
-        else a.dl_span[2 * (size_t)(t.X - nleaf)] = qd;
-        a.m2l_span[2 * (size_t)t.X] = qm;
-    }
-    if (last) {
-        if (t.leaf) a.p2p_span[2 * (size_t)t.X + 1] = qp + np;
-        else a.dl_span[2 * (size_t)(t.X - nleaf) + 1] = qd + nd;
-        a.m2l_span[2 * (size_t)t.X + 1] = qm + nm;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// the target and the source entry of pair q (npad: records rounded up to tiles)
+__device__ __forceinline__ void pair_of(const WalkArgs &a, int64_t npad, int64_t q, int &T, int32_t &entry)
+{
+    const DownRec D = a.din[q < npad ? q : q - npad];
+    T = target_of(D, q < npad ? 0 : 1);
+    entry = D.entry;
+}
+__device__ __forceinline__ int64_t npad_of(const WalkArgs &a)
+{
+    return (nrec_of(a) + kTile - 1) / kTile * kTile;
+}
+
+// the traversal passes' grid: persistent wavefronts, each with its own continuation slots
+constexpr int kTravWavesMax = 16384;
+constexpr int kContPerWave = 64;
+
+// a pair's outputs: its packed counts, or its continuation's totals
+__device__ __forceinline__ void pair_counts(const WalkArgs &a, uint64_t w, uint32_t &p, uint32_t &m, uint32_t &d)
+{
+    if (w & 1u) {
+        const Cont &ct = a.cont[w >> 1];  // a slot in use
+        p = ct.tp;
+        m = ct.tm;
+        d = ct.td;
+    } else {
+        unpack_cnt(w, p, m, d);
     }
 }
 
-// The continued traversals, one wavefront each (a fixed grid walking the list of them):
-// their counts into pcnt and, by atomic adds of integers (order-free), into their tiles'
-// sums.
-constexpr int kDeepWaves = 8192;
-
-__global__ void __launch_bounds__(256) k_walk_deep_count(WalkArgs a, Cnt3 *__restrict__ tile_cnt)
+// Count pass, pairs with a leaf source (most of them): one wavefront per tile of 256
+// pairs, two pairs per lane at a time, all lanes in step; their packed counts.  The pairs
+// with a node source (a traversal) are listed per tile instead, for k_walk_count_trav.
+__global__ void __launch_bounds__(kWalkThreads) k_walk_count(WalkArgs a)
 {
     const int lane = threadIdx.x % 64;
-    const int64_t nbig = a.sz->n_big;
-    for (int64_t k = (int64_t)blockIdx.x * 4 + threadIdx.x / 64; k < nbig; k += (int64_t)gridDim.x * 4) {
-        const Cont ct = a.big[k];
-        const int64_t q = ct.q;
-        const Target t = a.tgt[ct.j];
-        const int32_t entry = a.pend[t.pbeg + (q - a.pair_off[ct.j])];
-        uint32_t np = 0, nm = 0, nd = 0;
-        wave_traverse<false>(a, t, ct.r, ct.rend, entry_shift(entry), lane, np, nm, nd);
-        if (lane == 0) {
-            a.pcnt[3 * q] = ct.np + np;
-            a.pcnt[3 * q + 1] = ct.nm + nm;
-            a.pcnt[3 * q + 2] = ct.nd + nd;
-            unsigned long long *tc = reinterpret_cast<unsigned long long *>(tile_cnt + q / kWalkThreads);
-            atomicAdd(tc, (unsigned long long)np);
-            atomicAdd(tc + 1, (unsigned long long)nm);
-            atomicAdd(tc + 2, (unsigned long long)nd);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const TileMap tm = tile_map(a);
+    if (tm.tile < 0) return;  // the same for the whole wavefront
+    const int64_t q0 = tm.tile * kTile;
+    const int nleaf = a.sz->nleaf;
+    const int64_t z2[2] = {0, 0};
+    const uint32_t u2[2] = {0, 0};
+    bool trav[kTile / 64];
+    uint64_t bt[kTile / 64];
+    int ntrav = 0;
+#pragma unroll
+    for (int k = 0; k < kTile / 64; k += 2) {
+        int j[2], T[2];
+        DownRec D[2];
+        bool use[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            j[i] = lane + 64 * (k + i);
+            D[i] = j[i] < tm.npair ? a.din[tm.g0 + j[i]] : DownRec{0, 0, -1};
+            T[i] = j[i] < tm.npair ? target_of(D[i], tm.c) : -1;
+            use[i] = T[i] >= 0 && src_leaf(D[i]);
+            trav[k + i] = T[i] >= 0 && !src_leaf(D[i]);
+            bt[k + i] = __ballot(trav[k + i]);
+            ntrav += __popcll(bt[k + i]);
+        }
+        uint32_t np[2], nm[2], nd[2];
+        leaf_source_pairs<false>(a, nleaf, use, T, D, np, nm, nd, z2, z2, z2, u2, u2, u2);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            if (j[i] < tm.npair && !trav[k + i]) a.pcnt[q0 + j[i]] = pack_cnt(a, np[i], nm[i], nd[i]);
+    }
+    int at = 0;
+#pragma unroll
+    for (int k = 0; k < kTile / 64; k++) {
+        if (trav[k]) {
+            const DownRec D = a.din[tm.g0 + lane + 64 * k];
+            a.tlist[q0 + at + __popcll(bt[k] & below)] = TravRec{(int32_t)(q0 + lane + 64 * k), target_of(D, tm.c), D.entry};
+        }
+        at += __popcll(bt[k]);
+    }
+    if (lane == 0) a.tcount[tm.tile] = ntrav;
+}
+
+// The listed pairs of a depth, by super-chunks of kSuper consecutive tiles (tiles of one
+// child stream and neighbouring records: the same sources); a wavefront walks its
+// super-chunks' lists as one sequence.
+constexpr int kSuper = 4;
+
+struct SuperChunk {
+    int64_t t0;    // first tile
+    int pre[kSuper + 1];  // listed pairs before each tile of the chunk
+};
+
+__device__ __forceinline__ SuperChunk super_chunk(const WalkArgs &a, int64_t s, int64_t ntile_dev)
+{
+    SuperChunk sc;
+    sc.t0 = s * kSuper;
+    const int lane = threadIdx.x % 64;
+    int cnt = 0;
+    if (lane < kSuper && sc.t0 + lane < ntile_dev) cnt = a.tcount[sc.t0 + lane];
+    sc.pre[0] = 0;
+#pragma unroll
+    for (int i = 0; i < kSuper; i++) sc.pre[i + 1] = sc.pre[i] + __shfl(cnt, i, 64);
+    return sc;
+}
+
+// the listed pair at position idx (< sc.pre[kSuper]) of a super-chunk
+__device__ __forceinline__ TravRec super_pair(const WalkArgs &a, const SuperChunk &sc, int idx)
+{
+    int i = 0;
+#pragma unroll
+    for (int k = 1; k < kSuper; k++) i += idx >= sc.pre[k];
+    return a.tlist[(sc.t0 + i) * kTile + (idx - sc.pre[i])];
+}
+
+// A wavefront's stream of listed pairs: its super-chunks s = gw, gw + trav_waves, ...  in
+// turn, without waiting at the end of one for the lanes still busy with its pairs.
+struct TravStream {
+    int64_t s, nsuper, ntile_dev;
+    SuperChunk sc;
+    int cn, next;  // listed pairs of the current super-chunk, handed out
+    bool more;     // a current super-chunk
+};
+
+__device__ __forceinline__ void stream_open(const WalkArgs &a, TravStream &ts, int64_t gw)
+{
+    ts.ntile_dev = 2 * (npad_of(a) / kTile);
+    ts.nsuper = (ts.ntile_dev + kSuper - 1) / kSuper;
+    ts.s = gw;
+    ts.cn = ts.next = 0;
+    ts.more = ts.s < ts.nsuper;
+    if (ts.more) {
+        ts.sc = super_chunk(a, ts.s, ts.ntile_dev);
+        ts.cn = ts.sc.pre[kSuper];
+    }
+}
+
+// the next super-chunk while the current one is handed out (the same in every lane)
+__device__ __forceinline__ void stream_advance(const WalkArgs &a, TravStream &ts)
+{
+    while (ts.more && ts.next >= ts.cn) {
+        ts.s += a.trav_waves;
+        ts.more = ts.s < ts.nsuper;
+        ts.cn = ts.next = 0;
+        if (ts.more) {
+            ts.sc = super_chunk(a, ts.s, ts.ntile_dev);
+            ts.cn = ts.sc.pre[kSuper];
         }
     }
 }
 
-// The continued traversals' outputs: the pair's offsets (the tile's offset plus the
-// outputs of the tile's earlier pairs, summed by the wavefront) past what its thread
-// wrote.
-__global__ void __launch_bounds__(256) k_walk_deep_write(WalkArgs a, const Cnt3 *__restrict__ tile_off)
+// Count pass, the listed pairs (a node source): super-chunks per wavefront; lanes take the
+// pairs in turn and a lane that finishes one takes the next, so a long traversal holds up
+// one lane only; each step of the loop is one rule (one record load) per busy lane.  A
+// traversal that reaches kThreadVisits steps is handed to k_walk_deep_count through one
+// of the wavefront's continuation slots; when they are used up, its lane goes on alone.
+__global__ void __launch_bounds__(kWalkThreads) k_walk_count_trav(WalkArgs a)
+{
+    const int lane = threadIdx.x % 64;
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t gw = (int64_t)blockIdx.x * kTileWaves + threadIdx.x / 64;
+    Cont *slots = a.cont + gw * kContPerWave;
+    int ncont = 0;  // slots in use (the same in every lane)
+    TravStream ts;
+    stream_open(a, ts, gw);
+    LanePair L;
+    L.q = -1;
+    for (;;) {
+        stream_advance(a, ts);
+        if (!ts.more && !__ballot(L.q >= 0)) break;
+        const bool want = L.q < 0;
+        const uint64_t bw = __ballot(want);
+        if (want) {
+            const int idx = ts.next + __popcll(bw & below);
+            if (idx < ts.cn) {
+                const TravRec tr = super_pair(a, ts.sc, idx);
+                lane_start(a, L, tr.q, tr.T, tr.entry);
+            }
+        }
+        ts.next = min(ts.next + __popcll(bw), ts.cn);
+        bool fin = false, cap = false;
+        if (L.q >= 0) {
+            L.r = visit<false>(a, L.t, L.R, L.r, L.sid, L.sx, L.sy, L.sz, L.np, L.nm, L.nd, 0, 0, 0, 0, 0, 0);
+            L.visits++;
+            if (L.r >= L.rend) fin = true;
+            else if (L.visits == kThreadVisits) cap = true;
+        }
+        const uint64_t bh = __ballot(cap);
+        const bool room = ncont + __popcll(bh) <= kContPerWave;  // the same in every lane
+        const bool hit = cap && room;
+        if (hit) {
+            const int k = ncont + __popcll(bh & below);
+            Cont ct{};
+            ct.q = (int32_t)L.q;
+            ct.r = L.r;
+            ct.rend = L.rend;
+            ct.part = pack_cnt(a, L.np, L.nm, L.nd);
+            slots[k] = ct;
+            a.pcnt[L.q] = ((uint64_t)(gw * kContPerWave + k) << 1) | 1u;
+        }
+        if (room) ncont += __popcll(bh);
+        if (fin) a.pcnt[L.q] = pack_cnt(a, L.np, L.nm, L.nd);
+        if (fin || hit) L.q = -1;
+        else if (L.q >= 0) L.R = a.rec[L.r];
+    }
+    if (lane == 0) a.ccount[gw] = ncont;
+}
+
+// the continuation slots in use, listed in order (coff: the scan of ccount)
+__global__ void k_walk_cont_index(WalkArgs a)
+{
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= (int64_t)a.trav_waves * kContPerWave) return;
+    const int64_t g = k / kContPerWave, i = k % kContPerWave;
+    if (i < a.ccount[g]) a.cidx[a.coff[g] + i] = (int32_t)k;
+}
+
+__device__ __forceinline__ int64_t cont_total(const WalkArgs &a)
+{
+    return (int64_t)a.coff[a.trav_waves - 1] + a.ccount[a.trav_waves - 1];
+}
+
+// the continued traversals, kGroupLanes lanes each, over a fixed grid: their totals
+constexpr int kDeepBlocks = 2048;
+
+__global__ void __launch_bounds__(256) k_walk_deep_count(WalkArgs a)
+{
+    constexpr int G = kGroupLanes;
+    const int gl = threadIdx.x % G;
+    const int64_t npad = npad_of(a), ntot = cont_total(a);
+    const int64_t ngroup = (int64_t)gridDim.x * (blockDim.x / G);
+    for (int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; m < ntot; m += ngroup) {
+        const int64_t k = a.cidx[m];
+        const Cont ct = a.cont[k];
+        int T;
+        int32_t entry;
+        pair_of(a, npad, ct.q, T, entry);
+        const TgtRec t = a.trec[T];
+        uint32_t np = 0, nm = 0, nd = 0;
+        group_traverse<false, G>(a, t, ct.r, ct.rend, entry_shift(entry), gl, np, nm, nd);
+        if (gl == 0) {
+            uint32_t p0, m0, d0;
+            unpack_cnt(ct.part, p0, m0, d0);
+            a.cont[k].tp = p0 + np;
+            a.cont[k].tm = m0 + nm;
+            a.cont[k].td = d0 + nd;
+        }
+    }
+}
+
+// per tile, the sums of its pairs' outputs (scanned next); sums past the depth's tiles
+// are zero (the scan runs to the launch bound ntile, whose last entry is the total)
+__global__ void __launch_bounds__(kWalkThreads) k_walk_tile_sums(WalkArgs a, Cnt3 *__restrict__ tile_cnt, int ntile)
+{
+    const int lane = threadIdx.x % 64;
+    const TileMap tm = tile_map(a);
+    for (int64_t t = 2 * tm.H + (int64_t)blockIdx.x * kWalkThreads + threadIdx.x; t <= ntile;
+         t += (int64_t)gridDim.x * kWalkThreads)
+        tile_cnt[t] = Cnt3{0, 0, 0};
+    if (tm.tile < 0) return;
+    const int64_t q0 = tm.tile * kTile;
+    uint64_t sp = 0, sm = 0, sd = 0;
+#pragma unroll
+    for (int k = 0; k < kTile / 64; k++) {
+        const int j = lane + 64 * k;
+        if (j < tm.npair) {
+            uint32_t p, m, d;
+            pair_counts(a, a.pcnt[q0 + j], p, m, d);
+            sp += p;
+            sm += m;
+            sd += d;
+        }
+    }
+    sp = wave_sum(sp);
+    sm = wave_sum(sm);
+    sd = wave_sum(sd);
+    if (lane == 0) tile_cnt[tm.tile] = Cnt3{(int64_t)sp, (int64_t)sm, (int64_t)sd};
+}
+
+// Write pass, per tile: the pairs' outputs scanned by the wavefront into offsets from the
+// tile's base (LDS, one more entry: the counts are the differences); the first and last
+// pair of each target set its spans; the pairs with a leaf source are stored here, the
+// listed ones keep their offsets (qoff) for k_walk_write_trav.
+__global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const Cnt3 *__restrict__ tile_off)
+{
+    __shared__ int32_t s_own[kTileWaves][kTile];
+    __shared__ uint32_t s_off[kTileWaves][3][kTile + 1];
+    if (a.sz->overflow) return;  // the same for every thread
+    const int lane = threadIdx.x % 64, w = threadIdx.x / 64;
+    const TileMap tm = tile_map(a);
+    if (tm.tile < 0) return;  // the same for the whole wavefront; only wavefront barriers follow
+    const int64_t q0 = tm.tile * kTile;
+    for (int i = lane; i < tm.npair; i += 64) s_own[w][i] = owner_of(a.din[tm.g0 + i]);
+    // each lane scans 4 consecutive pairs; the wavefront scans the lanes' sums
+    {
+        uint32_t c[3][4];
+        uint32_t tot[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int j = 4 * lane + k;
+            uint32_t p = 0, m = 0, d = 0;
+            if (j < tm.npair) pair_counts(a, a.pcnt[q0 + j], p, m, d);
+            c[0][k] = p;
+            c[1][k] = m;
+            c[2][k] = d;
+            tot[0] += p;
+            tot[1] += m;
+            tot[2] += d;
+        }
+#pragma unroll
+        for (int x = 0; x < 3; x++) {
+            uint32_t incl = tot[x];
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t y = __shfl_up(incl, o, 64);
+                if (lane >= o) incl += y;
+            }
+            uint32_t run = incl - tot[x];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                s_off[w][x][4 * lane + k] = run;
+                run += c[x][k];
+            }
+            if (lane == 63) s_off[w][x][kTile] = run;
+        }
+    }
+    int32_t own_prev = -1, own_next = -1;  // the parents named by the records either side of the tile
+    if (lane == 0 && tm.g0 > 0) own_prev = owner_of(a.din[tm.g0 - 1]);
+    if (lane == 0 && tm.g0 + kTile < tm.n) own_next = owner_of(a.din[tm.g0 + kTile]);
+    own_prev = __shfl(own_prev, 0, 64);
+    own_next = __shfl(own_next, 0, 64);
+    wave_sync();
+    const Cnt3 to = tile_off[tm.tile];
+    const int64_t bp = a.sz->p2p_base + to.p, bm = a.sz->m2l_base + to.m, bd = to.d;
+    const int nleaf = a.sz->nleaf;
+#pragma unroll
+    for (int k = 0; k < kTile / 64; k += 2) {
+        int j[2], T[2];
+        DownRec D[2];
+        bool use[2];
+        int64_t qp[2], qm[2], qd[2];
+        uint32_t lp[2], lm[2], ld[2], o[2][3];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            j[i] = lane + 64 * (k + i);
+            const bool valid = j[i] < tm.npair;
+            D[i] = valid ? a.din[tm.g0 + j[i]] : DownRec{0, 0, -1};
+            T[i] = valid ? target_of(D[i], tm.c) : -1;
+            const int jj = valid ? j[i] : 0;
+#pragma unroll
+            for (int x = 0; x < 3; x++) o[i][x] = s_off[w][x][jj];
+            lp[i] = s_off[w][0][jj + 1] - o[i][0];
+            lm[i] = s_off[w][1][jj + 1] - o[i][1];
+            ld[i] = s_off[w][2][jj + 1] - o[i][2];
+            qp[i] = bp + o[i][0];
+            qm[i] = bm + o[i][1];
+            qd[i] = bd + o[i][2];
+            use[i] = T[i] >= 0 && src_leaf(D[i]);
+            if (T[i] >= 0 && !src_leaf(D[i]) && lp[i] + lm[i] + ld[i] > 0) {
+                uint32_t *qo = a.qoff + 3 * (q0 + j[i]);
+                qo[0] = o[i][0];
+                qo[1] = o[i][1];
+                qo[2] = o[i][2];
+            }
+            if (T[i] >= 0) {
+                // spans: the records of one parent are contiguous, and t0 names it
+                const int32_t own = owner_of(D[i]);
+                const int32_t prev = j[i] > 0 ? s_own[w][j[i] - 1] : own_prev;
+                const int32_t nxt = j[i] + 1 < tm.npair ? s_own[w][j[i] + 1] : own_next;
+                if (prev != own) {
+                    if (T[i] < nleaf) a.p2p_span[2 * (size_t)T[i]] = qp[i];
+                    a.m2l_span[2 * (size_t)T[i]] = qm[i];
+                }
+                if (nxt != own) {
+                    if (T[i] < nleaf) a.p2p_span[2 * (size_t)T[i] + 1] = qp[i] + lp[i];
+                    a.m2l_span[2 * (size_t)T[i] + 1] = qm[i] + lm[i];
+                }
+            }
+        }
+        uint32_t np[2], nm[2], nd[2];
+        leaf_source_pairs<true>(a, nleaf, use, T, D, np, nm, nd, qp, qm, qd, lp, lm, ld);
+    }
+}
+
+// Write pass, the listed pairs, as in k_walk_count_trav, storing at the offsets the tile
+// pass left; a continued traversal (flagged in its pair's word) stops where the count
+// pass handed it on and leaves where its rest goes.
+__global__ void __launch_bounds__(kWalkThreads) k_walk_write_trav(WalkArgs a, const Cnt3 *__restrict__ tile_off)
 {
     if (a.sz->overflow) return;
     const int lane = threadIdx.x % 64;
-    const int64_t nbig = a.sz->n_big;
-    for (int64_t k = (int64_t)blockIdx.x * 4 + threadIdx.x / 64; k < nbig; k += (int64_t)gridDim.x * 4) {
-        const Cont ct = a.big[k];
-        const int64_t q = ct.q;
-        const int64_t t0 = q / kWalkThreads * kWalkThreads;
-        int64_t sp = 0, sm = 0, sd = 0;
-        for (int64_t i = t0 + lane; i < q; i += 64) {
-            sp += a.pcnt[3 * i];
-            sm += a.pcnt[3 * i + 1];
-            sd += a.pcnt[3 * i + 2];
+    const uint64_t below = (1ull << lane) - 1ull;
+    const int64_t gw = (int64_t)blockIdx.x * kTileWaves + threadIdx.x / 64;
+    const int64_t p2p_base = a.sz->p2p_base, m2l_base = a.sz->m2l_base;
+    TravStream ts;
+    stream_open(a, ts, gw);
+    LanePair L;
+    L.q = -1;
+    int64_t qp = 0, qm = 0, qd = 0;
+    uint32_t lp = 0, lm = 0, ld = 0;
+    uint64_t wd = 0;
+    for (;;) {
+        stream_advance(a, ts);
+        if (!ts.more && !__ballot(L.q >= 0)) break;
+        const bool want = L.q < 0;
+        const uint64_t bw = __ballot(want);
+        if (want) {
+            const int idx = ts.next + __popcll(bw & below);
+            if (idx < ts.cn) {
+                const TravRec tr = super_pair(a, ts.sc, idx);
+                wd = a.pcnt[tr.q];
+                pair_counts(a, wd, lp, lm, ld);
+                if (lp + lm + ld > 0) {
+                    const Cnt3 to = tile_off[tr.q / kTile];
+                    const uint32_t *qo = a.qoff + 3 * (int64_t)tr.q;
+                    qp = p2p_base + to.p + qo[0];
+                    qm = m2l_base + to.m + qo[1];
+                    qd = to.d + qo[2];
+                    lane_start(a, L, tr.q, tr.T, tr.entry);
+                }
+            }
         }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) {
-            sp += __shfl_xor(sp, o, 64);
-            sm += __shfl_xor(sm, o, 64);
-            sd += __shfl_xor(sd, o, 64);
+        ts.next = min(ts.next + __popcll(bw), ts.cn);
+        if (L.q >= 0) {
+            L.r = visit<true>(a, L.t, L.R, L.r, L.sid, L.sx, L.sy, L.sz, L.np, L.nm, L.nd, qp, qm, qd, lp, lm, ld);
+            L.visits++;
+            if (L.r >= L.rend) {
+                L.q = -1;
+            } else if (L.visits == kThreadVisits && (wd & 1u)) {
+                // the rest is k_walk_deep_write's, from here (flagged by the count pass at
+                // the same step)
+                Cont &ct = a.cont[wd >> 1];
+                ct.qp = qp + L.np;
+                ct.qm = qm + L.nm;
+                ct.qd = qd + L.nd;
+                L.q = -1;
+            } else {
+                L.R = a.rec[L.r];
+            }
         }
-        const Cnt3 to = tile_off[q / kWalkThreads];
-        const int64_t qp = a.sz->p2p_base + to.p + sp + ct.np, qm = a.sz->m2l_base + to.m + sm + ct.nm;
-        const int64_t qd = to.d + sd + ct.nd;
-        const Target t = a.tgt[ct.j];
-        const int32_t entry = a.pend[t.pbeg + (q - a.pair_off[ct.j])];
-        uint32_t np = 0, nm = 0, nd = 0;
-        wave_traverse<true>(a, t, ct.r, ct.rend, entry_shift(entry), lane, np, nm, nd, qp, qm, qd,
-                            a.pcnt[3 * q] - ct.np, a.pcnt[3 * q + 1] - ct.nm, a.pcnt[3 * q + 2] - ct.nd);
     }
 }
 
-// Pending entries of each target of a depth: its parent's down list, or at the start
-// depth the start list (start_target < 0: target j is the root of box j and takes
-// entries j * nstart .. ; else start_target alone takes all nstart entries); and the
-// target's record.  Targets without pending entries get empty spans here (the write
-// pass sets the others).
-__global__ void k_walk_prep(WalkArgs a, int start, int start_target, int nstart, int64_t *__restrict__ cnt)
+// the continued traversals' outputs, where the write pass left them
+__global__ void __launch_bounds__(256) k_walk_deep_write(WalkArgs a)
 {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    const int ntgt = ntgt_of(a);
-    if (j == 0) {
-        a.sz->n_big = 0;  // the count pass lists this depth's continued traversals
-        if (ntgt_all(a) > a.tgt_cap) a.sz->overflow |= kOvfTargets;
+    constexpr int G = kGroupLanes;
+    if (a.sz->overflow) return;
+    const int gl = threadIdx.x % G;
+    const int64_t npad = npad_of(a), ntot = cont_total(a);
+    const int64_t ngroup = (int64_t)gridDim.x * (blockDim.x / G);
+    for (int64_t m = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) / G; m < ntot; m += ngroup) {
+        const int64_t k = a.cidx[m];
+        const Cont ct = a.cont[k];
+        int T;
+        int32_t entry;
+        pair_of(a, npad, ct.q, T, entry);
+        const TgtRec t = a.trec[T];
+        uint32_t p0, m0, d0;
+        unpack_cnt(ct.part, p0, m0, d0);
+        uint32_t np = 0, nm = 0, nd = 0;
+        group_traverse<true, G>(a, t, ct.r, ct.rend, entry_shift(entry), gl, np, nm, nd, ct.qp, ct.qm, ct.qd,
+                                ct.tp - p0, ct.tm - m0, ct.td - d0);
     }
-    if (j > a.tgt_cap) return;
-    if (j >= ntgt || a.sz->overflow) {
-        cnt[j] = 0;  // zeros up to the bound: the scan's entry ntgt is the total
-        return;
-    }
-    const int X = tgt_of(a)[j];
-    const int nleaf = a.sz->nleaf;
-    const CellInfo I = a.info[X];
-    int64_t b = 0, e = 0;
-    if (start) {
-        if (start_target < 0) {
-            b = (int64_t)j * nstart;
-            e = b + nstart;
-        } else if (X == start_target) {
-            e = nstart;
-        }
-    } else {
-        b = a.dl_span[2 * (size_t)(I.parent - nleaf)];
-        e = a.dl_span[2 * (size_t)(I.parent - nleaf) + 1];
-    }
-    Target t;
-    t.g = a.rec[I.dfs].g;
-    t.L = a.bx.len_of(X);
-    t.pbeg = b;
-    t.X = X;
-    t.rank = I.dfs;
-    t.leaf = X < nleaf;
-    t.son0r = t.leaf ? -1 : a.info[I.son0].dfs;
-    t.son1r = t.leaf ? -1 : a.info[I.son1].dfs;
-    t.pad = 0;
-    a.tgt[j] = t;
-    cnt[j] = e - b;
-    if (X < nleaf) {
-        a.p2p_span[2 * (size_t)X] = a.p2p_span[2 * (size_t)X + 1] = 0;
-    } else {
-        a.dl_span[2 * (size_t)(X - nleaf)] = a.dl_span[2 * (size_t)(X - nleaf) + 1] = 0;
-    }
-    a.m2l_span[2 * (size_t)X] = a.m2l_span[2 * (size_t)X + 1] = 0;
 }
 
-// The depth's totals (P2P, M2L and down entries: the tile scan's last entry) against the
+// The depth's totals (P2P, M2L and down records: the tile scan's last entry) against the
 // capacities: the list bases move past the previous depth's entries, and what would not
 // fit flags the overflow, after which nothing more is written.
 __global__ void k_walk_step(DevSizes *sz, const Cnt3 *__restrict__ tot, int64_t p2p_cap, int64_t m2l_cap,
-                            int64_t dl_cap, int64_t pair_cap)
+                            int64_t dl_cap)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     sz->p2p_base = sz->n_p2p;
@@ -568,7 +944,6 @@ __global__ void k_walk_step(DevSizes *sz, const Cnt3 *__restrict__ tot, int64_t 
     if (sz->n_p2p + t.p > p2p_cap) ovf |= kOvfP2P;
     if (sz->n_m2l + t.m > m2l_cap) ovf |= kOvfM2L;
     if (t.d > dl_cap) ovf |= kOvfDown;
-    if (2 * t.d > pair_cap) ovf |= kOvfPairs;
     if (ovf || sz->overflow) {
         sz->overflow |= ovf;
         sz->dl_total = 0;
@@ -579,40 +954,63 @@ __global__ void k_walk_step(DevSizes *sz, const Cnt3 *__restrict__ tot, int64_t 
     sz->dl_total = t.d;
 }
 
-// walk start: empty lists, bases at zero
-__global__ void k_walk_begin(DevSizes *sz)
+// between depths: the down records written become the next depth's records
+__global__ void k_walk_next(DevSizes *sz)
+{
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    sz->nrec = sz->dl_total;
+}
+
+// walk start: empty lists, bases at zero, n0 start records
+__global__ void k_walk_begin(DevSizes *sz, int64_t n0)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     sz->n_p2p = sz->n_m2l = sz->p2p_base = sz->m2l_base = 0;
     sz->dl_total = 0;
+    sz->nrec = n0;
 }
 
-// after the last depth of a planned walk: down entries left over mean deeper targets
-// that were never visited
+// after the last depth of a planned walk: records left over mean deeper targets that
+// were never visited
 __global__ void k_walk_end(DevSizes *sz)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     if (sz->dl_total > 0) sz->overflow |= kOvfDepths;
 }
 
-// cells by depth-first rank: geometry, cell id and subtree size, read by the traversals
+// by depth-first rank, the cells as sources (geometry, id, subtree size, children in rank
+// order); by cell, the cells as targets
 __global__ void k_walk_records(const DevSizes *__restrict__ sz, const CellInfo *__restrict__ info,
-                               const CellGeom *__restrict__ geom, WalkRec *__restrict__ rec)
+                               const CellGeom *__restrict__ geom, BoxView bx, WalkRec *__restrict__ rec,
+                               TgtRec *__restrict__ trec)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (sz->overflow || c >= sz->ncell) return;
     const CellInfo I = info[c];
+    const CellGeom g = geom[c];
     WalkRec R;
-    R.g = geom[c];
+    R.g = g;
     R.cell = c;
     R.dsize = I.dsize;
     R.kid[0] = R.kid[1] = -1;
+    TgtRec t;
+    t.g = g;
+    t.L = bx.len_of(c);
+    t.rank = I.dfs;
+    t.son[0] = t.son[1] = t.sonr[0] = t.sonr[1] = -1;
+    t.leaf = c < sz->nleaf;
     if (I.son0 >= 0) {
-        const bool first0 = info[I.son0].dfs == I.dfs + 1;
+        const int r0 = info[I.son0].dfs, r1 = info[I.son1].dfs;
+        const bool first0 = r0 == I.dfs + 1;
         R.kid[0] = first0 ? I.son0 : I.son1;
         R.kid[1] = first0 ? I.son1 : I.son0;
+        t.son[0] = I.son0;
+        t.son[1] = I.son1;
+        t.sonr[0] = r0;
+        t.sonr[1] = r1;
     }
     rec[I.dfs] = R;
+    trec[c] = t;
 }
 
 // every span empty (cells a walk from a given pair never reaches keep no entries); n is
@@ -623,15 +1021,19 @@ __global__ void k_clear_spans(int n, int64_t *__restrict__ span)
     if (i < 2 * (int64_t)n) span[i] = 0;
 }
 
-// start list: (root of box b, shift) for every box and shift, or (source, shift)
-__global__ void k_start_list(int32_t *f, const CellInfo *__restrict__ info, const int *__restrict__ box_root, int nbox,
-                             int B, int shift_lo, int shift_hi)
+// the start records: every box's root against itself (box_root given) or the given pair
+// (target A, source B), one record per shift in [shift_lo, shift_hi)
+__global__ void k_start_list(DownRec *f, const CellInfo *__restrict__ info, const int *__restrict__ box_root, int nbox,
+                             int A, int B, int shift_lo, int shift_hi)
 {
     const int nshift = shift_hi - shift_lo;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= (box_root ? nbox : 1) * nshift) return;
     const int s = shift_lo + i % nshift;
-    f[i] = pack_entry(info[box_root ? box_root[i / nshift] : B].dfs, s);  // pending entries name ranks
+    const int tgt = box_root ? box_root[i / nshift] : A;
+    const int src = box_root ? tgt : B;
+    const bool lf = info[src].son0 < 0;
+    f[i] = DownRec{pack_entry(lf ? src : info[src].dfs, s), tgt | (lf ? kSrcLeaf : 0), -1};
 }
 
 // ---------------------------------------------------------------------------- host side
@@ -651,98 +1053,97 @@ static void *grow_keep(DevBuf &b, size_t need, size_t keep, hipStream_t st)
 }
 
 // The walk from the pair (start_target, start_source) over the shifts [shift_lo, shift_hi),
-// or from every box's (root, root) when start_target < 0.  Per depth: pending counts and
-// their scan, the count pass, the scan of its tile sums (the host reads the depth's three
-// totals to size the outputs), the write pass.
+// or from every box's (root, root) when start_target < 0.  Per depth: the count pass and
+// its continuations, the scan of the tile sums (an ordinary walk reads the depth's three
+// totals to size the outputs), the write pass and its continuations.
 void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int start_source)
 {
     hipStream_t st = c.stream;
     const bool planned = c.plan.active;
-    const int nlb = nleaf_bound(c), ncb = ncell_bound(c), nnb = nnode_bound(c);
+    const int nlb = nleaf_bound(c), ncb = ncell_bound(c);
     if (ncb >= (1 << 27)) throw std::runtime_error("more than 2^27 cells");
     need(c.cell_box, sizeof(int) * (size_t)ncb, "cell boxes");
-    need(c.walk_tgt, sizeof(int) * (size_t)ncb, "walk targets");
     const bool forest = start_target < 0;
     if (planned && !forest) throw std::runtime_error("a planned walk starts from the roots");
-    // depths of the walk: the tree's (ordinary) or the plan's
-    const int ndepth = planned ? c.plan.depths : (int)c.tgt_off.size() - 1;
+    // depths of the walk: the plan's, or (ordinary) until no records are left, at most
+    // one more than the tree's levels
+    const int ndepth = planned ? c.plan.depths : kMaxDepth + 1;
 
     DevSizes *sz = dptr<DevSizes>(c.sizes);
     WalkArgs a{};
-    a.info = dptr<CellInfo>(c.info);
     a.sz = sz;
     a.theta = c.prm.theta;
     a.theta2 = c.prm.theta * c.prm.theta;
     a.rc = c.prm.rc;
     a.rc2 = c.prm.rc * c.prm.rc;
-    a.bx = box_view(c);
-    a.walk_tgt = dptr<int>(c.walk_tgt);
     WalkRec *rec = static_cast<WalkRec *>(ctx_buf(c, c.walk_rec, sizeof(WalkRec) * (size_t)ncb, "walk records"));
-    k_walk_records<<<(ncb + 255) / 256, 256, 0, st>>>(sz, dptr<CellInfo>(c.info), dptr<CellGeom>(c.geom), rec);
+    TgtRec *trec = static_cast<TgtRec *>(ctx_buf(c, c.walk_trec, sizeof(TgtRec) * (size_t)ncb, "walk records"));
+    k_walk_records<<<(ncb + 255) / 256, 256, 0, st>>>(sz, dptr<CellInfo>(c.info), dptr<CellGeom>(c.geom), box_view(c),
+                                                       rec, trec);
     a.rec = rec;
+    a.trec = trec;
     a.p2p_span = static_cast<int64_t *>(ctx_buf(c, c.p2p_off, sizeof(int64_t) * 2 * (size_t)std::max(1, nlb), "P2P list"));
     a.m2l_span = static_cast<int64_t *>(ctx_buf(c, c.m2l_off, sizeof(int64_t) * 2 * (size_t)ncb, "M2L list"));
-    a.dl_span = static_cast<int64_t *>(ctx_buf(c, c.dl_span, sizeof(int64_t) * 2 * (size_t)std::max(1, nnb), "down lists"));
     k_clear_spans<<<(2 * nlb + 255) / 256 + 1, 256, 0, st>>>(nlb, a.p2p_span);
     k_clear_spans<<<(2 * ncb + 255) / 256 + 1, 256, 0, st>>>(ncb, a.m2l_span);
-    k_walk_begin<<<1, 1, 0, st>>>(sz);
 
     // start: every box's root against itself, or the given pair; shifts [lo, hi)
     const int nshift = shift_hi - shift_lo;
     const int64_t n0 = (int64_t)(forest ? c.nbox : 1) * nshift;
     if (forest) need(c.d_box_root, sizeof(int) * (size_t)c.nbox, "box roots");
-    int32_t *slist = static_cast<int32_t *>(ctx_buf(c, c.start_list, sizeof(int32_t) * (size_t)std::max<int64_t>(n0, 1),
-                                                    "walk start"));
+    if (planned && n0 > c.plan.dl_cap) throw std::runtime_error("the plan holds fewer walk records than the start");
+    DevBuf *din = &c.dl_a, *dout = &c.dl_b;
+    DownRec *slist = static_cast<DownRec *>(ctx_buf(c, *din, sizeof(DownRec) * (size_t)(planned ? c.plan.dl_cap : n0),
+                                                    "walk records"));
     k_start_list<<<(int)((n0 + 255) / 256), 256, 0, st>>>(slist, dptr<CellInfo>(c.info),
                                                           forest ? dptr<int>(c.d_box_root) : nullptr, c.nbox,
-                                                          start_source, shift_lo, shift_hi);
-    int d0 = 0;
-    if (!forest) {
-        CellInfo I;
-        PH_CHECK(hipMemcpyAsync(&I, dptr<CellInfo>(c.info) + start_target, sizeof I, hipMemcpyDeviceToHost, st));
-        PH_CHECK(hipStreamSynchronize(st));
-        d0 = I.depth;
-    }
+                                                          start_target, start_source, shift_lo, shift_hi);
+    k_walk_begin<<<1, 1, 0, st>>>(sz, n0);
 
     int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 16;
-    int64_t npair = n0, n_p2p = 0, n_m2l = 0;  // host copies (ordinary evaluation)
-    int64_t max_pairs = n0, max_dl = 0;
+    int64_t nrec = n0, n_p2p = 0, n_m2l = 0;  // host copies (ordinary evaluation)
+    int64_t max_rec = n0;
     int depths = 0;
-    const int32_t *pend = slist;
-    DevBuf *dl_out = &c.dl_b, *dl_other = &c.dl_a;
-    for (int d = d0; d < ndepth && (planned || npair > 0); d++) {
-        // launch bounds of this depth: targets and pairs
-        const int64_t tcap = planned ? c.plan.tgt_cap : c.tgt_off[d + 1] - c.tgt_off[d];
-        const int64_t pcap = planned ? c.plan.pair_cap : npair;
-        a.d = d;
-        a.tgt_cap = tcap;
-        a.pend = pend;
-        int64_t *poff = static_cast<int64_t *>(ctx_buf(c, c.pair_off, sizeof(int64_t) * (size_t)(tcap + 1), "walk"));
-        if (pcap >= ((int64_t)1 << 31)) throw std::runtime_error("more than 2^31 walk pairs at one depth");
-        int64_t *cnt = static_cast<int64_t *>(ctx_buf(c, c.pend_span, sizeof(int64_t) * (size_t)(tcap + 1), "walk"));
-        a.pair_off = poff;
-        a.tgt = static_cast<Target *>(ctx_buf(c, c.walk_trec, sizeof(Target) * (size_t)std::max<int64_t>(tcap, 1), "walk"));
-        k_walk_prep<<<(int)((tcap + 1 + 255) / 256), 256, 0, st>>>(a, d == d0, forest ? -1 : start_target, nshift, cnt);
-        {
-            size_t bytes = 0;
-            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, cnt, poff, (int)(tcap + 1), st));
-            void *tmp = scan_storage(c, bytes);
-            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, cnt, poff, (int)(tcap + 1), st));
-        }
-        const int ntile = (int)((pcap + kWalkThreads - 1) / kWalkThreads);  // >= 1: pcap >= 1
-        a.pcnt = static_cast<uint32_t *>(ctx_buf(c, c.pair_cnt, sizeof(uint32_t) * 3 * (size_t)std::max<int64_t>(pcap, 1),
-                                                 "walk pairs"));
-        a.npair_cap = pcap;
-        a.big = static_cast<Cont *>(ctx_buf(c, c.big_pairs, sizeof(Cont) * (size_t)std::max<int64_t>(pcap, 1),
-                                            "walk pairs"));
-        int32_t *tile_tgt = static_cast<int32_t *>(ctx_buf(c, c.walk_tiles, sizeof(int32_t) * (size_t)ntile, "walk tiles"));
-        a.tile_tgt = tile_tgt;
-        k_tile_targets<<<(int)((std::max<int64_t>(tcap, 1) + 255) / 256), 256, 0, st>>>(a, tile_tgt);
+    for (int d = 0; d < ndepth && (planned || nrec > 0); d++) {
+        const int64_t rcap = planned ? c.plan.dl_cap : nrec;
+        if (2 * (rcap + kTile) >= ((int64_t)1 << 31))
+            throw std::runtime_error("more than 2^31 walk pairs at one depth");
+        a.din = static_cast<const DownRec *>(din->p);
+        a.rec_cap = rcap;
+        const int ntile = (int)(2 * ((rcap + kTile - 1) / kTile));  // >= 2: rcap >= 1
+        a.pcnt = static_cast<uint64_t *>(ctx_buf(c, c.pair_cnt, sizeof(uint64_t) * kTile * (size_t)ntile, "walk pairs"));
+        // the traversal lists, per tile; the continuation slots, a fixed number per
+        // traversal wavefront
+        a.tlist = static_cast<TravRec *>(ctx_buf(c, c.walk_tlist, sizeof(TravRec) * kTile * (size_t)ntile, "walk pairs"));
+        a.tcount = static_cast<int32_t *>(ctx_buf(c, c.walk_tcount, sizeof(int32_t) * (size_t)ntile, "walk pairs"));
+        a.qoff = static_cast<uint32_t *>(ctx_buf(c, c.walk_qoff, 3 * sizeof(uint32_t) * kTile * (size_t)ntile, "walk pairs"));
+        // the traversal passes: about one super-chunk per wavefront, at most kTravWavesMax
+        const int64_t nsuper_cap = (ntile + kSuper - 1) / kSuper;
+        const int tw = (int)std::min<int64_t>(kTravWavesMax, (nsuper_cap + kTileWaves - 1) / kTileWaves * kTileWaves);
+        const int nslot = tw * kContPerWave;
+        a.trav_waves = tw;
+        a.cont = static_cast<Cont *>(ctx_buf(c, c.big_pairs, sizeof(Cont) * (size_t)nslot, "walk continuations"));
+        int32_t *ccount = static_cast<int32_t *>(ctx_buf(c, c.walk_ccount, sizeof(int32_t) * 2 * (size_t)tw,
+                                                         "walk continuations"));
+        a.ccount = ccount;
+        a.coff = ccount + tw;
+        a.cidx = static_cast<int32_t *>(ctx_buf(c, c.walk_cidx, sizeof(int32_t) * (size_t)nslot, "walk continuations"));
         Cnt3 *tile_cnt = static_cast<Cnt3 *>(ctx_buf(c, c.cnt_tmp, sizeof(Cnt3) * 2 * (size_t)(ntile + 1), "walk tiles"));
         Cnt3 *tile_off = tile_cnt + (ntile + 1);
-        k_walk_count<<<ntile, kWalkThreads, 0, st>>>(a, tile_cnt, ntile);
-        const int deep_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(kDeepWaves / 4, (pcap + 3) / 4));
-        k_walk_deep_count<<<deep_blocks, 256, 0, st>>>(a, tile_cnt);
+        const int nblk = walk_blocks(ntile);
+        // the leaf-source pairs and the lists; the traversals and their continuations; the
+        // tile sums, scanned
+        k_walk_count<<<nblk, kWalkThreads, 0, st>>>(a);
+        k_walk_count_trav<<<tw / kTileWaves, kWalkThreads, 0, st>>>(a);
+        {
+            size_t bytes = 0;
+            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, ccount, ccount + tw, tw, st));
+            void *tmp = scan_storage(c, bytes);
+            PH_CHECK(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, ccount, ccount + tw, tw, st));
+        }
+        k_walk_cont_index<<<(nslot + 255) / 256, 256, 0, st>>>(a);
+        k_walk_deep_count<<<kDeepBlocks, 256, 0, st>>>(a);
+        k_walk_tile_sums<<<nblk, kWalkThreads, 0, st>>>(a, tile_cnt, ntile);
         {
             size_t bytes = 0;
             PH_CHECK(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, tile_cnt, tile_off, AddCnt3(), Cnt3{0, 0, 0},
@@ -751,47 +1152,49 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
             PH_CHECK(hipcub::DeviceScan::ExclusiveScan(tmp, bytes, tile_cnt, tile_off, AddCnt3(), Cnt3{0, 0, 0},
                                                        ntile + 1, st));
         }
-        int64_t p2p_cap, m2l_cap, dl_cap, next_pairs;
+        int64_t p2p_cap, m2l_cap, dl_cap;
         if (planned) {
             p2p_cap = c.plan.p2p_cap;
             m2l_cap = c.plan.m2l_cap;
             dl_cap = c.plan.dl_cap;
-            next_pairs = c.plan.pair_cap;
         } else {
             // the depth's totals size its outputs
             PH_CHECK(hipMemcpyAsync(pin, tile_off + ntile, sizeof(Cnt3), hipMemcpyDeviceToHost, st));
+            PH_CHECK(hipMemcpyAsync(pin + 3, &sz->overflow, sizeof(int), hipMemcpyDeviceToHost, st));
             PH_CHECK(hipEventRecord(c.counts_ready, st));
             PH_CHECK(hipEventSynchronize(c.counts_ready));
+            if (const int ovf = *reinterpret_cast<const int *>(pin + 3))
+                throw std::runtime_error("walk: a pair's outputs exceed their packed counts (overflow bits " +
+                                         std::to_string(ovf) + ")");
             const int64_t add_p2p = pin[0], add_m2l = pin[1], add_dl = pin[2];
             grow_keep(c.p2p_src, 4 * (size_t)std::max<int64_t>(n_p2p + add_p2p, 1), 4 * (size_t)n_p2p, st);
             grow_keep(c.m2l_src, 4 * (size_t)std::max<int64_t>(n_m2l + add_m2l, 1), 4 * (size_t)n_m2l, st);
-            // the pending list (the other ping-pong buffer) is read by this write: only the
-            // output buffer may move
-            grow_keep(*dl_out, 4 * (size_t)std::max<int64_t>(add_dl, 1), 0, st);
+            // this depth's records (the other ping-pong buffer) are read by the write: only
+            // the output buffer may move
+            grow_keep(*dout, sizeof(DownRec) * (size_t)std::max<int64_t>(add_dl, 1), 0, st);
             n_p2p += add_p2p;
             n_m2l += add_m2l;
-            npair = 2 * add_dl;  // the children of this depth's nodes read their parents' lists
-            max_pairs = std::max(max_pairs, npair);
-            max_dl = std::max(max_dl, add_dl);
+            nrec = add_dl;
+            max_rec = std::max(max_rec, nrec);
             depths = d + 1;
             p2p_cap = c.p2p_src.cap / 4;
             m2l_cap = c.m2l_src.cap / 4;
-            dl_cap = dl_out->cap / 4;
-            next_pairs = npair;
+            dl_cap = dout->cap / sizeof(DownRec);
         }
         // at the plan's capacities (allocated by the first planned evaluation, checked by the
         // others); an ordinary evaluation grew them above
         ctx_buf(c, c.p2p_src, sizeof(int32_t) * (size_t)std::max<int64_t>(p2p_cap, 1), "P2P list");
         ctx_buf(c, c.m2l_src, sizeof(int32_t) * (size_t)std::max<int64_t>(m2l_cap, 1), "M2L list");
-        ctx_buf(c, *dl_out, sizeof(int32_t) * (size_t)std::max<int64_t>(dl_cap, 1), "down lists");
+        ctx_buf(c, *dout, sizeof(DownRec) * (size_t)std::max<int64_t>(dl_cap, 1), "walk records");
         a.p2p_out = dptr<int32_t>(c.p2p_src);
         a.m2l_out = dptr<int32_t>(c.m2l_src);
-        a.dl_out = dptr<int32_t>(*dl_out);
-        k_walk_step<<<1, 1, 0, st>>>(sz, tile_off + ntile, p2p_cap, m2l_cap, dl_cap, next_pairs);
-        k_walk_write<<<ntile, kWalkThreads, 0, st>>>(a, tile_off);
-        k_walk_deep_write<<<deep_blocks, 256, 0, st>>>(a, tile_off);
-        pend = a.dl_out;
-        std::swap(dl_out, dl_other);
+        a.dl_out = dptr<DownRec>(*dout);
+        k_walk_step<<<1, 1, 0, st>>>(sz, tile_off + ntile, p2p_cap, m2l_cap, dl_cap);
+        k_walk_write<<<nblk, kWalkThreads, 0, st>>>(a, tile_off);
+        k_walk_write_trav<<<tw / kTileWaves, kWalkThreads, 0, st>>>(a, tile_off);
+        k_walk_deep_write<<<kDeepBlocks, 256, 0, st>>>(a);
+        k_walk_next<<<1, 1, 0, st>>>(sz);
+        std::swap(din, dout);
     }
     if (planned) {
         k_walk_end<<<1, 1, 0, st>>>(sz);
@@ -801,8 +1204,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         if (forest) {
             Measure &m = c.meas;
             m.depths = depths;
-            m.pairs = max_pairs;
-            m.dl = max_dl;
+            m.dl = max_rec;
             m.p2p = n_p2p;
             m.m2l = n_m2l;
         }

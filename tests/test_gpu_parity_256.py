@@ -5,10 +5,15 @@ The oracle builds the whole 256^3 tree and every multipole but walks and evaluat
 a seeded sample of target leaves (oracle.force_leaves, pinned against the full oracle
 bit for bit): the GPU FP64 path must match it to 1e-10 on every particle of those leaves,
 with the same P2P list per target leaf and the same number of M2L entries on the leaf
-and its ancestors.  The FP32-mixed path is compared with it too (the bench's gate:
+and its ancestors; and the whole P2P and M2L lists must hold as many entries as the
+oracle's full walk counts (a committed fixture).  The FP32-mixed path is compared with it too (the bench's gate:
 p99 <= 1e-4), and its largest relative error is reported with the particle's
 acceleration against the median, since relative errors blow up where the force cancels.
 """
+import hashlib
+import json
+import os
+
 import numpy as np
 import pytest
 
@@ -17,6 +22,7 @@ import oracle
 import photons_gpu as pg
 
 pytestmark = pytest.mark.gpu
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 @pytest.fixture(scope="module")
@@ -45,6 +51,14 @@ def test_fp64_matches_the_oracle_at_256cubed(headline):
     e = rel(acc[ids], want)
     print(f"FP64 vs oracle: max {e.max():.2e}", flush=True)
     assert e.max() <= 1e-10
+    # the whole lists: as many entries as the oracle's full walk counts
+    # (tests/golden/make_counts256.py)
+    with open(os.path.join(GOLDEN, "lists_clustered256.json")) as f:
+        tot = json.load(f)
+    assert hashlib.sha256(np.ascontiguousarray(pos).tobytes()).hexdigest() == tot["positions_sha256"]
+    st = ctx.stats()
+    assert (st["nleaf"], st["nnode"]) == (tot["nleaf"], tot["nnode"])
+    assert (st["p2p_pairs"], st["m2l_pairs"]) == (tot["p2p_entries"], tot["m2l_entries"])
     # the same lists for the sampled targets
     cells = ctx.cells()
     nleaf = cells["nleaf"]
