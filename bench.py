@@ -47,9 +47,13 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+PLAN_REUSE = True  # --no-plan-reuse: every evaluation sizes its buffers as it goes
+
+
 def run_precision(torch, pos_dev, n, prm, precision, steps, warmup):
     ctx = pg.Context(prm, precision)
     ctx.set_timing(True)
+    ctx.set_plan_reuse(PLAN_REUSE)
     acc_dev = torch.empty((n, 3), dtype=torch.float64, device="cuda")
     name = "fp32-mixed" if precision == pg.FP32_MIXED else "fp64"
     for w in range(warmup):
@@ -413,7 +417,11 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="first copy MiB mebibytes device to device (a plain copy kernel of known size, "
                          "to scale hardware byte counters)")
+    ap.add_argument("--no-plan-reuse", action="store_true",
+                    help="size every evaluation's buffers as it goes (host reads per tree level and walk depth)")
     args = ap.parse_args()
+    global PLAN_REUSE
+    PLAN_REUSE = not args.no_plan_reuse
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (each box or domain runs on one GPU);"
               " run with --gpus 1", file=sys.stderr)

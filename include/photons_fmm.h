@@ -75,6 +75,12 @@ int photons_destroy(photons_ctx *ctx);
 int photons_set_params(photons_ctx *ctx, const photons_params *prm);
 int photons_get_params(const photons_ctx *ctx, photons_params *prm);
 int photons_set_timing(photons_ctx *ctx, int enabled);
+/* Plan reuse (no counterpart in the reference; on by default): the first evaluation of a
+ * box set reads counts back to size its buffers and plans the next ones, which then run
+ * without host round trips and wait once at the end for the overflow check (particles
+ * that outgrew the plan are evaluated again the first way).  Same result bit for bit;
+ * off, every evaluation sizes its buffers as it goes.                            */
+int photons_set_plan_reuse(photons_ctx *ctx, int enabled);
 int photons_get_phase_ms(const photons_ctx *ctx, double *ms /* [PHOTONS_NPHASE] */);
 int photons_get_stats(const photons_ctx *ctx, photons_stats *st);
 

@@ -97,15 +97,29 @@ struct Plan {
     int depths = 0;        // walk depths (0 .. depths-1)
     int scap = 0;          // nodes of one level
     int tcap = 0, lcap = 0;  // temporary nodes, leaves
-    int64_t dl_cap = 0;    // walk records (down records) of one depth
+    int64_t dl_cap = 0;    // walk records (down records) of one depth, the largest
+    std::vector<int64_t> rec_depth;  // walk records of each depth (launch bounds)
     int64_t p2p_cap = 0, m2l_cap = 0;  // list entries
     int item_cap = 0;      // M2L work items
+};
+
+// A plan kept for the next evaluations of the same boxes: an ordinary evaluation measures
+// and plans; the next ones run planned (no host round trip inside) and check the overflow
+// bits once at the end, measuring again if the particles outgrew the plan.
+struct Reuse {
+    bool on = true;
+    bool have = false;       // a plan for the box set below
+    bool allocated = false;  // its buffers are allocated
+    std::vector<int64_t> box_off;
+    std::vector<double> box_len;
+    Plan plan;
 };
 
 // What an ordinary evaluation measured, for planning: the largest count of each kind
 struct Measure {
     int levels = 0, scap = 0, ntmp = 0, nleaf = 0, depths = 0, items = 0;
     int64_t dl = 0, p2p = 0, m2l = 0;
+    std::vector<int64_t> dl_depth;  // walk records of each depth
 };
 
 struct Timing {
@@ -161,6 +175,7 @@ struct Ctx {
     hipEvent_t counts_ready = nullptr;  // a count copied to `pinned` while later work runs
 
     Plan plan;                     // active: the evaluation runs without host round trips
+    Reuse reuse;                   // the plan kept for the next evaluations of these boxes
     Measure meas;                  // filled by ordinary evaluations
     hipGraph_t graph = nullptr;    // a captured planned evaluation (photons_graph_*)
     hipGraphExec_t graph_exec = nullptr;

@@ -283,6 +283,13 @@ int photons_set_timing(photons_ctx *x, int enabled)
     return 0;
 }
 
+int photons_set_plan_reuse(photons_ctx *x, int enabled)
+{
+    x->c.reuse.on = enabled != 0;
+    if (!x->c.reuse.on) x->c.reuse.have = false;
+    return 0;
+}
+
 int photons_get_phase_ms(const photons_ctx *x, double *ms)
 {
     for (int i = 0; i < PHOTONS_NPHASE; i++) ms[i] = x->c.phase_ms[i];
@@ -376,6 +383,50 @@ static void single_box(Ctx &c, int64_t n) { set_boxes(c, 1, &n, &c.prm.boxsize);
 // evaluations would move them under the captured graph.
 static const char *kGraphHeld = "the context holds a planned evaluation (photons_graph_destroy releases it)";
 
+static void plan_from_measure(Ctx &c, double margin);
+static void planned_evaluation(Ctx &c, const double *d_pos, double *d_acc);
+static int read_sizes(Ctx &c);
+
+// capacities of a kept plan above the measured counts (particles move between steps)
+constexpr double kReuseMargin = 0.125;
+
+// One force evaluation of the context's boxes (set_boxes done).  The first one for a box
+// set is ordinary (it reads counts back to size its buffers) and plans the next ones;
+// those run planned, without a host round trip, and wait once at the end for the
+// overflow bits: particles that outgrew the plan are evaluated again the ordinary way
+// (and planned anew).  Both forms run the same kernels on the same counts, so they give
+// the same result bit for bit.
+static void force_eval(Ctx &c, const double *d_pos, double *d_acc)
+{
+    Reuse &ru = c.reuse;
+    if (ru.on && ru.have && ru.box_off == c.box_off && ru.box_len == c.box_len) {
+        c.plan = ru.plan;
+        c.plan.alloc = !ru.allocated;
+        try {
+            planned_evaluation(c, d_pos, d_acc);
+        } catch (...) {
+            c.plan = Plan();
+            ru.have = false;
+            throw;
+        }
+        ru.allocated = true;
+        c.plan = Plan();
+        if (read_sizes(c) == 0) return;
+        ru.have = false;  // outgrown: measured again below
+    }
+    prepare(c, d_pos);
+    evaluate(c, d_acc, true);
+    if (ru.on) {
+        plan_from_measure(c, kReuseMargin);
+        ru.plan = c.plan;
+        c.plan = Plan();
+        ru.have = true;
+        ru.allocated = false;
+        ru.box_off = c.box_off;
+        ru.box_len = c.box_len;
+    }
+}
+
 }  // namespace photons
 
 extern "C" {
@@ -394,8 +445,7 @@ int photons_force_device(photons_ctx *x, int64_t n, const double *d_pos, double 
             return 0;
         }
         single_box(c, n);
-        prepare(c, d_pos);
-        evaluate(c, d_acc, true);
+        force_eval(c, d_pos, d_acc);
         collect_timing(c);
         return 0;
     } catch (const std::exception &e) {
@@ -414,8 +464,7 @@ int photons_force_boxes_device(photons_ctx *x, int nbox, const int64_t *counts, 
         PH_CHECK(hipSetDevice(c.device));
         set_boxes(c, nbox, counts, lengths);
         c.n = c.box_off[nbox];
-        prepare(c, d_pos);
-        evaluate(c, d_acc, true);
+        force_eval(c, d_pos, d_acc);
         collect_timing(c);
         return 0;
     } catch (const std::exception &e) {
@@ -487,6 +536,10 @@ static void plan_from_measure(Ctx &c, double margin)
     p.tcap = (int)up(m.ntmp, 1024);
     p.lcap = (int)up(m.nleaf, 1024);
     p.dl_cap = up(m.dl, 1024);
+    // per depth; depths past the measured ones (a deeper tree) as the last measured one
+    p.rec_depth.assign(p.depths, 1024);
+    for (int d = 0; d < p.depths && !m.dl_depth.empty(); d++)
+        p.rec_depth[d] = std::min(p.dl_cap, up(m.dl_depth[std::min<size_t>(d, m.dl_depth.size() - 1)], 1024));
     p.p2p_cap = up(m.p2p, 4096);
     p.m2l_cap = up(m.m2l, 4096);
     p.item_cap = (int)up(m.items, 1024);
@@ -764,8 +817,7 @@ int photons_force_boxes_host(photons_ctx *x, int nbox, const int64_t *counts, co
         double *dp = reinterpret_cast<double *>(grow(x->io_pos, sizeof(double) * 3 * (size_t)n));
         double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)n));
         PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
-        prepare(c, dp);
-        evaluate(c, da, true);
+        force_eval(c, dp, da);
         PH_CHECK(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c.stream));
         PH_CHECK(hipStreamSynchronize(c.stream));
         collect_timing(c);
@@ -789,8 +841,7 @@ int photons_force_host(photons_ctx *x, int64_t n, const double *pos, double *acc
         double *dp = reinterpret_cast<double *>(grow(x->io_pos, sizeof(double) * 3 * (size_t)n));
         double *da = reinterpret_cast<double *>(grow(x->io_acc, sizeof(double) * 3 * (size_t)n));
         PH_CHECK(hipMemcpyAsync(dp, pos, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, c.stream));
-        prepare(c, dp);
-        evaluate(c, da, true);
+        force_eval(c, dp, da);
         PH_CHECK(hipMemcpyAsync(acc, da, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToHost, c.stream));
         PH_CHECK(hipStreamSynchronize(c.stream));
         collect_timing(c);

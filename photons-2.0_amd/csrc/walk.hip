@@ -184,7 +184,7 @@ constexpr int kWalkThreads = 64 * kTileWaves;
 // on (a target leaf beside a dense cluster opens one source node after another) is
 // continued from the rank it reached by a group of kGroupLanes lanes, that many ranks at
 // a time.
-constexpr int kThreadVisits = 32;
+constexpr int kThreadVisits = 64;
 constexpr int kGroupLanes = 16;
 
 // A pair's outputs as one 64-bit word: bit 0 clear, then down (19 bits), M2L (20), P2P
@@ -483,7 +483,7 @@ __device__ __forceinline__ int64_t npad_of(const WalkArgs &a)
 
 // the traversal passes' grid: persistent wavefronts, each with its own continuation slots
 constexpr int kTravWavesMax = 16384;
-constexpr int kContPerWave = 64;
+constexpr int kContPerWave = 256;
 
 // a pair's outputs: its packed counts, or its continuation's totals
 __device__ __forceinline__ void pair_counts(const WalkArgs &a, uint64_t w, uint32_t &p, uint32_t &m, uint32_t &d)
@@ -1103,9 +1103,12 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     int64_t *pin = reinterpret_cast<int64_t *>(c.pinned.p) + 16;
     int64_t nrec = n0, n_p2p = 0, n_m2l = 0;  // host copies (ordinary evaluation)
     int64_t max_rec = n0;
+    std::vector<int64_t> rec_depth;
     int depths = 0;
     for (int d = 0; d < ndepth && (planned || nrec > 0); d++) {
-        const int64_t rcap = planned ? c.plan.dl_cap : nrec;
+        // launch bound of this depth's records: the plan's (per depth), or the count
+        const int64_t rcap = planned ? c.plan.rec_depth[d] : nrec;
+        if (!planned) rec_depth.push_back(nrec);
         if (2 * (rcap + kTile) >= ((int64_t)1 << 31))
             throw std::runtime_error("more than 2^31 walk pairs at one depth");
         a.din = static_cast<const DownRec *>(din->p);
@@ -1156,7 +1159,8 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         if (planned) {
             p2p_cap = c.plan.p2p_cap;
             m2l_cap = c.plan.m2l_cap;
-            dl_cap = c.plan.dl_cap;
+            // the records written here are the next depth's, within its launch bound
+            dl_cap = d + 1 < ndepth ? std::min(c.plan.dl_cap, c.plan.rec_depth[d + 1]) : c.plan.dl_cap;
         } else {
             // the depth's totals size its outputs
             PH_CHECK(hipMemcpyAsync(pin, tile_off + ntile, sizeof(Cnt3), hipMemcpyDeviceToHost, st));
@@ -1205,6 +1209,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
             Measure &m = c.meas;
             m.depths = depths;
             m.dl = max_rec;
+            m.dl_depth = rec_depth;
             m.p2p = n_p2p;
             m.m2l = n_m2l;
         }

@@ -55,6 +55,7 @@ def lib():
         L.photons_destroy.argtypes = [vp]
         L.photons_set_params.argtypes = [vp, C.POINTER(Params)]
         L.photons_set_timing.argtypes = [vp, C.c_int]
+        L.photons_set_plan_reuse.argtypes = [vp, C.c_int]
         L.photons_get_phase_ms.argtypes = [vp, dp]
         L.photons_get_stats.argtypes = [vp, C.POINTER(Stats)]
         L.photons_force_host.argtypes = [vp, C.c_int64, dp, dp]
@@ -143,6 +144,10 @@ class Context:
 
     def set_timing(self, on=True):
         lib().photons_set_timing(self.h, 1 if on else 0)
+
+    def set_plan_reuse(self, on=True):
+        """later evaluations of the same boxes run on the first one's plan (default on)"""
+        lib().photons_set_plan_reuse(self.h, 1 if on else 0)
 
     def phase_ms(self):
         ms = np.zeros(len(PHASES))
