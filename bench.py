@@ -111,6 +111,65 @@ def _ref_rank(args):
         return len(pos), time.perf_counter() - t0
 
 
+def _headline_leaves(args):
+    """one core: the FP64 oracle (p = 4) on the 256^3 headline box, building the whole tree
+    and every multipole and evaluating only k seeded target leaves (walks, M2L, L2L, P2P,
+    L2P of those leaves), timed around the call"""
+    k, seed, nleaf = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    pos, prm = ic.workload_clustered(nside=256, order=4)
+    # the oracle's tree is the GPU's (same leaves, tested): leaf ids below its leaf count
+    sel = np.sort(np.random.default_rng(seed).choice(nleaf, k, replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    oracle.force_leaves(pos, prm, sel)
+    return k, time.perf_counter() - t0
+
+
+def cpu_headline(nleaf, cores):
+    """The CPU side of the headline workload itself: `cores` processes at once on the GPU
+    box's host, each the FP64 oracle (the reference's algorithm, p = 4, single threaded)
+    on the 256^3 box, half of them evaluating 24 sampled target leaves and half 96; the two
+    call times give the per-leaf cost and the tree + multipole cost, and the whole
+    evaluation on one core is modelled as tree + nleaf x per-leaf, spread over `cores`
+    ranks as the reference would (one rank per core, each its share of the leaves and of
+    the tree)."""
+    import multiprocessing as mp
+    ks = [24 if i % 2 == 0 else 96 for i in range(cores)]
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(cores) as pool:
+        t0 = time.perf_counter()
+        res = pool.map(_headline_leaves, [(k, 700 + i, nleaf) for i, k in enumerate(ks)])
+        wall = time.perf_counter() - t0
+    lo = np.mean([t for k, t in res if k == 24])
+    hi = np.mean([t for k, t in res if k == 96])
+    per_leaf = max((hi - lo) / 72.0, 1e-9)
+    t_tree = max(lo - 24 * per_leaf, 0.0)
+    t_one = t_tree + nleaf * per_leaf
+    n = 256 ** 3
+    return {"value": float(n / (t_one / cores)), "unit": "particles/s", "cores": cores, "kind": "port",
+            "sample": f"{cores} concurrent single-threaded processes of the FP64 oracle (p=4) on the 256^3 "
+                      f"headline box itself, each building the whole tree and multipoles and evaluating 24 or 96 "
+                      f"seeded target leaves ({wall:.1f} s wall); per leaf {per_leaf * 1e3:.1f} ms, tree and "
+                      f"multipoles {t_tree:.1f} s; the whole evaluation modelled as tree + {nleaf} leaves x per "
+                      f"leaf = {t_one:.0f} s on one core, over {cores} ranks",
+            "one_core": {"value": float(n / t_one), "seconds_modelled": float(t_one)}}
+
+
+def lattice32_one_core():
+    """The small CPU-runnable case on one core: 32^3 lattice + Gaussian displacements,
+    one FP64 force evaluation (the oracle, p = 4), timed"""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    pos, prm = ic.workload_lattice32(order=4)
+    t0 = time.perf_counter()
+    oracle.force(pos, prm)
+    t = time.perf_counter() - t0
+    return {"value": len(pos) / t, "unit": "particles/s", "cores": 1, "kind": "port", "seconds": t,
+            "sample": "32^3 lattice + Gaussian displacements (sigma 0.2 cell), periodic, p=4, theta 0.4, leaf 32, "
+                      "rc 6 rs: one FP64 force evaluation of the oracle on one core"}
+
+
 def cpu_baseline(nside=48, cores=None):
     """The reference CPU force path beside the GPU: the reference's own FMM sources compiled
     here (oracle/_ref/refharness, -O2, single threaded like one reference rank; its expansion
@@ -504,9 +563,14 @@ def main():
         out["fp32_over_fp64"] = out["value"] / out["fp64"]["value"]
         out["error_fp32_vs_fp64"] = {"p50": float(np.median(e)), "p99": float(np.percentile(e, 99)),
                                      "max": float(e.max())}
-    if not args.no_cpu_baseline:
-        log("cpu baseline (oracle: one rank, then one rank per core)")
-        out["cpu_baseline"] = cpu_baseline()
+    if not args.no_cpu_baseline and args.nside == 256:
+        cores = min(16, len(os.sched_getaffinity(0)))
+        log(f"cpu baseline: the oracle on the 256^3 box itself ({cores} processes)")
+        cb = cpu_headline(int(st["nleaf"]), cores)
+        log("cpu baseline: the reference's own force path on small boxes, and the 32^3 lattice on one core")
+        cb["reference_small_boxes"] = cpu_baseline(cores=cores)
+        cb["lattice32_one_core"] = lattice32_one_core()
+        out["cpu_baseline"] = cb
     print(json.dumps(out), flush=True)
 
 
