@@ -961,13 +961,15 @@ __global__ void k_walk_next(DevSizes *sz)
     sz->nrec = sz->dl_total;
 }
 
-// walk start: empty lists, bases at zero, n0 start records
+// walk start: empty lists, bases at zero, n0 start records -- none when the tree has
+// overflowed (a planned evaluation whose tree outgrew the plan wrote no cells and no walk
+// records, and its buffers may be fresh: nothing of them is read)
 __global__ void k_walk_begin(DevSizes *sz, int64_t n0)
 {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     sz->n_p2p = sz->n_m2l = sz->p2p_base = sz->m2l_base = 0;
     sz->dl_total = 0;
-    sz->nrec = n0;
+    sz->nrec = sz->overflow ? 0 : n0;
 }
 
 // after the last depth of a planned walk: records left over mean deeper targets that
@@ -1023,12 +1025,13 @@ __global__ void k_clear_spans(int n, int64_t *__restrict__ span)
 
 // the start records: every box's root against itself (box_root given) or the given pair
 // (target A, source B), one record per shift in [shift_lo, shift_hi)
-__global__ void k_start_list(DownRec *f, const CellInfo *__restrict__ info, const int *__restrict__ box_root, int nbox,
-                             int A, int B, int shift_lo, int shift_hi)
+__global__ void k_start_list(const DevSizes *__restrict__ sz, DownRec *f, const CellInfo *__restrict__ info,
+                             const int *__restrict__ box_root, int nbox, int A, int B, int shift_lo, int shift_hi)
 {
     const int nshift = shift_hi - shift_lo;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (box_root ? nbox : 1) * nshift) return;
+    // after a tree overflow the roots and cells were not written: the walk starts empty
+    if (sz->overflow || i >= (box_root ? nbox : 1) * nshift) return;
     const int s = shift_lo + i % nshift;
     const int tgt = box_root ? box_root[i / nshift] : A;
     const int src = box_root ? tgt : B;
@@ -1095,7 +1098,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
     DevBuf *din = &c.dl_a, *dout = &c.dl_b;
     DownRec *slist = static_cast<DownRec *>(ctx_buf(c, *din, sizeof(DownRec) * (size_t)(planned ? c.plan.dl_cap : n0),
                                                     "walk records"));
-    k_start_list<<<(int)((n0 + 255) / 256), 256, 0, st>>>(slist, dptr<CellInfo>(c.info),
+    k_start_list<<<(int)((n0 + 255) / 256), 256, 0, st>>>(sz, slist, dptr<CellInfo>(c.info),
                                                           forest ? dptr<int>(c.d_box_root) : nullptr, c.nbox,
                                                           start_target, start_source, shift_lo, shift_hi);
     k_walk_begin<<<1, 1, 0, st>>>(sz, n0);

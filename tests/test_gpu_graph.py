@@ -124,6 +124,28 @@ def test_outgrown_plan_is_reported(boxes):
     assert np.array_equal(ctx.force_boxes(counts, lengths, dense), want)
 
 
+@pytest.mark.parametrize("precision", [pg.FP64, pg.FP32_MIXED])
+def test_reused_plan_outgrown_by_the_tree(boxes, precision):
+    """Repeated evaluations of one box set run on the first one's plan (plan reuse); a
+    tree that outgrows it is measured again.  Clustered positions planned, then uniform
+    ones (more nodes per level than the plan holds), then the clustered ones under the
+    uniform plan (more levels): both trees overflow in the first planned evaluation of
+    their plan, whose buffers are new, and every result equals a fresh context's, bit for
+    bit (the tree and walk must read nothing of a tree that overflowed)."""
+    counts, lengths, pos, prm = boxes
+    rng = np.random.default_rng(3)
+    uni = np.concatenate([rng.uniform(0.0, lengths[b], (counts[b], 3)) for b in range(len(counts))])
+    ctx = pg.Context(prm, precision)
+    for k, p in enumerate((pos, uni, pos, pos)):
+        print(f"evaluation {k}: precision {precision}", flush=True)
+        got = ctx.force_boxes(counts, lengths, p)
+        fresh = pg.Context(prm, precision)
+        want = fresh.force_boxes(counts, lengths, p)
+        assert np.array_equal(got, want), k
+        for key in ("nleaf", "nnode", "p2p_pairs", "m2l_pairs"):
+            assert ctx.stats()[key] == fresh.stats()[key], (k, key)
+
+
 def test_small_box_latency():
     """one 16^3 clustered box: the replayed graph against the ordinary evaluation
     (host round trips per tree level and walk depth)"""
