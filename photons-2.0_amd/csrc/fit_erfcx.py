@@ -1,13 +1,14 @@
-"""Fits of erfcx(u) = exp(u^2) erfc(u) used by the pair kernels in fmm_math.h.
+"""Fits of erfcx(u) = exp(u^2) erfc(u) used by kernels in fmm_math.h.
 
-The split factor of the short-range force is G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2)
-= exp(-u^2) (erfcx(u) + 2u/sqrt(pi)), u = r / 2 rs; the kernels evaluate erfcx as a
-polynomial in t = 1 / (1 + u/2) (the variable of the Numerical Recipes erfcc form,
-Press et al., sec. 6.2), which maps u in [0, inf) onto t in (0, 1]:
+erfc(u) = exp(-u^2) erfcx(u); the kernels evaluate erfcx as t times a polynomial in
+t = 1 / (1 + u/2) (the variable of the Numerical Recipes erfcc form, Press et al.,
+sec. 6.2), which maps u in [0, inf) onto t in (0, 1]:
 
-  erfcx_tq    float   t * Q10(t),  u in [0, 1e4]   (the general float form)
-  erfcx_t6    float   P6(t),       u in [0, 6]     (the FP32 P2P pair force)
-  erfcx_tq20  double  t * Q20(t),  u >= 0          (the FP64 P2P pair force)
+  erfcx_tq    float   t * Q10(t),  u in [0, 1e4]   (the FP32 M2L radial factors)
+  erfcx_tq20  double  t * Q20(t),  u >= 0          (FP64 pair force beyond its table)
+
+(The P2P kernels read the split factor of the pair force from the tables that
+split_table.py writes, on u in [0, U); erfcx_tq20 gives the FP64 one from U on.)
 
 Each fit minimises the relative error: least squares on Chebyshev nodes in t, weights
 1/erfcx, then a few passes of iteratively reweighted least squares that push the
@@ -18,10 +19,10 @@ largest relative errors down (Lawson's algorithm), all in 50-digit arithmetic
 
 to print the coefficients (highest power first, as the Horner loops in fmm_math.h take
 them) and their errors; it takes about two minutes.  It prints fits of the same form
-with relative errors 1.8e-7 (erfcx_tq), 4.8e-6 (erfcx_t6) and 3.7e-14 (erfcx_tq20).  The
-coefficients fmm_math.h holds came from an earlier run of this procedure with other
-node sets and iteration counts (3.2e-7, 4.7e-6, 2.6e-14); tests/test_split_fits.py
-checks them against those bounds, evaluated as the kernels evaluate them.
+with relative errors 1.8e-7 (erfcx_tq) and 3.7e-14 (erfcx_tq20).  The coefficients
+fmm_math.h holds came from an earlier run of this procedure with other node sets and
+iteration counts (3.2e-7, 2.6e-14); tests/test_split_fits.py checks them against those
+bounds, evaluated as the kernels evaluate them.
 """
 from __future__ import annotations
 
@@ -97,7 +98,6 @@ def max_rel_error(coef, times_t, u_hi, dtype, npts=20000):
 
 def main():
     jobs = [("erfcx_tq", 10, True, t_of(mp.mpf(10000)), 1, np.float32, 1e4, 600, 12),
-            ("erfcx_t6", 6, False, t_of(mp.mpf(6)), 1, np.float32, 6.0, 600, 30),
             ("erfcx_tq20", 20, True, mp.mpf("1e-8"), 1, np.float64, 1e4, 1200, 30)]
     for name, deg, times_t, lo, hi, dtype, u_hi, nodes, passes in jobs:
         c = fit(deg, times_t, lo, hi, nodes, passes)

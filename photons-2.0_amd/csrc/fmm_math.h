@@ -199,35 +199,10 @@ __device__ __forceinline__ float erfcx_tq(float t)
     return t * q;
 }
 
-// erfcx(u) directly as a degree-6 polynomial in t = 1/(1 + u/2), no factor t: fitted
-// here (iteratively reweighted least squares on the relative error) over u in [0, 6],
-// t in [0.25, 1].  Relative error 4.7e-6 in float over that range; for u > 6 the
-// polynomial stays within [0, 0.1] and exp(-u^2) < 3e-16 takes the term to zero.
-// (t times a degree-7 polynomial is ten times more accurate and costs two more
-// instructions per interaction, which the FP32 pair force does not need.)
-__device__ __forceinline__ float erfcx_t6(float t)
-{
-    float q = 1.316356659e-02f;
-    q = __builtin_fmaf(q, t, -2.537588775e-01f);
-    q = __builtin_fmaf(q, t, 6.368674636e-01f);
-    q = __builtin_fmaf(q, t, -4.070590064e-02f);
-    q = __builtin_fmaf(q, t, 3.774936199e-01f);
-    q = __builtin_fmaf(q, t, 2.658120990e-01f);
-    q = __builtin_fmaf(q, t, 1.123473747e-03f);
-    return q;
-}
-
 // exp(-u^2) in float via the hardware base-2 exponential
 __device__ __forceinline__ float gauss_f32(float u)
 {
     return __builtin_amdgcn_exp2f(-1.4426950408889634f * u * u);
-}
-
-// G(u) = erfc(u) + 2/sqrt(pi) u exp(-u^2), the factor of the split pair force
-__device__ __forceinline__ float split_factor_f32(float u)
-{
-    float t = __builtin_amdgcn_rcpf(__builtin_fmaf(0.5f, u, 1.0f));
-    return gauss_f32(u) * __builtin_fmaf(1.1283791670955126f, u, erfcx_tq(t));
 }
 
 // erfcx(u) = t Q20(t), t = 1/(1+u/2), u >= 0: a least-squares fit of the relative
@@ -249,7 +224,9 @@ __device__ __forceinline__ double erfcx_tq20(double t)
 
 // G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2) = exp(-u^2) (erfcx(u) + 2u/sqrt(pi)): one exp
 // instead of the two inside erfc and beside it; t = 1/(1+u/2) by v_rcp_f64 and two
-// Newton steps.  Relative error ~3e-14 against the erfc/exp form.
+// Newton steps.  Relative error ~3e-14 against the erfc/exp form.  The FP64 P2P kernel
+// takes it beyond the end of its table (u >= 6), where the table's G is zero but the
+// reference's is not.
 __device__ __forceinline__ double split_factor_f64(double u)
 {
     const double x = fma(0.5, u, 1.0);

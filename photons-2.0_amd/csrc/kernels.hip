@@ -18,6 +18,7 @@
 
 #include "engine.h"
 #include "fmm_math.h"
+#include "split_table.h"
 
 namespace photons {
 
@@ -387,71 +388,66 @@ static void p2p_order(Ctx &c, int n)
     k_order_scatter<<<ntile, kOrderTile, 0, st>>>(n, sz, off, info, ntile, base, order);
 }
 
-// FP32 pair force per interaction, about 28 VALU instructions:
+// FP32 pair force per interaction, 20 VALU instructions and one LDS read:
 //   r2 = |d|^2 + 1e-30 (keeps rsqrt finite for coincident particles, whose d = 0)
 //   1/max(r, eps)^3 = min(rsqrt(r2), 1/eps)^3   (the reference's softening branch)
-//   G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2) = exp(-u^2) (erfcx(u) + 2u/sqrt(pi)),
-//          u = r/(2 rs), exp(-u^2) straight from r2, erfcx(u) = P6(t), t = 1/(1+u/2)
-// Three of them (rsqrt, rcp, exp2) go to the transcendental unit, which bounds the
-// loop more than the FMAs do: measured at 256^3, dropping two FMAs per pair changed
-// nothing, dropping the exp2 saved 12% of the P2P time.
-// Coordinates are in units of 2 rs / sqrt(log2 e) (scaled once when staged), so that
-// exp2(-r2) = exp(-u^2) needs no multiply; the particle mass and the factor the scaling
-// leaves on the sum (the square of the scale) multiply it once at the end.
+//   G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2) from the cubic table in LDS (split_table.h)
+// Only the rsqrt goes to the transcendental unit (the erfc/exp form took three: rsqrt,
+// rcp and exp2, and that unit paced the loop).  Coordinates are in table units
+// (u N / U per unit of r, scaled once when staged), so that r itself indexes the table;
+// the particle mass and the factor the scaling leaves on the sum (the square of the
+// scale) multiply it once at the end.
 struct PairConsts {
-    float ieps, inv4rs, c2u;
+    float ieps;  // 1/eps in table units, capped
+    float vmax;  // the table's last row: G = 0 from there on
 };
 
-// the scale and the constants of those units
-constexpr double kSqrtLog2e = 1.2011224087864498;  // sqrt(log2 e)
+using SplitRow32 = float4;
+constexpr int kSplitN32 = kSplitTab32N;
 
-__device__ __forceinline__ void pair_f32(const PairConsts &k, float dx, float dy, float dz, float &ax, float &ay,
-                                         float &az)
+__device__ __forceinline__ void pair_f32(const PairConsts &k, const SplitRow32 *tab, float dx, float dy, float dz,
+                                         float &ax, float &ay, float &az)
 {
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, 1e-30f)));
     const float ri = __builtin_amdgcn_rsqf(r2);
-    // min(ri, 1/eps) on the bit patterns: both are positive floats, whose order is
-    // that of their integer images (one v_min_i32, no NaN canonicalisation)
+    // min(ri, 1/eps) and min(r, vmax) on the bit patterns: all are non-negative floats,
+    // whose order is that of their integer images (one v_min_i32, no NaN canonicalisation)
     const float rc = __int_as_float(min(__float_as_int(ri), __float_as_int(k.ieps)));
-    const float r = r2 * ri;
-    const float t = __builtin_amdgcn_rcpf(__builtin_fmaf(r, k.inv4rs, 1.0f));
-    const float R = __builtin_fmaf(r, k.c2u, erfcx_t6(t));
-    const float e = __builtin_amdgcn_exp2f(-r2);
-    const float f = rc * rc * rc * (e * R);
-    ax = __builtin_fmaf(dx, f, ax);
-    ay = __builtin_fmaf(dy, f, ay);
-    az = __builtin_fmaf(dz, f, az);
+    const float v = __int_as_float(min(__float_as_int(r2 * ri), __float_as_int(k.vmax)));
+    const SplitRow32 c = tab[(int)v];
+    const float f = __builtin_amdgcn_fractf(v);
+    const float g = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(c.w, f, c.z), f, c.y), f, c.x);
+    const float fr = rc * rc * (rc * g);
+    ax = __builtin_fmaf(dx, fr, ax);
+    ay = __builtin_fmaf(dy, fr, ay);
+    az = __builtin_fmaf(dz, fr, az);
 }
 
-// Two interactions of one target with a shared reciprocal: 1/d1 = d2/(d1 d2) and
-// 1/d2 = d1/(d1 d2) from one v_rcp_f32 (d in [1, ~50], so the product stays far from
-// overflow; about 2 ulp in t).  The transcendental unit, not the FMAs, bounds this loop,
-// so trading one of its six operations for three multiplies pays.
-__device__ __forceinline__ void pair2_f32(const PairConsts &k, float dx, float dy, float dz, float ex, float ey,
-                                          float ez, float &ax, float &ay, float &az)
+// Two interactions of one target, their steps interleaved so that each one's LDS
+// table read is in flight while the other computes.
+__device__ __forceinline__ void pair2_f32(const PairConsts &k, const SplitRow32 *tab, float dx, float dy, float dz,
+                                          float ex, float ey, float ez, float &ax, float &ay, float &az)
 {
     const float r2 = __builtin_fmaf(dx, dx, __builtin_fmaf(dy, dy, __builtin_fmaf(dz, dz, 1e-30f)));
     const float s2 = __builtin_fmaf(ex, ex, __builtin_fmaf(ey, ey, __builtin_fmaf(ez, ez, 1e-30f)));
     const float ri = __builtin_amdgcn_rsqf(r2);
     const float si = __builtin_amdgcn_rsqf(s2);
+    const float vr = __int_as_float(min(__float_as_int(r2 * ri), __float_as_int(k.vmax)));
+    const float vs = __int_as_float(min(__float_as_int(s2 * si), __float_as_int(k.vmax)));
+    const SplitRow32 cr = tab[(int)vr];
+    const SplitRow32 cs = tab[(int)vs];
     const float rc = __int_as_float(min(__float_as_int(ri), __float_as_int(k.ieps)));
     const float sc = __int_as_float(min(__float_as_int(si), __float_as_int(k.ieps)));
-    const float r = r2 * ri;
-    const float s = s2 * si;
-    const float dr = __builtin_fmaf(r, k.inv4rs, 1.0f);
-    const float ds = __builtin_fmaf(s, k.inv4rs, 1.0f);
-    const float q = __builtin_amdgcn_rcpf(dr * ds);
-    const float tr = ds * q;
-    const float ts = dr * q;
-    const float Rr = __builtin_fmaf(r, k.c2u, erfcx_t6(tr));
-    const float Rs = __builtin_fmaf(s, k.c2u, erfcx_t6(ts));
-    const float er = __builtin_amdgcn_exp2f(-r2);
-    const float es = __builtin_amdgcn_exp2f(-s2);
-    const float fr = rc * rc * rc * (er * Rr);
-    const float fs = sc * sc * sc * (es * Rs);
-    ax = __builtin_fmaf(ex, fs, __builtin_fmaf(dx, fr, ax));
-    ay = __builtin_fmaf(ey, fs, __builtin_fmaf(dy, fr, ay));
-    az = __builtin_fmaf(ez, fs, __builtin_fmaf(dz, fr, az));
+    const float fr = __builtin_amdgcn_fractf(vr);
+    const float fs = __builtin_amdgcn_fractf(vs);
+    const float rc2 = rc * rc, sc2 = sc * sc;
+    const float gr = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cr.w, fr, cr.z), fr, cr.y), fr, cr.x);
+    const float gs = __builtin_fmaf(__builtin_fmaf(__builtin_fmaf(cs.w, fs, cs.z), fs, cs.y), fs, cs.x);
+    const float qr = rc2 * (rc * gr);
+    const float qs = sc2 * (sc * gs);
+    ax = __builtin_fmaf(ex, qs, __builtin_fmaf(dx, qr, ax));
+    ay = __builtin_fmaf(ey, qs, __builtin_fmaf(dy, qr, ay));
+    az = __builtin_fmaf(ez, qs, __builtin_fmaf(dz, qr, az));
 }
 
 // Source particles staged in LDS per batch, as float offsets from the target leaf's
@@ -508,14 +504,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     float *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ float ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
+    __shared__ SplitRow32 tab[kSplitN32 + 1];
     const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
     const int lane = threadIdx.x;
     if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
     if (n == 0) return;
+    // the split-factor table into LDS (read after the first barrier below)
+    for (int j = lane; j <= kSplitN32; j += 64) tab[j] = reinterpret_cast<const SplitRow32 *>(kSplitTab32)[j];
     const LaneMap L = lane_map(n, lane);
-    const double lam_d = kSqrtLog2e * a.inv2rs;  // length -> pair-force units
+    const double lam_d = a.inv2rs * (kSplitTab32N / kSplitTab32U);  // length -> table units
     const float lam = (float)lam_d;
     const float txA = a.ox[IT.ipart + L.tA] * lam, tyA = a.oy[IT.ipart + L.tA] * lam,
                 tzA = a.oz[IT.ipart + L.tA] * lam;
@@ -528,8 +527,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     // with itself (r2 = 1e-30, d = 0) then adds 0 * finite = 0, as the reference's skip
     // of that pair does; the cap only touches pairs closer than 1e-12 units
     pc.ieps = (float)fmin(1.0 / (a.soft * lam_d), 1e12);
-    pc.inv4rs = (float)(0.5 / kSqrtLog2e);                // u/2 per unit length
-    pc.c2u = (float)(1.1283791670955126 / kSqrtLog2e);   // 2/sqrt(pi) u per unit length
+    pc.vmax = (float)kSplitN32;
     float axA = 0.f, ayA = 0.f, azA = 0.f, axB = 0.f, ayB = 0.f, azB = 0.f;
     const int64_t e1 = span_end(a.off, T);
     for (int64_t base = span_beg(a.off, T); base < e1;) {
@@ -597,18 +595,18 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             const float *p = sx + min(L.kA, tot);  // idle lanes: start at the end
             const float *const pend = sx + max(tot - st, 0);
             for (; p < pend; p += 2 * st)
-                pair2_f32(pc, p[0] - txA, p[STAGE] - tyA, p[2 * STAGE] - tzA, p[st] - txA, p[st + STAGE] - tyA,
-                          p[st + 2 * STAGE] - tzA, axA, ayA, azA);
-            if (p < sx + tot) pair_f32(pc, p[0] - txA, p[STAGE] - tyA, p[2 * STAGE] - tzA, axA, ayA, azA);
+                pair2_f32(pc, tab, p[0] - txA, p[STAGE] - tyA, p[2 * STAGE] - tzA, p[st] - txA,
+                          p[st + STAGE] - tyA, p[st + 2 * STAGE] - tzA, axA, ayA, azA);
+            if (p < sx + tot) pair_f32(pc, tab, p[0] - txA, p[STAGE] - tyA, p[2 * STAGE] - tzA, axA, ayA, azA);
         }
         if (L.e) {
             const int st = L.sB;
             const float *p = sx + min(L.kB, tot);  // idle lanes: start at the end
             const float *const pend = sx + max(tot - st, 0);
             for (; p < pend; p += 2 * st)
-                pair2_f32(pc, p[0] - txB, p[STAGE] - tyB, p[2 * STAGE] - tzB, p[st] - txB, p[st + STAGE] - tyB,
-                          p[st + 2 * STAGE] - tzB, axB, ayB, azB);
-            if (p < sx + tot) pair_f32(pc, p[0] - txB, p[STAGE] - tyB, p[2 * STAGE] - tzB, axB, ayB, azB);
+                pair2_f32(pc, tab, p[0] - txB, p[STAGE] - tyB, p[2 * STAGE] - tzB, p[st] - txB,
+                          p[st + STAGE] - tyB, p[st + 2 * STAGE] - tzB, axB, ayB, azB);
+            if (p < sx + tot) pair_f32(pc, tab, p[0] - txB, p[STAGE] - tyB, p[2 * STAGE] - tzB, axB, ayB, azB);
         }
         base += ne;
         __syncthreads();  // the staging arrays are rewritten by the next batch
@@ -665,9 +663,32 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
 // FP64 pair force as the reference forms it (src/fmm.c:834-852): d = x_s - x_t,
 // r = |d|, m/eps^3 when r < eps else m/r^3, times G(r/2rs).  1/r comes from
 // v_rsq_f64 and two Newton steps (r = r^2 / r, m/r^3 = m (1/r)^3): the same values
-// to an ulp, without the IEEE square root and divide sequences.
-__device__ __forceinline__ void pair_f64(double dx, double dy, double dz, double m, double eps, double meps3,
-                                         double inv2rs, double &ax, double &ay, double &az)
+// to an ulp, without the IEEE square root and divide sequences.  G comes from the
+// degree-9 table in LDS (split_table.h, 2e-15 from erfc/exp), read at
+// v = min(r * vscale, N) with vscale = N / (2 rs U); pairs beyond the table (u >= 6,
+// G < 1.6e-15: rare, in sparse regions) take the erfc/exp form, so that G stays
+// relatively exact where the reference's is.
+using SplitRow64 = double2;  // a row of the table: five of these
+constexpr int kSplitN64 = kSplitTab64N;
+constexpr int kSplitW64 = (kSplitTab64Deg + 1) / 2;
+static_assert(kSplitTab64Deg % 2 == 1, "rows of an even number of coefficients");
+
+__device__ __forceinline__ double split_table_f64(const SplitRow64 *tab, double v)
+{
+    const SplitRow64 *row = tab + kSplitW64 * (int)v;
+    SplitRow64 c[kSplitW64];
+#pragma unroll
+    for (int k = 0; k < kSplitW64; k++) c[k] = row[k];
+    const double f = __builtin_amdgcn_fract(v);
+    double g = fma(c[kSplitW64 - 1].y, f, c[kSplitW64 - 1].x);
+#pragma unroll
+    for (int k = kSplitW64 - 2; k >= 0; k--) g = fma(fma(g, f, c[k].y), f, c[k].x);
+    return g;
+}
+
+__device__ __forceinline__ void pair_f64(const SplitRow64 *tab, double dx, double dy, double dz, double m, double eps,
+                                         double meps3, double vscale, double inv2rs, double &ax, double &ay,
+                                         double &az)
 {
     const double r2 = dx * dx + dy * dy + dz * dz;
     // floored so that coincident particles (d = 0) keep a finite 1/r and take the
@@ -681,7 +702,10 @@ __device__ __forceinline__ void pair_f64(double dx, double dy, double dz, double
     // a particle with itself (d = 0) adds exactly nothing, also with eps = 0, where the
     // reference skips the pair (src/fmm.c:829) and 1/r^3 would overflow
     double f = r2 == 0.0 ? 0.0 : (dr < eps ? meps3 : m * (ri * ri * ri));
-    f *= split_factor_f64(dr * inv2rs);
+    const double v = dr * vscale;
+    double g = split_table_f64(tab, fmin(v, (double)kSplitN64));
+    if (v >= (double)kSplitN64) g = split_factor_f64(dr * inv2rs);
+    f *= g;
     ax += dx * f;
     ay += dy * f;
     az += dz * f;
@@ -698,16 +722,21 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     double *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ double ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
+    __shared__ SplitRow64 tab[kSplitW64 * (kSplitN64 + 1)];
     const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
     const int lane = threadIdx.x;
     if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
     const int n = IT.npart;
     if (n == 0) return;
+    // the split-factor table into LDS (read after the first barrier below)
+    for (int j = lane; j < kSplitW64 * (kSplitN64 + 1); j += 64)
+        tab[j] = reinterpret_cast<const SplitRow64 *>(kSplitTab64)[j];
     const LaneMap L = lane_map(n, lane);
     const double xA = a.px[IT.ipart + L.tA], yA = a.py[IT.ipart + L.tA], zA = a.pz[IT.ipart + L.tA];
     const double xB = a.px[IT.ipart + L.tB], yB = a.py[IT.ipart + L.tB], zB = a.pz[IT.ipart + L.tB];
-    const double m = a.mass, eps = a.soft, meps3 = eps > 0.0 ? m / (eps * eps * eps) : 0.0, inv2rs = a.inv2rs;
+    const double m = a.mass, eps = a.soft, meps3 = eps > 0.0 ? m / (eps * eps * eps) : 0.0;
+    const double inv2rs = a.inv2rs, vscale = inv2rs * (kSplitTab64N / kSplitTab64U);
     const double boxlen = a.bx.len_of(T);
     double axA = 0.0, ayA = 0.0, azA = 0.0, axB = 0.0, ayB = 0.0, azB = 0.0;
     const int64_t e1 = span_end(a.off, T);
@@ -751,10 +780,10 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
         }
         __syncthreads();
         for (int q = L.kA; q < tot; q += L.sA)
-            pair_f64(sx[q] - xA, sy[q] - yA, sz[q] - zA, m, eps, meps3, inv2rs, axA, ayA, azA);
+            pair_f64(tab, sx[q] - xA, sy[q] - yA, sz[q] - zA, m, eps, meps3, vscale, inv2rs, axA, ayA, azA);
         if (L.e)
             for (int q = L.kB; q < tot; q += L.sB)
-                pair_f64(sx[q] - xB, sy[q] - yB, sz[q] - zB, m, eps, meps3, inv2rs, axB, ayB, azB);
+                pair_f64(tab, sx[q] - xB, sy[q] - yB, sz[q] - zB, m, eps, meps3, vscale, inv2rs, axB, ayB, azB);
         base += ne;
         __syncthreads();  // the staging and entry arrays are rewritten by the next batch
     }

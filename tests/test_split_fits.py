@@ -1,15 +1,17 @@
-"""The erfcx fits of the pair kernels (photons-2.0_amd/csrc/fmm_math.h) against their
-stated error bounds.
+"""The split-factor evaluations of the pair kernels against their stated error bounds.
 
-The split factor G(u) = exp(-u^2) (erfcx(u) + 2u/sqrt(pi)) of the short-range force
-(src/fmm.c:834-852 evaluates it with erfc and exp) is computed on the GPU from
-polynomial fits of erfcx in t = 1/(1 + u/2), written by photons-2.0_amd/csrc/
-fit_erfcx.py.  The coefficients are read from the kernel source and evaluated as the
-kernels evaluate them (Horner, one fused multiply-add per step, rounded to the kernel's
-precision), against erfcx from mpmath in 30-digit arithmetic.
+The split factor of the short-range pair force, G(u) = erfc(u) + 2u/sqrt(pi) exp(-u^2),
+u = r/2rs (src/fmm.c:845-847 evaluates it with erfc and exp), is read by the P2P
+kernels from piecewise-polynomial tables (photons-2.0_amd/csrc/split_table.h, written
+by split_table.py in the same directory); the FP32 M2L radial factors use erfc through a
+fit of erfcx in t = 1/(1 + u/2) (fmm_math.h, from fit_erfcx.py).  The coefficients are
+read from the sources and evaluated as the kernels evaluate them (Horner, one fused
+multiply-add per step, rounded to the kernel's precision), against mpmath in 30- to
+40-digit arithmetic.
 """
 import os
 import re
+import sys
 
 import mpmath as mp
 import numpy as np
@@ -81,22 +83,10 @@ def test_float_erfcx_tq_bound():
     assert e <= 3.2e-7
 
 
-def test_float_erfcx_t6_bound():
-    """P6(t) of the FP32 P2P kernel: relative error <= 4.7e-6 in float over u in
-    [0, 6]; beyond, the polynomial stays within [0, 0.1] (exp(-u^2) < 3e-16 there)"""
-    c = float_coefficients("erfcx_t6")
-    assert len(c) == 7
-    e = max_rel(c, False, 6.0, np.float32, 3000)
-    print(f"erfcx_t6: {e:.2e}")
-    assert e <= 4.7e-6
-    for u in np.geomspace(6.0, 1e4, 200):
-        v = float(horner(c, np.float32(1.0 / (1.0 + 0.5 * u)), np.float32))
-        assert 0.0 <= v <= 0.1
-
-
 def test_double_erfcx_tq20_bound():
-    """t * Q20(t) of the FP64 P2P kernel: relative error <= 2.6e-14 in double over u >= 0
-    (checked up to 1e4, where t = 2e-4 and t Q20(t) follows the 1/(u sqrt(pi)) tail)"""
+    """t * Q20(t), the FP64 pair force beyond its table: relative error <= 2.6e-14 in
+    double over u >= 0 (checked up to 1e4, where t = 2e-4 and t Q20(t) follows the
+    1/(u sqrt(pi)) tail)"""
     c = double_coefficients("erfcx_tq20")
     assert len(c) == 21
     e = max_rel(c, True, 1e4, np.float64)
@@ -104,10 +94,90 @@ def test_double_erfcx_tq20_bound():
     assert e <= 2.65e-14  # 2.6e-14 to two digits
 
 
-@pytest.mark.parametrize("name", ["erfcx_tq", "erfcx_t6"])
-def test_a_wrong_coefficient_is_caught(name):
+def test_a_wrong_coefficient_is_caught():
     """the check has teeth: the constant term perturbed by 1e-4 of itself fails"""
-    c = float_coefficients(name)
+    c = float_coefficients("erfcx_tq")
     c[-1] = np.float32(c[-1] * (1 + 1e-4))
-    bound = 3.2e-7 if name == "erfcx_tq" else 4.7e-6
-    assert max_rel(c, name == "erfcx_tq", 6.0, np.float32, 300) > bound
+    assert max_rel(c, True, 6.0, np.float32, 300) > 3.2e-7
+
+
+# ---------------------------------------------------------------- split-factor tables
+
+TABLE_SRC = os.path.join(os.path.dirname(SRC), "split_table.h")
+
+
+def table_rows(name):
+    """the rows of a table in split_table.h, and its N and U"""
+    text = open(TABLE_SRC).read()
+    n = int(re.search(name + r"N = (\d+);", text).group(1))
+    u = float(re.search(name + r"U = ([0-9.]+);", text).group(1))
+    i = text.index("{", text.index(name + "["))
+    j = text.index("};", i)
+    rows = [[float(x.rstrip("f")) for x in re.findall(r"[-+]?\d\.\d+e[-+]\d+f?", r)]
+            for r in text[i:j].split("},")[:-1]]
+    return rows, n, u
+
+
+def G(u):
+    u = mp.mpf(u)
+    return mp.erfc(u) + 2 * u / mp.sqrt(mp.pi) * mp.exp(-u * u)
+
+
+def table_error(rows, n, umax, dtype, npts=8000):
+    """largest |table - G| and largest relative error for u < 3, at the u the kernel
+    indexes with (v = u N / U rounded to dtype, clamped to N), evaluated as the kernel
+    does: i = floor(v), f = v - i, Horner with one rounding per step"""
+    R = np.array(rows, dtype=dtype)
+    worst, worst_rel = 0.0, 0.0
+    with mp.workdps(40):
+        for u in np.linspace(0.0, 1.1 * umax, npts):
+            v = min(dtype(dtype(u) * dtype(n / umax)), dtype(n))
+            i = int(v)
+            f = dtype(v - dtype(i))
+            c = R[i]
+            g = c[-1]
+            for k in range(len(c) - 2, -1, -1):
+                g = dtype(np.longdouble(g) * np.longdouble(f) + np.longdouble(c[k]))
+            ref = G(float(v) * umax / n) if v < n else G(u)
+            e = abs(float(mp.mpf(float(g)) - ref))
+            worst = max(worst, e)
+            if u < 3.0:
+                worst_rel = max(worst_rel, e / float(ref))
+    return worst, worst_rel
+
+
+def test_split_table_header_is_the_generators_output():
+    """split_table.h is what split_table.py writes (no hand edits, no stale table)"""
+    sys.path.insert(0, os.path.dirname(SRC))
+    import split_table
+    with mp.workdps(40):
+        assert split_table.render() == open(TABLE_SRC).read()
+
+
+def test_fp32_split_table_bound():
+    """the FP32 P2P table (cubic, 256 intervals on u in [0, 9.5)): |error| <= 6.5e-8 (G <= 1),
+    relative <= 6e-7 for u < 3, zero from u = 9.5 on (G(9.5) = 6e-39, below float's
+    normal range)"""
+    rows, n, umax = table_rows("kSplitTab32")
+    assert (len(rows), len(rows[0])) == (n + 1, 4) and rows[n] == [0.0] * 4
+    e, er = table_error(rows, n, umax, np.float32)
+    print(f"FP32 table: max |error| {e:.2e}, relative (u < 3) {er:.2e}")
+    assert e <= 6.5e-8 and er <= 6e-7
+
+
+def test_fp64_split_table_bound():
+    """the FP64 P2P table (degree 9, 48 intervals on u in [0, 6)): |error| <= 2e-15,
+    relative <= 4e-15 for u < 3, zero from u = 6 on (G(6) = 1.6e-15)"""
+    rows, n, umax = table_rows("kSplitTab64")
+    assert (len(rows), len(rows[0])) == (n + 1, 10) and rows[n] == [0.0] * 10
+    e, er = table_error(rows, n, umax, np.float64)
+    print(f"FP64 table: max |error| {e:.2e}, relative (u < 3) {er:.2e}")
+    assert e <= 2e-15 and er <= 4e-15
+
+
+def test_a_wrong_table_entry_is_caught():
+    """the check has teeth: one coefficient of one FP32 row off by 1e-6 fails"""
+    rows, n, umax = table_rows("kSplitTab32")
+    rows[20][1] += 1e-6
+    e, _ = table_error(rows, n, umax, np.float32, 3000)
+    assert e > 6.5e-8
