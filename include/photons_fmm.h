@@ -38,7 +38,7 @@ typedef struct {
     double rs;        /* splitRadius                                          */
     double rc;        /* cutoffRadius                                         */
     double theta;     /* open_angle                                           */
-    int32_t maxleaf;  /* MAXLEAF (>= 2)                                       */
+    int32_t maxleaf;  /* MAXLEAF, MaxPackage of the parameter file (2 .. 256)   */
     int32_t order;    /* expansion order p: 3 (reference), 4 or 5             */
     int32_t periodic; /* 1: the 26 image walks of fmm_ext (PERIODIC_CONDITION)*/
     int32_t precision;/* PHOTONS_FP64 or PHOTONS_FP32_MIXED                   */
@@ -117,9 +117,9 @@ int photons_force_boxes_device(photons_ctx *ctx, int nbox, const int64_t *counts
  * threads) race.                                                              */
 int photons_force_device_static(photons_ctx *ctx, int64_t n, const double *d_pos, double *d_acc);
 
-/* Tuning (no counterpart in the reference, whose MAXLEAF is a compile-time
- * constant, inc/photoNs.h).  The FP32-mixed P2P kernel stages 256 or 512 source
- * particles per batch in LDS (default 512).  photons_autotune_boxes times leaf
+/* Tuning (no counterpart in the reference, which takes MAXLEAF from its parameter
+ * file, src/initial.c:172-173, untuned).  The FP32-mixed P2P kernel stages 256 or
+ * 512 source particles per batch in LDS (default 256).  photons_autotune_boxes times leaf
  * sizes 16/32/64 x staging 256/512 (staging only for FP32-mixed) on the
  * caller's batch, keeps the fastest in the context and leaves its result in
  * d_acc; ms[6] (or NULL) gets the mean ms per evaluation of each candidate,

@@ -137,7 +137,7 @@ struct Ctx {
     int64_t n = 0;
     int nleaf = 0, nnode = 0, ncell = 0, nlevel = 0, maxdepth = 0;
     int nbox = 1;
-    int p2p_stage = 512;           // FP32 P2P staging slots (256 or 512; tuned)
+    int p2p_stage = 256;           // FP32 P2P staging slots (256 or 512; tuned)
     std::vector<int64_t> box_off;  // particle offsets of the boxes (nbox + 1)
     std::vector<double> box_len;   // box lengths
     DevBuf d_box_off, d_box_len;   // device copies

@@ -191,7 +191,9 @@ static std::string check_params(const photons_params *p)
     if (!(p->rs > 0.0) || !(p->rc > 0.0)) return "rs and rc must be positive";
     if (!(p->soft >= 0.0)) return "soft must be non-negative";
     if (!(p->theta > 0.0)) return "theta must be positive";
-    if (p->maxleaf < 2 || p->maxleaf > 64) return "maxleaf must be in [2, 64]";
+    // a source leaf must fit one staging batch of the P2P kernels (256 particles); target
+    // leaves of more than 64 run in chunks of 64
+    if (p->maxleaf < 2 || p->maxleaf > 256) return "maxleaf must be in [2, 256]";
     if (p->order < 3 || p->order > 5) return "order must be 3, 4 or 5";
     if (p->precision != PHOTONS_FP64 && p->precision != PHOTONS_FP32_MIXED) return "unknown precision";
     return "";

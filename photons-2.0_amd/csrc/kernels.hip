@@ -7,10 +7,12 @@
 // stays inside the float range:
 //   M'_a = M_a / rs^|a|,  L_a = L'_a / rs^(|a|+1),  acc = rs^-2 * grad L'(d/rs).
 //
-// P2P: one wavefront per target leaf.  Lane l handles target l mod nT and every
-// nsub-th source particle (nsub = 64 / nT), so a 22-particle leaf keeps 44 lanes
-// busy; each source leaf is staged in LDS once for the whole wavefront, and the
-// per-lane partial sums are combined in a fixed order (deterministic results).
+// P2P: one wavefront per target leaf (per chunk of 64 of its particles for leaves of
+// more than 64).  Its source list is read in batches whose particles are staged in LDS
+// once for the whole wavefront; every (target, source) pair of a batch goes to one lane
+// by a two-pass layout that keeps all 64 lanes busy for any leaf size (LaneMap below);
+// the per-lane partial sums are combined in a fixed order (deterministic results).  The
+// split factor of the pair force comes from a table in LDS (split_table.h).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -22,7 +24,8 @@
 
 namespace photons {
 
-constexpr int kP2PMaxLeaf = 64;
+// a source leaf must fit one staging batch (the smallest stage holds 256 particles)
+constexpr int kP2PMaxLeaf = 256;
 
 // ---------------------------------------------------------------------------- upward
 
@@ -260,6 +263,7 @@ struct P2PArgs {
     double *out;          // [n][3]
     const int32_t *order; // target leaf of each launch slot (heaviest first; past nleaf: none)
     const DevSizes *sz;
+    int nchunk;           // workgroups per target leaf: chunks of 64 targets (maxleaf / 64 up)
 };
 
 __device__ __forceinline__ void shift_vec(int sid, double L, double &sx, double &sy, double &sz)
@@ -489,37 +493,41 @@ __device__ __forceinline__ LaneMap lane_map(int n, int lane)
 // entries are read at once, one per lane (entry -> cell -> centre offset, formed in
 // double), a wave prefix sum of their particle counts picks the leading entries that
 // fit STAGE particles, and their particles are staged in LDS as float offsets
-// from the target centre, all loads of a batch issued together.  The pairs of the
-// batch are then spread over the lanes by the two-pass layout above.  Partial sums
-// are combined per target in a fixed lane order, so results are deterministic.
-// amdgpu_waves_per_eu(6) caps the kernel at 80 VGPRs (one 8-byte value spills, outside
-// the pair loops).  Occupancy stays bounded by LDS either way (19 workgroups of 8.4 KB per
-// 160 KB CU with STAGE 512); the tighter allocation schedules the pair loop better
-// (measured 6 ms faster at 256^3 than the 95-VGPR build).
+// from the target centre, all loads of a batch issued together.  The batch's entries
+// stay in registers (the staging lanes read them by shuffle), so LDS holds only the
+// staged particles and the split-factor table.  The pairs of the batch are then spread
+// over the lanes by the two-pass layout above.  Partial sums are combined per target in a
+// fixed lane order, so results are deterministic.
+// Occupancy: amdgpu_waves_per_eu(6) caps the kernel at 80 VGPRs; with STAGE 256 a
+// workgroup holds 7.2 KB of LDS (22 per 160 KB CU, 5.5 waves per SIMD), with STAGE 512
+// 10.3 KB (15, 3.75 waves).  Measured at 256^3: 160 ms at STAGE 256 against 172 ms at
+// 512 (and 165 ms at 192, 167 ms at 128: more batches per target), so 256 is the default.
 template <int P, int STAGE>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_p2p_f32(P2PArgs a)
 {
-    static_assert(STAGE >= 128 && STAGE % 64 == 0, "stage size");
+    static_assert(STAGE >= kP2PMaxLeaf && STAGE % 64 == 0, "stage size: a source leaf fits one batch");
     __shared__ float stage[3 * STAGE];
     float *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
-    __shared__ float ecx[64], ecy[64], ecz[64];
-    __shared__ int eincl[64], ecnt[64], eip[64];
     __shared__ SplitRow32 tab[kSplitN32 + 1];
-    const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
+    // a.nchunk consecutive workgroups per target leaf, one per chunk of 64 of its
+    // particles (one chunk for leaves of up to 64)
+    const int chunk = blockIdx.x % a.nchunk;
+    const int T = a.order[p2p_target(blockIdx.x / a.nchunk, gridDim.x / a.nchunk)];
     const int lane = threadIdx.x;
     if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
-    const int n = IT.npart;
-    if (n == 0) return;
+    const int c0 = 64 * chunk;
+    if (c0 >= IT.npart) return;
+    const int n = min(64, IT.npart - c0), i0 = IT.ipart + c0;  // this chunk's targets
     // the split-factor table into LDS (read after the first barrier below)
     for (int j = lane; j <= kSplitN32; j += 64) tab[j] = reinterpret_cast<const SplitRow32 *>(kSplitTab32)[j];
     const LaneMap L = lane_map(n, lane);
     const double lam_d = a.inv2rs * (kSplitTab32N / kSplitTab32U);  // length -> table units
     const float lam = (float)lam_d;
-    const float txA = a.ox[IT.ipart + L.tA] * lam, tyA = a.oy[IT.ipart + L.tA] * lam,
-                tzA = a.oz[IT.ipart + L.tA] * lam;
-    const float txB = a.ox[IT.ipart + L.tB] * lam, tyB = a.oy[IT.ipart + L.tB] * lam,
-                tzB = a.oz[IT.ipart + L.tB] * lam;
+    const float txA = a.ox[i0 + L.tA] * lam, tyA = a.oy[i0 + L.tA] * lam,
+                tzA = a.oz[i0 + L.tA] * lam;
+    const float txB = a.ox[i0 + L.tB] * lam, tyB = a.oy[i0 + L.tB] * lam,
+                tzB = a.oz[i0 + L.tB] * lam;
     const CellGeom gT = a.geom[T];
     const double boxlen = a.bx.len_of(T);
     PairConsts pc;
@@ -532,7 +540,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     const int64_t e1 = span_end(a.off, T);
     for (int64_t base = span_beg(a.off, T); base < e1;) {
         const int nb = (int)min((int64_t)64, e1 - base);
-        int cnt = 0;
+        // entry `lane` of the batch: its centre offset, its particle count and the slot
+        // offset of its first particle (in registers; other lanes read them by shuffle)
+        int cnt = 0, first = 0;
+        float ecx = 0.f, ecy = 0.f, ecz = 0.f;
         if (lane < nb) {
             const int32_t s = a.src[base + lane];
             const int S = entry_cell(s), sid = entry_shift(s);
@@ -540,10 +551,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             const CellGeom gS = a.geom[S];
             double shx, shy, shz;
             shift_vec(sid, boxlen, shx, shy, shz);
-            ecx[lane] = (float)(((gS.c[0] + shx) - gT.c[0]) * lam_d);
-            ecy[lane] = (float)(((gS.c[1] + shy) - gT.c[1]) * lam_d);
-            ecz[lane] = (float)(((gS.c[2] + shz) - gT.c[2]) * lam_d);
-            eip[lane] = IS.ipart;
+            ecx = (float)(((gS.c[0] + shx) - gT.c[0]) * lam_d);
+            ecy = (float)(((gS.c[1] + shy) - gT.c[1]) * lam_d);
+            ecz = (float)(((gS.c[2] + shz) - gT.c[2]) * lam_d);
+            first = IS.ipart;
             cnt = IS.npart;
         }
         int incl = cnt;
@@ -552,13 +563,15 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             const int v = __shfl_up(incl, d, 64);
             if (lane >= d) incl += v;
         }
-        // the counts are at most 64 <= STAGE, so at least one entry fits
+        // the counts are at most maxleaf <= STAGE, so at least one entry fits (the break
+        // only keeps the loop finite if that were ever broken)
         const int ne = __popcll(__ballot(lane < nb && incl <= STAGE));
+        if (ne == 0) break;
         const int tot = __shfl(incl, ne - 1, 64);
-        ecnt[lane] = cnt;
-        eincl[lane] = lane < ne ? incl : 0x7fffffff;
-        __syncthreads();
-        // stage: slot p belongs to the first entry whose inclusive count exceeds p
+        const int eincl = lane < ne ? incl : 0x7fffffff;
+        const int ebase = first - (incl - cnt);  // slot p of this entry is particle ebase + p
+        // stage: slot p belongs to the first entry whose inclusive count exceeds p (the
+        // shuffles run in every lane; only the loads depend on p < tot)
         constexpr int R = STAGE / 64;
         float vx[R], vy[R], vz[R];
         int js[R];
@@ -568,10 +581,10 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             int j = 0;
 #pragma unroll
             for (int s = 32; s >= 1; s >>= 1)
-                if (eincl[j + s - 1] <= p) j += s;
+                if (__shfl(eincl, j + s - 1, 64) <= p) j += s;
             js[r] = j;
+            const int gi = __shfl(ebase, j, 64) + p;
             if (p < tot) {
-                const int gi = eip[j] + p - (eincl[j] - ecnt[j]);
                 vx[r] = a.ox[gi];
                 vy[r] = a.oy[gi];
                 vz[r] = a.oz[gi];
@@ -580,11 +593,12 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
 #pragma unroll
         for (int r = 0; r < R; r++) {
             const int p = lane + 64 * r;
+            const int j = js[r];
+            const float cx = __shfl(ecx, j, 64), cy = __shfl(ecy, j, 64), cz = __shfl(ecz, j, 64);
             if (p < tot) {
-                const int j = js[r];
-                sx[p] = __builtin_fmaf(vx[r], lam, ecx[j]);
-                sy[p] = __builtin_fmaf(vy[r], lam, ecy[j]);
-                sz[p] = __builtin_fmaf(vz[r], lam, ecz[j]);
+                sx[p] = __builtin_fmaf(vx[r], lam, cx);
+                sy[p] = __builtin_fmaf(vy[r], lam, cy);
+                sz[p] = __builtin_fmaf(vz[r], lam, cz);
             }
         }
         __syncthreads();
@@ -641,7 +655,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
         bx *= m;
         by *= m;
         bz *= m;
-        const int it = IT.ipart + t;
+        const int it = i0 + t;
         if (a.with_l2p) {
             constexpr int NT = nterms(P);
             const float *Lp = reinterpret_cast<const float *>(a.L) + (size_t)T * NT;
@@ -718,23 +732,28 @@ template <int P>
 __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
 {
     constexpr int STAGE = 256;
+    static_assert(STAGE >= kP2PMaxLeaf, "a source leaf fits one batch");
     __shared__ double stage[3 * STAGE];
     double *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ double ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
     __shared__ SplitRow64 tab[kSplitW64 * (kSplitN64 + 1)];
-    const int T = a.order[p2p_target(blockIdx.x, gridDim.x)];
+    // a.nchunk consecutive workgroups per target leaf, one per chunk of 64 of its
+    // particles (one chunk for leaves of up to 64)
+    const int chunk = blockIdx.x % a.nchunk;
+    const int T = a.order[p2p_target(blockIdx.x / a.nchunk, gridDim.x / a.nchunk)];
     const int lane = threadIdx.x;
     if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
-    const int n = IT.npart;
-    if (n == 0) return;
+    const int c0 = 64 * chunk;
+    if (c0 >= IT.npart) return;
+    const int n = min(64, IT.npart - c0), i0 = IT.ipart + c0;  // this chunk's targets
     // the split-factor table into LDS (read after the first barrier below)
     for (int j = lane; j < kSplitW64 * (kSplitN64 + 1); j += 64)
         tab[j] = reinterpret_cast<const SplitRow64 *>(kSplitTab64)[j];
     const LaneMap L = lane_map(n, lane);
-    const double xA = a.px[IT.ipart + L.tA], yA = a.py[IT.ipart + L.tA], zA = a.pz[IT.ipart + L.tA];
-    const double xB = a.px[IT.ipart + L.tB], yB = a.py[IT.ipart + L.tB], zB = a.pz[IT.ipart + L.tB];
+    const double xA = a.px[i0 + L.tA], yA = a.py[i0 + L.tA], zA = a.pz[i0 + L.tA];
+    const double xB = a.px[i0 + L.tB], yB = a.py[i0 + L.tB], zB = a.pz[i0 + L.tB];
     const double m = a.mass, eps = a.soft, meps3 = eps > 0.0 ? m / (eps * eps * eps) : 0.0;
     const double inv2rs = a.inv2rs, vscale = inv2rs * (kSplitTab64N / kSplitTab64U);
     const double boxlen = a.bx.len_of(T);
@@ -762,6 +781,7 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
             if (lane >= d) incl += v;
         }
         const int ne = __popcll(__ballot(lane < nb && incl <= STAGE));
+        if (ne == 0) break;  // as in k_p2p_f32
         const int tot = __shfl(incl, ne - 1, 64);
         ecnt[lane] = cnt;
         eincl[lane] = lane < ne ? incl : 0x7fffffff;
@@ -812,7 +832,7 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
                 bz += red[320 + l];
             }
         }
-        const int it = IT.ipart + t;
+        const int it = i0 + t;
         if (a.with_l2p) {
             constexpr int NT = nterms(P);
             const double *Lp = reinterpret_cast<const double *>(a.L) + (size_t)T * NT;
@@ -986,7 +1006,7 @@ void downward_pass(Ctx &c)
 
 void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, double *out)
 {
-    if (c.prm.maxleaf > kP2PMaxLeaf) throw std::runtime_error("maxleaf above 64 is not supported");
+    if (c.prm.maxleaf > kP2PMaxLeaf) throw std::runtime_error("maxleaf above 256 is not supported");
     const bool f32p = c.prm.precision == PHOTONS_FP32_MIXED;
     const int nlb = nleaf_bound(c), ncb = ncell_bound(c);
     need(c.p2p_off, sizeof(int64_t) * 2 * (size_t)nlb, "P2P list");
@@ -1016,6 +1036,7 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     a.inv_rs = 1.0 / c.prm.rs;
     a.bx = box_view(c);
     a.with_l2p = with_l2p;
+    a.nchunk = (c.prm.maxleaf + 63) / 64;
     a.accumulate = accumulate;
     a.out_caller_order = caller_order;
     a.out = out;
@@ -1024,10 +1045,12 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     a.order = dptr<int32_t>(c.p2p_order);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
     hipStream_t st = c.stream;
+    if ((int64_t)nlb * a.nchunk >= ((int64_t)1 << 31)) throw std::runtime_error("too many P2P workgroups");
+    const int nwg = nlb * a.nchunk;
 #define P2P_L(PP)                                              \
-    if (f32 && c.p2p_stage == 256) k_p2p_f32<PP, 256><<<nlb, 64, 0, st>>>(a); \
-    else if (f32) k_p2p_f32<PP, 512><<<nlb, 64, 0, st>>>(a);          \
-    else k_p2p_f64<PP><<<nlb, 64, 0, st>>>(a);
+    if (f32 && c.p2p_stage == 256) k_p2p_f32<PP, 256><<<nwg, 64, 0, st>>>(a); \
+    else if (f32) k_p2p_f32<PP, 512><<<nwg, 64, 0, st>>>(a);          \
+    else k_p2p_f64<PP><<<nwg, 64, 0, st>>>(a);
     if (nlb > 0) PH_DISPATCH(c.prm.order, P2P_L)
 #undef P2P_L
     PH_CHECK(hipGetLastError());

@@ -414,15 +414,11 @@ __device__ __forceinline__ void lane_start(const WalkArgs &a, LanePair &L, int64
 // Two pairs whose source is a leaf (use[i]; target T[i] >= 0), one rule each, no
 // traversal: a target leaf takes P2P (leaf-leaf pairs always do); a node target an M2L
 // entry, a down record or nothing, by the rule on the two cells.  The loads of both pairs
-// go out before either is decided.  WRITE = false counts; WRITE = true stores at qp, qm,
-// qd within the counts found (lp, lm, ld).
-template <bool WRITE>
+// go out before either is decided.  Counts only: the write pass stores each pair's one
+// output from its count.
 __device__ __forceinline__ void leaf_source_pairs(const WalkArgs &a, int nleaf, const bool (&use)[2],
                                                   const int (&T)[2], const DownRec (&D)[2], uint32_t (&np)[2],
-                                                  uint32_t (&nm)[2], uint32_t (&nd)[2], const int64_t (&qp)[2],
-                                                  const int64_t (&qm)[2], const int64_t (&qd)[2],
-                                                  const uint32_t (&lp)[2], const uint32_t (&lm)[2],
-                                                  const uint32_t (&ld)[2])
+                                                  uint32_t (&nm)[2], uint32_t (&nd)[2])
 {
     TgtRec t[2];
     CellGeom g[2];
@@ -439,7 +435,6 @@ __device__ __forceinline__ void leaf_source_pairs(const WalkArgs &a, int nleaf, 
         if (!use[i]) continue;
         const int cell = entry_cell(D[i].entry), sid = entry_shift(D[i].entry);
         if (!node[i]) {
-            if (WRITE && lp[i] > 0) a.p2p_out[qp[i]] = pack_entry(cell, sid);
             np[i] = 1;
             continue;
         }
@@ -451,13 +446,8 @@ __device__ __forceinline__ void leaf_source_pairs(const WalkArgs &a, int nleaf, 
         double sx = 0.0, sy = 0.0, sz = 0.0;
         if (sid != 0) shift_of(sid, t[i].L, sx, sy, sz);
         const int act = decide(a, t[i], R, -1, sid, sx, sy, sz);  // rank -1: not the target itself
-        if (act == kActM2L) {
-            if (WRITE && lm[i] > 0) a.m2l_out[qm[i]] = pack_entry(cell, sid);
-            nm[i] = 1;
-        } else if (act == kActOpenA) {
-            if (WRITE && ld[i] > 0) write_down(a, qd[i], cell, true, sid, t[i]);
-            nd[i] = 1;
-        }
+        if (act == kActM2L) nm[i] = 1;
+        else if (act == kActOpenA) nd[i] = 1;
     }
 }
 
@@ -509,8 +499,6 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_count(WalkArgs a)
     if (tm.tile < 0) return;  // the same for the whole wavefront
     const int64_t q0 = tm.tile * kTile;
     const int nleaf = a.sz->nleaf;
-    const int64_t z2[2] = {0, 0};
-    const uint32_t u2[2] = {0, 0};
     bool trav[kTile / 64];
     uint64_t bt[kTile / 64];
     int ntrav = 0;
@@ -530,7 +518,7 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_count(WalkArgs a)
             ntrav += __popcll(bt[k + i]);
         }
         uint32_t np[2], nm[2], nd[2];
-        leaf_source_pairs<false>(a, nleaf, use, T, D, np, nm, nd, z2, z2, z2, u2, u2, u2);
+        leaf_source_pairs(a, nleaf, use, T, D, np, nm, nd);
 #pragma unroll
         for (int i = 0; i < 2; i++)
             if (j[i] < tm.npair && !trav[k + i]) a.pcnt[q0 + j[i]] = pack_cnt(a, np[i], nm[i], nd[i]);
@@ -843,8 +831,21 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const C
                 }
             }
         }
-        uint32_t np[2], nm[2], nd[2];
-        leaf_source_pairs<true>(a, nleaf, use, T, D, np, nm, nd, qp, qm, qd, lp, lm, ld);
+        // a pair with a leaf source has one output at most, and its count (from the count
+        // pass's rule) says which: stored from the record alone, without the rule again
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            if (!use[i]) continue;
+            const int cell = entry_cell(D[i].entry), sid = entry_shift(D[i].entry);
+            if (lp[i]) {
+                a.p2p_out[qp[i]] = pack_entry(cell, sid);
+            } else if (lm[i]) {
+                a.m2l_out[qm[i]] = pack_entry(cell, sid);
+            } else if (ld[i]) {
+                const TgtRec &t = a.trec[T[i]];
+                a.dl_out[qd[i]] = DownRec{pack_entry(cell, sid), t.son[0] | kSrcLeaf, t.son[1]};
+            }
+        }
     }
 }
 

@@ -6,8 +6,9 @@
  *
  * PARAMFILE has the reference's "Name value" lines (# starts a comment).  Read here:
  * InputPath, OutputPath, OutputName, OPENANGLE, NumMeshSide, GravConstant,
- * SOFTENING, SPLITSCALE, SnapFormat (2, Gadget-2, only) and SnapNumber (1 only);
- * the other names are accepted and ignored.  As the reference: theta 0.3 unless
+ * SOFTENING, SPLITSCALE, MaxPackage (the leaf size; 32 when absent, --maxleaf
+ * overrides it), SnapFormat (2, Gadget-2, only) and SnapNumber (1 only); the other
+ * names are accepted and ignored.  As the reference: theta 0.3 unless
  * OPENANGLE is given, rs = 1.25 L/NumMeshSide, soft = 0.03 L/N^(1/3), rc = 4.5 rs,
  * the cosmology, box, mass and start redshift from the IC header, 512 steps to
  * a = 1, output OutputPath/OutputName_3.0.
@@ -47,7 +48,7 @@ static double now_s(void)
 typedef struct {
     char input[768], out_path[768], out_name[768];
     double theta, grav, soft, split;
-    int nside, format, nfiles;
+    int nside, format, nfiles, maxleaf;
 } run_params;
 
 /* src/initial.c:60-175: one "name value" pair per line */
@@ -70,6 +71,7 @@ static void read_params(const char *path, run_params *p)
         else if (!strcmp(name, "SPLITSCALE")) p->split = atof(value);
         else if (!strcmp(name, "SnapFormat")) p->format = atoi(value);
         else if (!strcmp(name, "SnapNumber")) p->nfiles = atoi(value);
+        else if (!strcmp(name, "MaxPackage")) p->maxleaf = atoi(value);
     }
     fclose(f);
 }
@@ -78,11 +80,11 @@ int main(int argc, char **argv)
 {
     if (argc < 2) die("usage: photons_run PARAMFILE [--nstep N] [--af A] [--fp32] [--order P] [--maxleaf L] [--no-pm]", NULL);
     run_params rp = {.theta = 0.3, .grav = 43007.105732, .soft = -1.0, .split = -1.0, .nside = 0, .format = 2,
-                     .nfiles = 1};
+                     .nfiles = 1, .maxleaf = 32};
     snprintf(rp.out_path, sizeof rp.out_path, ".");
     snprintf(rp.out_name, sizeof rp.out_name, "snapshot");
     read_params(argv[1], &rp);
-    int nstep = 512, order = 3, maxleaf = 32, with_pm = 1, precision = PHOTONS_FP64;
+    int nstep = 512, order = 3, maxleaf = rp.maxleaf, with_pm = 1, precision = PHOTONS_FP64;
     double af = 1.0;
     for (int i = 2; i < argc; i++) {
         if (!strcmp(argv[i], "--nstep") && i + 1 < argc) nstep = atoi(argv[++i]);

@@ -122,10 +122,10 @@ def test_bad_boxes_are_refused(ensemble):
 def test_p2p_stage_variants_agree(ensemble):
     counts, lengths, pos, prm, off = ensemble
     ctx = pg.Context(prm, pg.FP32_MIXED)
-    assert ctx.p2p_stage() == 512
-    a512 = ctx.force_boxes(counts, lengths, pos)
-    ctx.set_p2p_stage(256)
+    assert ctx.p2p_stage() == 256
     a256 = ctx.force_boxes(counts, lengths, pos)
+    ctx.set_p2p_stage(512)
+    a512 = ctx.force_boxes(counts, lengths, pos)
     assert np.percentile(rel(a256, a512), 99) <= 1e-5
     with pytest.raises(pg.PhotonsError):
         ctx.set_p2p_stage(300)

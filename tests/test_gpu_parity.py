@@ -114,13 +114,24 @@ def test_higher_orders_match_oracle(order):
     assert np.percentile(rel_err(a32, ref), 99) <= 1e-4
 
 
-@pytest.mark.parametrize("maxleaf", [2, 16, 64])
+@pytest.mark.parametrize("maxleaf", [2, 16, 64, 100, 256])
 def test_leaf_sizes_match_oracle(maxleaf):
+    """the reference takes the leaf size from its parameter file (MaxPackage,
+    src/initial.c:172-173); leaves of more than 64 particles run in chunks of 64 targets"""
     pos, prm = case_inputs("clus16_pm32")
     prm.maxleaf = maxleaf
     ref, _ = oracle.force(pos, prm)
     acc = pg.Context(prm, pg.FP64).force(pos)
     assert rel_err(acc, ref).max() <= 1e-10
+    a32 = pg.Context(prm, pg.FP32_MIXED).force(pos)
+    assert np.percentile(rel_err(a32, ref), 99) <= 1e-4
+
+
+def test_leaf_size_above_the_limit_is_refused():
+    _, prm = case_inputs("clus16_pm32")
+    prm.maxleaf = 257
+    with pytest.raises(pg.PhotonsError, match="maxleaf"):
+        pg.Context(prm, pg.FP64)
 
 
 def test_repeatable_bit_for_bit():
