@@ -11,8 +11,10 @@ Three more workloads print their own JSON line (not the default):
     --workload ensemble   64 independent clustered boxes of 32^3..128^3 particles (log-uniform,
                           seed 5) in one batched evaluation, leaf size and P2P staging autotuned
     --workload serving    256 resident boxes of 16^3..192^3 particles (seed 6), single-step
-                          (force + kick/drift) requests with Poisson arrivals at swept rates,
-                          HBM budget 200 GB: throughput and p50/p99 step latency per rate
+                          TreePM requests (short-range and PM forces + kick/drift; --serve-no-pm:
+                          short-range only) with Poisson arrivals at swept rates up to saturation,
+                          HBM budget 200 GB: throughput and p50/p99 step latency per rate, and the
+                          FP32-mixed against FP64 error of a served step
     --workload treepm     whole TreePM steps (kick, drift, short-range and PM forces, kick) of the
                           --nside^3 clustered box, particles resident in HBM
 """
@@ -381,9 +383,43 @@ def run_treepm(args, torch):
     }
 
 
+MESH_CLASSES = (16, 32, 48, 64, 96, 128, 192, 256)
+
+
+def serving_mesh(L, rs):
+    """a served box's PM mesh: the reference's NSIDE = 1.25 L / rs rounded up to one of a
+    few sizes (a lane keeps one PM context per mesh size)"""
+    want = 1.25 * L / rs
+    return next((m for m in MESH_CLASSES if m >= want - 1e-9), MESH_CLASSES[-1])
+
+
+def serving_step_error(prm, boxes, dkh, dd, pm_on):
+    """FP32-mixed against FP64 served steps: the same boxes, one step each, in a server of
+    each precision; relative error of the new short-range force and of the velocity change"""
+    got = {}
+    for name, prec in (("fp32", pg.FP32_MIXED), ("fp64", pg.FP64)):
+        srv = pg.Server(prm, prec, nlanes=2, hbm_budget=200e9, max_batch_particles=64 << 20)
+        ids = [srv.add_box(p, v, L, nside=serving_mesh(L, prm.rs) if pm_on else None) for p, v, L in boxes]
+        srv.drain()
+        for r in [srv.submit(b, dkh, dd) for b in ids]:
+            srv.wait(r)
+        got[name] = [srv.read_box(b) for b in ids]
+        srv.close()
+    ea, ev = [], []
+    for (p32, v32, a32), (p64, v64, a64), (p0, v0, L) in zip(got["fp32"], got["fp64"], boxes):
+        ea.append(np.linalg.norm(a32 - a64, axis=1) / np.linalg.norm(a64, axis=1))
+        ev.append(np.linalg.norm(v32 - v64, axis=1) / np.linalg.norm(v64 - v0, axis=1))
+    ea, ev = np.concatenate(ea), np.concatenate(ev)
+    return {"boxes": len(boxes), "particles": int(len(ea)),
+            "force": {"p50": float(np.median(ea)), "p99": float(np.percentile(ea, 99)), "max": float(ea.max())},
+            "velocity_change": {"p50": float(np.median(ev)), "p99": float(np.percentile(ev, 99))}}
+
+
 def run_serving(args, torch):
-    """Resident boxes, Poisson arrivals of single-step requests at swept rates."""
+    """Resident boxes, Poisson arrivals of single-step requests at swept rates (each
+    precision's sweep stops once the server falls behind the offered rate)."""
     nbox = args.serve_boxes
+    pm_on = not args.serve_no_pm
     log(f"generating {nbox} boxes")
     counts, lengths, pos, prm = ic.workload_ensemble(nbox, 16 ** 3, 192 ** 3, seed=6, order=args.order)
     off = np.concatenate([[0], np.cumsum(counts)])
@@ -394,7 +430,8 @@ def run_serving(args, torch):
     for name, prec, rates in (("fp32", pg.FP32_MIXED, args.rates32), ("fp64", pg.FP64, args.rates64)):
         srv = pg.Server(prm, prec, nlanes=2, hbm_budget=200e9, max_batch_particles=64 << 20)
         t0 = time.perf_counter()
-        ids = [srv.add_box(pos[off[b]:off[b + 1]], vel[off[b]:off[b + 1]], lengths[b]) for b in range(nbox)]
+        ids = [srv.add_box(pos[off[b]:off[b + 1]], vel[off[b]:off[b + 1]], lengths[b],
+                           nside=serving_mesh(lengths[b], prm.rs) if pm_on else None) for b in range(nbox)]
         srv.drain()  # the first force of every box
         log(f"[{name}] {nbox} boxes resident ({int(counts.sum())} particles), first forces in "
             f"{time.perf_counter() - t0:.1f} s")
@@ -438,12 +475,19 @@ def run_serving(args, torch):
                    "mean_boxes_per_batch": ((s1["mean_boxes_per_batch"] * s1["batches"] -
                                              s0["mean_boxes_per_batch"] * s0["batches"]) /
                                             max(1, s1["batches"] - s0["batches"]))}
+            row["max_lanes_busy"] = s1["max_lanes_busy"]
             rows.append(row)
             log(f"[{name}] rate {rate}/s: {row['particles_per_s'] / 1e6:.1f} M particles/s, "
+                f"{row['achieved_req_per_s']:.1f} req/s, "
                 f"latency p50 {row['latency_ms']['p50']:.0f} ms p99 {row['latency_ms']['p99']:.0f} ms, "
                 f"{row['mean_boxes_per_batch']:.1f} boxes per batch")
+            if row["achieved_req_per_s"] < 0.8 * rate:
+                break  # saturated: the server falls behind the offered rate
         out[name] = rows
         srv.close()
+    log("serving: FP32-mixed against FP64 steps of the first 16 boxes")
+    err = serving_step_error(prm, [(pos[off[b]:off[b + 1]], vel[off[b]:off[b + 1]], lengths[b])
+                                   for b in range(min(16, nbox))], dkh, dd, pm_on)
     best32 = max(r["particles_per_s"] for r in out["fp32"])
     best64 = max(r["particles_per_s"] for r in out["fp64"])
     sat = max(out["fp32"], key=lambda r: r["particles_per_s"])
@@ -452,10 +496,12 @@ def run_serving(args, torch):
         "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32-mixed", "data": "synthetic",
         "config": {"workload": f"serving: {nbox} resident clustered boxes of 16^3..192^3 particles (log-uniform, "
-                               "seed 6), Poisson single-step requests (force + FP64 kick/drift), 2 lanes, "
-                               "HBM budget 200 GB", "particles": int(counts.sum()), "boxes": nbox},
+                               "seed 6), Poisson single-step requests (short-range force"
+                               + (", PM force on a mesh of about 1.25 L / rs cells per side" if pm_on else "")
+                               + " + FP64 kick/drift), 2 lanes, HBM budget 200 GB",
+                   "particles": int(counts.sum()), "boxes": nbox, "pm": pm_on},
         "latency_ms": sat["latency_ms"], "sweep": out, "fp64": {"value": best64},
-        "fp32_over_fp64": best32 / best64,
+        "fp32_over_fp64": best32 / best64, "error_fp32_vs_fp64_per_step": err,
     }
 
 
@@ -471,8 +517,9 @@ def main():
     ap.add_argument("--workload", choices=["clustered", "ensemble", "serving", "treepm"], default="clustered")
     ap.add_argument("--serve-boxes", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival span per rate")
-    ap.add_argument("--rates32", type=float, nargs="+", default=[4, 8, 16, 32, 64, 128])
-    ap.add_argument("--rates64", type=float, nargs="+", default=[2, 4, 8, 16, 32])
+    ap.add_argument("--rates32", type=float, nargs="+", default=[4, 8, 16, 32, 64, 128, 256, 512])
+    ap.add_argument("--rates64", type=float, nargs="+", default=[2, 4, 8, 16, 32, 64])
+    ap.add_argument("--serve-no-pm", action="store_true", help="served steps without the PM part")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MiB",
                     help="first copy MiB mebibytes device to device (a plain copy kernel of known size, "
                          "to scale hardware byte counters)")
@@ -546,11 +593,11 @@ def main():
                      "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": (P2P_FP32_TRAFFIC_256 if fp32 and args.nside == 256 and prm.order == 4 else None),
                      "note": "27 flops per interaction x interactions / mean P2P kernel time (HIP events on the "
-                             "library's stream); compute bound on the vector ALU, where 2.5 of the ~30 "
-                             "instructions per interaction (rsqrt, exp2, half a shared rcp) go to the "
-                             "transcendental unit that paces the loop; packed FP32 is not used, so 78.6 of "
-                             "the 157.3 TFLOP/s peak is reachable at most; traffic = HBM bytes per launch "
-                             "from rocprofv3 PMC"},
+                             "library's stream); compute bound on the vector ALU: ~24 VALU instructions per "
+                             "interaction in the pair loop, one of them transcendental (rsqrt; the split factor "
+                             "is a cubic read from an LDS table); packed FP32 is not used, so 78.6 of the "
+                             "157.3 TFLOP/s peak is reachable at most; traffic = HBM bytes per launch from "
+                             "rocprofv3 PMC"},
     }
     if "fp32" in res and "fp64" in res:
         a32, a64 = res["fp32"]["acc"], res["fp64"]["acc"]

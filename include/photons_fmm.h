@@ -195,13 +195,17 @@ int photons_l2p_batch(photons_ctx *ctx, int order, int n, const double *d_pos, c
 /* ---- serving: resident boxes and continuous batching of step requests ----
  * No counterpart in the reference (its driver steps one domain in a loop,
  * src/photoNs.c:136-313).  A step request advances one box by the reference's
- * loop body without PM (src/photoNs.c:208-268), in FP64 on the GPU:
- *   v += acc*dkh; x += v*dd; wrap into [0, L); acc = force(x); v += acc*dkh
+ * loop body (src/photoNs.c:136-268), in FP64 on the GPU:
+ *   v += acc_pm*dkh + acc*dkh; x += v*dd; wrap into [0, L);
+ *   acc = force(x); acc_pm = pm(x); v += acc*dkh + acc_pm*dkh
  * with dkh = G*kick/2 and dd = drift (photons_kick_factor/_drift_factor over
- * the step's ln a interval, src/initial.c:650-683).  Boxes stay in HBM between
- * steps and are admitted while pos/vel/acc (72 B per particle) fit the HBM
- * budget minus what the lanes reserve for evaluation buffers (about 1 KB per
- * particle of max_batch_particles, per lane).  Each of nlanes lanes (own
+ * the step's ln a interval, src/initial.c:650-683); the acc_pm terms only for a
+ * box added with a mesh (photons_server_add_box_pm: nside^3 cells, the PM of
+ * photons_pm_* with the server's rs and mass), the others take the short-range
+ * force alone.  Boxes stay in HBM between steps and are admitted while
+ * pos/vel/acc (72 B per particle, 96 B with acc_pm) fit the HBM budget minus
+ * what the lanes reserve for evaluation buffers (about 1 KB per particle of
+ * max_batch_particles, per lane; a lane's PM meshes come on top).  Each of nlanes lanes (own
  * context and stream) takes, whenever it is free, every queued request whose
  * box is idle (arrival order, one request per box at a time, up to
  * max_batch_particles) and evaluates them as one batch of boxes; the lanes'
@@ -210,7 +214,7 @@ typedef struct photons_server photons_server;
 typedef struct {
     int64_t requests_done, particles_stepped, batches, queued;
     double mean_boxes_per_batch, span_s, resident_bytes, reserved_bytes;
-    int32_t nboxes, pad;
+    int32_t nboxes, max_lanes_busy;  /* lanes that evaluated batches at the same time, at most */
 } photons_server_stats;
 
 const char *photons_server_last_error(void);
@@ -221,13 +225,22 @@ int photons_server_destroy(photons_server *srv);
  * first request computes it, as the reference does before its first step)  */
 int photons_server_add_box(photons_server *srv, int64_t n, double boxsize, const double *pos, const double *vel,
                            const double *acc, int *box_id);
+/* the same with a PM mesh of nside^3 cells (even, in [8, 1624]); acc and acc_pm
+ * are both given or both NULL (then the first request computes both)          */
+int photons_server_add_box_pm(photons_server *srv, int64_t n, double boxsize, int nside, const double *pos,
+                              const double *vel, const double *acc, const double *acc_pm, int *box_id);
 int photons_server_remove_box(photons_server *srv, int box_id);
 int photons_server_submit(photons_server *srv, int box_id, double dkh, double dd, int64_t *request_id);
 /* waits for a request and reports it (latency: submit to completion, host clock);
  * each request is waited on once, the report releases it                     */
 int photons_server_wait(photons_server *srv, int64_t request_id, double *latency_ms);
 int photons_server_drain(photons_server *srv);
+/* paused != 0: the lanes finish their batches but take no new one until resumed
+ * (requests submitted meanwhile are batched together on resume); drain waits
+ * forever on a paused server with queued requests                             */
+int photons_server_pause(photons_server *srv, int paused);
 int photons_server_read_box(photons_server *srv, int box_id, double *pos, double *vel, double *acc);
+int photons_server_read_pm(photons_server *srv, int box_id, double *acc_pm);  /* a box with a mesh */
 int photons_server_get_stats(photons_server *srv, photons_server_stats *st);
 double photons_kick_factor(double loga_i, double loga_f, double omega_m, double omega_x);
 double photons_drift_factor(double loga_i, double loga_f, double omega_m, double omega_x);
@@ -245,6 +258,9 @@ const char *photons_pm_last_error(void);
 /* nside even, in [8, 1624]: the mesh indices are 32-bit (N^3 < 2^32)          */
 int photons_pm_create(photons_pm **pm, int nside, double boxsize, double rs, double mass, int device, void *stream);
 int photons_pm_destroy(photons_pm *pm);
+/* the box length of the next evaluations, the mesh shape unchanged (one context for
+ * boxes of one mesh size; no counterpart in the reference, one box per run)    */
+int photons_pm_set_boxsize(photons_pm *pm, double boxsize);
 int photons_pm_force_device(photons_pm *pm, int64_t n, const double *d_pos, double *d_acc);
 int photons_pm_force_host(photons_pm *pm, int64_t n, const double *pos, double *acc);
 int photons_pm_sync(photons_pm *pm);

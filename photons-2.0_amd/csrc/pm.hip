@@ -330,6 +330,16 @@ int photons_pm_destroy(photons_pm *p)
     return 0;
 }
 
+// the box length of the next evaluations (the mesh shape stays): one context serves
+// boxes of one mesh size and different lengths in turn
+int photons_pm_set_boxsize(photons_pm *p, double boxsize)
+{
+    if (!p) return fail("null handle");
+    if (!(boxsize > 0.0)) return fail("boxsize must be positive");
+    p->L = boxsize;
+    return 0;
+}
+
 // positions [n][3] in [0, L) on the device -> acc_pm [n][3], caller order, queued on
 // the context's stream
 int photons_pm_force_device(photons_pm *p, int64_t n, const double *d_pos, double *d_acc)

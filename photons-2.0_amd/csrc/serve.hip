@@ -2,9 +2,12 @@
 // single-step (force + kick/drift) requests on one GPU.
 //
 // Each box keeps pos, vel and the last force (acc) in HBM, caller order, FP64.  A
-// step is the reference's loop body without PM (src/photoNs.c:208-268): half kick
-// with the previous force, drift, wrap into [0, L), new short-range force, half
-// kick.  Requests queue in arrival order.  Each of nlanes worker lanes (its own
+// step is the reference's loop body (src/photoNs.c:136-268): half kicks with the
+// previous forces, drift, wrap into [0, L), new forces, half kicks; a box added with
+// a mesh (photons_server_add_box_pm) also keeps its PM force (acc_pm) and takes the
+// long-range half kicks too, its PM force computed on the evaluating lane's stream
+// (one PM context per lane and mesh shape, made when first needed); a box without
+// one is stepped with the short-range force alone.  Requests queue in arrival order.  Each of nlanes worker lanes (its own
 // force context and HIP stream) takes, whenever it is free, every queued request
 // whose box is idle, up to its particle capacity, and evaluates them as one forest
 // (photons_force_boxes_device), so requests that arrive while others run are taken
@@ -24,6 +27,9 @@
 #include <thread>
 #include <vector>
 
+#include <algorithm>
+#include <utility>
+
 #include "../../include/photons_fmm.h"
 #include "kick.h"
 
@@ -40,6 +46,7 @@ namespace {
 // batch's own pos/acc copies, rounded up
 constexpr double kTransientBytesPerParticle = 1024.0;
 constexpr double kResidentBytesPerParticle = 72.0;  // pos, vel, acc (3 doubles each)
+constexpr int kLaneMeshes = 8;                      // PM contexts kept per lane
 
 double now_s()
 {
@@ -50,6 +57,8 @@ struct Box {
     int64_t n = 0;
     double L = 0.0;
     double *pos = nullptr, *vel = nullptr, *acc = nullptr;
+    int nside = 0;              // PM mesh cells per side, 0: no long-range part
+    double *acc_pm = nullptr;   // the PM force (nside > 0)
     bool busy = false;      // a request of this box is being evaluated
 };
 
@@ -69,6 +78,10 @@ struct Lane {
     hipStream_t stream = nullptr;
     double *in = nullptr, *out = nullptr;  // batch pos / acc, [cap][3]
     int64_t cap = 0;
+    // PM contexts by mesh side, on this lane's stream (the least recently used goes
+    // beyond kLaneMeshes); a context takes each box's length in turn
+    std::map<int, std::pair<photons_pm *, int64_t>> pm;
+    int64_t pm_clock = 0;
     std::thread th;
 };
 
@@ -88,9 +101,11 @@ struct photons_server {
     int next_box = 0;
     int64_t next_req = 0;
     bool stop = false;
+    bool paused = false;  // lanes take no new batch (a burst of requests queued at once)
     std::vector<Lane> lanes;
     // statistics
     int64_t n_done = 0, particles_stepped = 0, batches = 0, batch_boxes = 0;
+    int lanes_busy = 0, max_lanes_busy = 0;  // lanes evaluating a batch, now and at most at once
     double t_first = -1.0, t_last = 0.0;
     std::string fatal;
 };
@@ -103,6 +118,29 @@ int fail(const std::string &m)
 {
     g_err = m;
     return -1;
+}
+
+// the lane's PM context for meshes of nside^3 cells, made when first needed; the least
+// recently used one is released past kLaneMeshes (its stream, the lane's, holds no
+// work of it: the lane's batches run one at a time and end with a wait)
+photons_pm *lane_mesh(photons_server *s, Lane &ln, int nside)
+{
+    auto it = ln.pm.find(nside);
+    if (it == ln.pm.end()) {
+        if ((int)ln.pm.size() >= kLaneMeshes) {
+            auto lru = ln.pm.begin();
+            for (auto jt = ln.pm.begin(); jt != ln.pm.end(); ++jt)
+                if (jt->second.second < lru->second.second) lru = jt;
+            photons_pm_destroy(lru->second.first);
+            ln.pm.erase(lru);
+        }
+        photons_pm *pm = nullptr;
+        if (photons_pm_create(&pm, nside, 1.0, s->prm.rs, s->prm.mass, s->device, ln.stream) != 0)
+            throw std::runtime_error(std::string("PM mesh: ") + photons_pm_last_error());
+        it = ln.pm.emplace(nside, std::make_pair(pm, (int64_t)0)).first;
+    }
+    it->second.second = ++ln.pm_clock;
+    return it->second.first;
 }
 
 // one batch on one lane: requests `ids` (their boxes marked busy by the caller)
@@ -123,8 +161,8 @@ void run_batch(photons_server *s, Lane &ln, const std::vector<int64_t> &ids)
     for (size_t k = 0; k < ids.size(); k++) {
         const int64_t n3 = 3 * bx[k].n;
         const int step = rq[k].kind == kStep;
-        k_kick_drift<<<kick_blocks(n3), 256, 0, ln.stream>>>(n3, bx[k].pos, bx[k].vel, bx[k].acc, nullptr, rq[k].dkh,
-                                                            rq[k].dd, bx[k].L, step, ln.in + 3 * off);
+        k_kick_drift<<<kick_blocks(n3), 256, 0, ln.stream>>>(n3, bx[k].pos, bx[k].vel, bx[k].acc, bx[k].acc_pm,
+                                                            rq[k].dkh, rq[k].dd, bx[k].L, step, ln.in + 3 * off);
         counts.push_back(bx[k].n);
         lengths.push_back(bx[k].L);
         off += bx[k].n;
@@ -132,10 +170,21 @@ void run_batch(photons_server *s, Lane &ln, const std::vector<int64_t> &ids)
     SV_CHECK(hipGetLastError());
     if (photons_force_boxes_device(ln.ctx, (int)counts.size(), counts.data(), lengths.data(), ln.in, ln.out) != 0)
         throw std::runtime_error(std::string("force: ") + photons_last_error());
+    // the PM force of the boxes with a mesh, at the drifted positions, on this lane's stream
+    off = 0;
+    for (size_t k = 0; k < ids.size(); k++) {
+        if (bx[k].nside > 0) {
+            photons_pm *pm = lane_mesh(s, ln, bx[k].nside);
+            if (photons_pm_set_boxsize(pm, bx[k].L) != 0 ||
+                photons_pm_force_device(pm, bx[k].n, ln.in + 3 * off, bx[k].acc_pm) != 0)
+                throw std::runtime_error(std::string("PM force: ") + photons_pm_last_error());
+        }
+        off += bx[k].n;
+    }
     off = 0;
     for (size_t k = 0; k < ids.size(); k++) {
         const int64_t n3 = 3 * bx[k].n;
-        k_kick<<<kick_blocks(n3), 256, 0, ln.stream>>>(n3, bx[k].vel, bx[k].acc, ln.out + 3 * off, nullptr,
+        k_kick<<<kick_blocks(n3), 256, 0, ln.stream>>>(n3, bx[k].vel, bx[k].acc, ln.out + 3 * off, bx[k].acc_pm,
                                                       rq[k].dkh, rq[k].kind == kStep);
         off += bx[k].n;
     }
@@ -153,6 +202,10 @@ void lane_loop(photons_server *s, int lane_id)
             std::unique_lock<std::mutex> g(s->mu);
             for (;;) {
                 if (s->stop) return;
+                if (s->paused) {
+                    s->cv_work.wait(g);
+                    continue;
+                }
                 // every queued request whose box is idle, in arrival order, within capacity
                 int64_t total = 0;
                 std::vector<int> taken;
@@ -173,6 +226,8 @@ void lane_loop(photons_server *s, int lane_id)
                 }
                 if (!ids.empty()) {
                     for (int t : taken) s->boxes[t].busy = true;
+                    s->lanes_busy++;
+                    s->max_lanes_busy = std::max(s->max_lanes_busy, s->lanes_busy);
                     break;
                 }
                 s->cv_work.wait(g);
@@ -201,6 +256,7 @@ void lane_loop(photons_server *s, int lane_id)
                 // a box's initial force request has no id a caller could wait on
                 if (r.kind == kForce) s->reqs.erase(id);
             }
+            s->lanes_busy--;
             s->batches++;
             s->batch_boxes += (int64_t)ids.size();
             s->t_last = t;
@@ -291,6 +347,8 @@ int photons_server_destroy(photons_server *s)
     if (!s) return 0;
     {
         std::unique_lock<std::mutex> g(s->mu);
+        s->paused = false;  // the queued requests run first
+        s->cv_work.notify_all();
         s->cv_done.wait(g, [&] {
             if (!s->fatal.empty()) return true;  // the lanes take no more work
             if (!s->queue.empty()) return false;
@@ -306,6 +364,7 @@ int photons_server_destroy(photons_server *s)
     (void)hipSetDevice(s->device);
     for (Lane &ln : s->lanes) {
         if (ln.stream) (void)hipStreamSynchronize(ln.stream);
+        for (auto &kv : ln.pm) photons_pm_destroy(kv.second.first);
         if (ln.ctx) photons_destroy(ln.ctx);
         if (ln.in) (void)hipFree(ln.in);
         if (ln.out) (void)hipFree(ln.out);
@@ -315,57 +374,91 @@ int photons_server_destroy(photons_server *s)
         (void)hipFree(kv.second.pos);
         (void)hipFree(kv.second.vel);
         (void)hipFree(kv.second.acc);
+        if (kv.second.acc_pm) (void)hipFree(kv.second.acc_pm);
     }
     delete s;
     return 0;
 }
 
+}  // extern "C"
+
+namespace {
+
+int add_box(photons_server *s, int64_t n, double boxsize, int nside, const double *pos, const double *vel,
+            const double *acc, const double *acc_pm, int *box_id)
+{
+    if (!s || !pos || !vel || !box_id) return fail("null argument");
+    if (!(boxsize > 0.0)) return fail("boxsize must be positive");
+    if (n <= (int64_t)s->prm.maxleaf) return fail("a periodic box needs more than maxleaf particles");
+    if (n > s->lane_cap) return fail("the box is larger than a lane's batch capacity");
+    if (nside > 0 && (acc != nullptr) != (acc_pm != nullptr))
+        return fail("a box with a mesh takes both forces or neither");
+    const double bytes = (kResidentBytesPerParticle + (nside > 0 ? 24.0 : 0.0)) * (double)n;
+    std::unique_lock<std::mutex> g(s->mu);
+    if (s->resident + bytes > s->budget - s->reserve) return fail("HBM budget exhausted: box not admitted");
+    s->resident += bytes;
+    s->peak = std::max(s->peak, s->resident + s->reserve);
+    g.unlock();
+    Box b;
+    b.n = n;
+    b.L = boxsize;
+    b.nside = nside;
+    try {
+        SV_CHECK(hipSetDevice(s->device));
+        const size_t nb = sizeof(double) * 3 * (size_t)n;
+        SV_CHECK(hipMalloc(&b.pos, nb));
+        SV_CHECK(hipMalloc(&b.vel, nb));
+        SV_CHECK(hipMalloc(&b.acc, nb));
+        if (nside > 0) SV_CHECK(hipMalloc(&b.acc_pm, nb));
+        SV_CHECK(hipMemcpy(b.pos, pos, nb, hipMemcpyHostToDevice));
+        SV_CHECK(hipMemcpy(b.vel, vel, nb, hipMemcpyHostToDevice));
+        if (acc) SV_CHECK(hipMemcpy(b.acc, acc, nb, hipMemcpyHostToDevice));
+        if (acc_pm && nside > 0) SV_CHECK(hipMemcpy(b.acc_pm, acc_pm, nb, hipMemcpyHostToDevice));
+    } catch (...) {
+        // give back what this box took: its allocations and its share of the budget
+        for (double *p : {b.pos, b.vel, b.acc, b.acc_pm})
+            if (p) (void)hipFree(p);
+        std::lock_guard<std::mutex> r(s->mu);
+        s->resident -= bytes;
+        throw;
+    }
+    g.lock();
+    const int id = s->next_box++;
+    s->boxes[id] = b;
+    // without a given force the box's first request computes it (the forces before the
+    // reference's first step, src/photoNs.c:97-127)
+    if (!acc) enqueue(s, id, kForce, 0.0, 0.0);
+    *box_id = id;
+    g.unlock();
+    s->cv_work.notify_all();
+    return 0;
+}
+
+double resident_bytes(const Box &b) { return (kResidentBytesPerParticle + (b.nside > 0 ? 24.0 : 0.0)) * (double)b.n; }
+
+}  // namespace
+
+extern "C" {
+
 int photons_server_add_box(photons_server *s, int64_t n, double boxsize, const double *pos, const double *vel,
                            const double *acc, int *box_id)
 {
     try {
-        if (!s || !pos || !vel || !box_id) return fail("null argument");
-        if (!(boxsize > 0.0)) return fail("boxsize must be positive");
-        if (n <= (int64_t)s->prm.maxleaf) return fail("a periodic box needs more than maxleaf particles");
-        if (n > s->lane_cap) return fail("the box is larger than a lane's batch capacity");
-        const double bytes = kResidentBytesPerParticle * (double)n;
-        std::unique_lock<std::mutex> g(s->mu);
-        if (s->resident + bytes > s->budget - s->reserve) return fail("HBM budget exhausted: box not admitted");
-        s->resident += bytes;
-        s->peak = std::max(s->peak, s->resident + s->reserve);
-        g.unlock();
-        Box b;
-        b.n = n;
-        b.L = boxsize;
-        try {
-            SV_CHECK(hipSetDevice(s->device));
-            const size_t nb = sizeof(double) * 3 * (size_t)n;
-            SV_CHECK(hipMalloc(&b.pos, nb));
-            SV_CHECK(hipMalloc(&b.vel, nb));
-            SV_CHECK(hipMalloc(&b.acc, nb));
-            SV_CHECK(hipMemcpy(b.pos, pos, nb, hipMemcpyHostToDevice));
-            SV_CHECK(hipMemcpy(b.vel, vel, nb, hipMemcpyHostToDevice));
-            if (acc) SV_CHECK(hipMemcpy(b.acc, acc, nb, hipMemcpyHostToDevice));
-        } catch (...) {
-            // give back what this box took: its allocations and its share of the budget
-            for (double *p : {b.pos, b.vel, b.acc})
-                if (p) (void)hipFree(p);
-            std::lock_guard<std::mutex> r(s->mu);
-            s->resident -= bytes;
-            throw;
-        }
-        g.lock();
-        const int id = s->next_box++;
-        s->boxes[id] = b;
-        // without a given force the box's first request computes it (the force before the
-        // reference's first step, src/photoNs.c:97-116)
-        if (!acc) enqueue(s, id, kForce, 0.0, 0.0);
-        *box_id = id;
-        g.unlock();
-        s->cv_work.notify_all();
-        return 0;
+        return add_box(s, n, boxsize, 0, pos, vel, acc, nullptr, box_id);
     } catch (const std::exception &e) {
         return fail(std::string("photons_server_add_box: ") + e.what());
+    }
+}
+
+int photons_server_add_box_pm(photons_server *s, int64_t n, double boxsize, int nside, const double *pos,
+                              const double *vel, const double *acc, const double *acc_pm, int *box_id)
+{
+    try {
+        // the mesh shape photons_pm_create accepts (checked here, before the box is admitted)
+        if (nside < 8 || nside > 1624 || nside % 2) return fail("nside must be even, in [8, 1624]");
+        return add_box(s, n, boxsize, nside, pos, vel, acc, acc_pm, box_id);
+    } catch (const std::exception &e) {
+        return fail(std::string("photons_server_add_box_pm: ") + e.what());
     }
 }
 
@@ -387,12 +480,13 @@ int photons_server_remove_box(photons_server *s, int box_id)
     if (it == s->boxes.end()) return fail("no such box");
     Box b = it->second;
     s->boxes.erase(it);
-    s->resident -= kResidentBytesPerParticle * (double)b.n;
+    s->resident -= resident_bytes(b);
     g.unlock();
     (void)hipSetDevice(s->device);
     (void)hipFree(b.pos);
     (void)hipFree(b.vel);
     (void)hipFree(b.acc);
+    if (b.acc_pm) (void)hipFree(b.acc_pm);
     return 0;
 }
 
@@ -426,6 +520,16 @@ int photons_server_wait(photons_server *s, int64_t request_id, double *latency_m
     s->reqs.erase(it);
     if (latency_ms) *latency_ms = 1e3 * (r.t_done - r.t_submit);
     if (!r.error.empty()) return fail(r.error);
+    return 0;
+}
+
+int photons_server_pause(photons_server *s, int paused)
+{
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->paused = paused != 0;
+    }
+    s->cv_work.notify_all();
     return 0;
 }
 
@@ -474,6 +578,36 @@ int photons_server_read_box(photons_server *s, int box_id, double *pos, double *
     }
 }
 
+int photons_server_read_pm(photons_server *s, int box_id, double *acc_pm)
+{
+    try {
+        std::unique_lock<std::mutex> g(s->mu);
+        auto it = s->boxes.find(box_id);
+        if (it == s->boxes.end()) return fail("no such box");
+        if (!it->second.acc_pm) return fail("the box has no mesh");
+        s->cv_done.wait(g, [&] { return !it->second.busy; });
+        const Box b = it->second;
+        it->second.busy = true;  // no lane takes the box while it is read
+        g.unlock();
+        std::string err;
+        try {
+            SV_CHECK(hipSetDevice(s->device));
+            if (acc_pm) SV_CHECK(hipMemcpy(acc_pm, b.acc_pm, sizeof(double) * 3 * (size_t)b.n, hipMemcpyDeviceToHost));
+        } catch (const std::exception &e) {
+            err = e.what();
+        }
+        g.lock();
+        s->boxes[box_id].busy = false;
+        g.unlock();
+        s->cv_work.notify_all();
+        s->cv_done.notify_all();
+        if (!err.empty()) return fail(err);
+        return 0;
+    } catch (const std::exception &e) {
+        return fail(std::string("photons_server_read_pm: ") + e.what());
+    }
+}
+
 int photons_server_get_stats(photons_server *s, photons_server_stats *st)
 {
     std::lock_guard<std::mutex> g(s->mu);
@@ -483,6 +617,7 @@ int photons_server_get_stats(photons_server *s, photons_server_stats *st)
     st->mean_boxes_per_batch = s->batches ? (double)s->batch_boxes / s->batches : 0.0;
     st->span_s = s->t_first >= 0.0 ? s->t_last - s->t_first : 0.0;
     st->nboxes = (int)s->boxes.size();
+    st->max_lanes_busy = s->max_lanes_busy;
     st->resident_bytes = s->resident;
     st->reserved_bytes = s->reserve;
     st->queued = (int64_t)s->queue.size();

@@ -312,7 +312,7 @@ class ServerStats(C.Structure):
     _fields_ = [("requests_done", C.c_int64), ("particles_stepped", C.c_int64), ("batches", C.c_int64),
                 ("queued", C.c_int64), ("mean_boxes_per_batch", C.c_double), ("span_s", C.c_double),
                 ("resident_bytes", C.c_double), ("reserved_bytes", C.c_double), ("nboxes", C.c_int32),
-                ("pad", C.c_int32)]
+                ("max_lanes_busy", C.c_int32)]
 
     def as_dict(self):
         return {f[0]: getattr(self, f[0]) for f in self._fields_ if f[0] != "pad"}
@@ -326,6 +326,10 @@ def _server_lib():
         L.photons_server_create.argtypes = [C.POINTER(vp), C.POINTER(Params), C.c_int, C.c_int, C.c_double, C.c_int64]
         L.photons_server_destroy.argtypes = [vp]
         L.photons_server_add_box.argtypes = [vp, C.c_int64, C.c_double, dp, dp, dp, C.POINTER(C.c_int)]
+        L.photons_server_add_box_pm.argtypes = [vp, C.c_int64, C.c_double, C.c_int, dp, dp, dp, dp,
+                                                C.POINTER(C.c_int)]
+        L.photons_server_read_pm.argtypes = [vp, C.c_int, dp]
+        L.photons_server_pause.argtypes = [vp, C.c_int]
         L.photons_server_remove_box.argtypes = [vp, C.c_int]
         L.photons_server_submit.argtypes = [vp, C.c_int, C.c_double, C.c_double, i64p]
         L.photons_server_wait.argtypes = [vp, C.c_int64, dp]
@@ -376,13 +380,21 @@ class Server:
         except Exception:
             pass
 
-    def add_box(self, pos, vel, boxsize, acc=None):
+    def add_box(self, pos, vel, boxsize, acc=None, nside=None, acc_pm=None):
+        """a resident box; with nside, a PM mesh of nside^3 cells and the long-range half
+        kicks in its steps (acc and acc_pm both given or both None)"""
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         vel = np.ascontiguousarray(vel, dtype=np.float64)
-        a = None if acc is None else _dp(np.ascontiguousarray(acc, dtype=np.float64))
+        keep = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (acc, acc_pm)]
+        a, apm = (None if x is None else _dp(x) for x in keep)
         bid = C.c_int()
-        _scheck(_server_lib().photons_server_add_box(self.h, pos.shape[0], float(boxsize), _dp(pos), _dp(vel), a,
-                                                     C.byref(bid)), "photons_server_add_box")
+        L = _server_lib()
+        if nside is None:
+            _scheck(L.photons_server_add_box(self.h, pos.shape[0], float(boxsize), _dp(pos), _dp(vel), a,
+                                             C.byref(bid)), "photons_server_add_box")
+        else:
+            _scheck(L.photons_server_add_box_pm(self.h, pos.shape[0], float(boxsize), int(nside), _dp(pos), _dp(vel),
+                                                a, apm, C.byref(bid)), "photons_server_add_box_pm")
         self._n[bid.value] = pos.shape[0]
         return bid.value
 
@@ -404,12 +416,20 @@ class Server:
     def drain(self):
         _scheck(_server_lib().photons_server_drain(self.h), "photons_server_drain")
 
+    def pause(self, paused=True):
+        _scheck(_server_lib().photons_server_pause(self.h, 1 if paused else 0), "photons_server_pause")
+
     def read_box(self, bid):
         n = self._n[bid]
         pos, vel, acc = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
         _scheck(_server_lib().photons_server_read_box(self.h, bid, _dp(pos), _dp(vel), _dp(acc)),
                 "photons_server_read_box")
         return pos, vel, acc
+
+    def read_pm(self, bid):
+        acc_pm = np.zeros((self._n[bid], 3))
+        _scheck(_server_lib().photons_server_read_pm(self.h, bid, _dp(acc_pm)), "photons_server_read_pm")
+        return acc_pm
 
     def stats(self):
         st = ServerStats()
