@@ -39,10 +39,10 @@ METRIC = "particles/s per force eval, FP32-mixed vs FP64 @1e-4 err; p50/p99 late
 PEAK_FP32_VALU = 157.3  # TFLOP/s, MI355X vector FP32
 PEAK_FP64_VALU = 78.6
 P2P_FLOPS = 27  # per particle-particle interaction
-# HBM bytes per FP32 P2P launch at 256^3 clustered, p = 4, from rocprofv3 --pmc
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy in the same run);
-# the counts and commands are kept with the P2P counter summary under profiles/
-P2P_FP32_TRAFFIC_256 = (2.0 * 32198865 + 1176594) * 1024
+# HBM bytes per FP32 P2P launch (k_p2p_f32<4, 256>) at 256^3 clustered, p = 4, from
+# rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy in the
+# same run); the counts and commands are kept with the P2P counter summary under profiles/
+P2P_FP32_TRAFFIC_256 = (2.0 * 13181521 + 793589) * 1024
 
 
 def log(msg):
