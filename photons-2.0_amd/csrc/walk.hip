@@ -535,36 +535,57 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_count(WalkArgs a)
     if (lane == 0) a.tcount[tm.tile] = ntrav;
 }
 
-// The listed pairs of a depth, by super-chunks of kSuper consecutive tiles (tiles of one
-// child stream and neighbouring records: the same sources); a wavefront walks its
-// super-chunks' lists as one sequence.
+// The listed pairs of a depth, by super-chunks of kSuper tiles: two consecutive tiles of
+// the first child stream and the same two of the second.  Both streams hold the same
+// records, so a record with a node source is listed in both (the two children of its
+// target against the same source subtree), at the same place: the chunk hands them out
+// interleaved, so that the two traversals of one subtree start side by side in a
+// wavefront and their record loads coincide where the two children decide alike.  A
+// wavefront walks its super-chunks' lists as one sequence.
 constexpr int kSuper = 4;
 
 struct SuperChunk {
-    int64_t t0;    // first tile
-    int pre[kSuper + 1];  // listed pairs before each tile of the chunk
+    int64_t k0;           // first tile of the first stream (the second's: H + k0)
+    int64_t H;            // tiles per stream
+    int n0[2], m[2];      // per tile pair: listed in the first stream, min of both streams
+    int pre[3];           // listed pairs before each tile pair of the chunk
 };
 
 __device__ __forceinline__ SuperChunk super_chunk(const WalkArgs &a, int64_t s, int64_t ntile_dev)
 {
     SuperChunk sc;
-    sc.t0 = s * kSuper;
+    sc.H = ntile_dev / 2;
+    sc.k0 = 2 * s;
     const int lane = threadIdx.x % 64;
+    // lanes 0, 1: the first stream's tiles k0, k0 + 1; lanes 2, 3: the second stream's
     int cnt = 0;
-    if (lane < kSuper && sc.t0 + lane < ntile_dev) cnt = a.tcount[sc.t0 + lane];
+    if (lane < kSuper && sc.k0 + (lane & 1) < sc.H) cnt = a.tcount[(lane >> 1) * sc.H + sc.k0 + (lane & 1)];
     sc.pre[0] = 0;
 #pragma unroll
-    for (int i = 0; i < kSuper; i++) sc.pre[i + 1] = sc.pre[i] + __shfl(cnt, i, 64);
+    for (int j = 0; j < 2; j++) {
+        const int c0 = __shfl(cnt, j, 64), c1 = __shfl(cnt, 2 + j, 64);
+        sc.n0[j] = c0;
+        sc.m[j] = min(c0, c1);
+        sc.pre[j + 1] = sc.pre[j] + c0 + c1;
+    }
     return sc;
 }
 
-// the listed pair at position idx (< sc.pre[kSuper]) of a super-chunk
+// the listed pair at position idx (< sc.pre[2]) of a super-chunk: within a tile pair,
+// first and second stream alternate while both have pairs, then the longer one's rest
 __device__ __forceinline__ TravRec super_pair(const WalkArgs &a, const SuperChunk &sc, int idx)
 {
-    int i = 0;
-#pragma unroll
-    for (int k = 1; k < kSuper; k++) i += idx >= sc.pre[k];
-    return a.tlist[(sc.t0 + i) * kTile + (idx - sc.pre[i])];
+    const int j = idx >= sc.pre[1] ? 1 : 0;
+    const int r = idx - sc.pre[j], m = sc.m[j];
+    int st, i;
+    if (r < 2 * m) {
+        st = r & 1;
+        i = r >> 1;
+    } else {
+        st = sc.n0[j] > m ? 0 : 1;
+        i = m + (r - 2 * m);
+    }
+    return a.tlist[(st * sc.H + sc.k0 + j) * kTile + i];
 }
 
 // A wavefront's stream of listed pairs: its super-chunks s = gw, gw + trav_waves, ...  in
@@ -579,13 +600,13 @@ struct TravStream {
 __device__ __forceinline__ void stream_open(const WalkArgs &a, TravStream &ts, int64_t gw)
 {
     ts.ntile_dev = 2 * (npad_of(a) / kTile);
-    ts.nsuper = (ts.ntile_dev + kSuper - 1) / kSuper;
+    ts.nsuper = (ts.ntile_dev / 2 + 1) / 2;  // pairs of tiles of the first stream
     ts.s = gw;
     ts.cn = ts.next = 0;
     ts.more = ts.s < ts.nsuper;
     if (ts.more) {
         ts.sc = super_chunk(a, ts.s, ts.ntile_dev);
-        ts.cn = ts.sc.pre[kSuper];
+        ts.cn = ts.sc.pre[2];
     }
 }
 
@@ -598,7 +619,7 @@ __device__ __forceinline__ void stream_advance(const WalkArgs &a, TravStream &ts
         ts.cn = ts.next = 0;
         if (ts.more) {
             ts.sc = super_chunk(a, ts.s, ts.ntile_dev);
-            ts.cn = ts.sc.pre[kSuper];
+            ts.cn = ts.sc.pre[2];
         }
     }
 }
