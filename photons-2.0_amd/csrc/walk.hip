@@ -136,6 +136,7 @@ struct WalkArgs {
     int32_t *tcount;          // per tile, how many
     uint32_t *qoff;           // [3 per pair]: a listed pair's output offsets from its tile's base
     int trav_waves;           // wavefronts of the traversal passes (a multiple of kTileWaves)
+    int trav_split;           // slices per super-chunk handed to the traversal wavefronts
     Cont *cont;               // continued traversals: kContPerWave slots per traversal wavefront
     int32_t *ccount;          // per traversal wavefront, its slots in use
     const int32_t *coff;      // their exclusive scan
@@ -473,6 +474,8 @@ __device__ __forceinline__ int64_t npad_of(const WalkArgs &a)
 
 // the traversal passes' grid: persistent wavefronts, each with its own continuation slots
 constexpr int kTravWavesMax = 16384;
+constexpr int kTravWavesFill = 2048;  // 8 wavefronts per CU
+constexpr int kTravSplitMax = 16;     // slices of at least 64 of a super-chunk's 1024 positions
 constexpr int kContPerWave = 256;
 
 // a pair's outputs: its packed counts, or its continuation's totals
@@ -588,39 +591,45 @@ __device__ __forceinline__ TravRec super_pair(const WalkArgs &a, const SuperChun
     return a.tlist[(st * sc.H + sc.k0 + j) * kTile + i];
 }
 
-// A wavefront's stream of listed pairs: its super-chunks s = gw, gw + trav_waves, ...  in
-// turn, without waiting at the end of one for the lanes still busy with its pairs.
+// A wavefront's stream of listed pairs: its units u = gw, gw + trav_waves, ...  in turn,
+// without waiting at the end of one for the lanes still busy with its pairs.  A unit is
+// one of trav_split equal slices of a super-chunk's positions (more than one at a depth
+// with few records, so that its traversals spread over more wavefronts).
 struct TravStream {
-    int64_t s, nsuper, ntile_dev;
+    int64_t u, nunit, ntile_dev;
     SuperChunk sc;
-    int cn, next;  // listed pairs of the current super-chunk, handed out
-    bool more;     // a current super-chunk
+    int cn, next;  // end of the current unit's positions, next position handed out
+    bool more;     // a current unit
 };
+
+__device__ __forceinline__ void stream_load(const WalkArgs &a, TravStream &ts)
+{
+    ts.more = ts.u < ts.nunit;
+    ts.cn = ts.next = 0;
+    if (ts.more) {
+        const int split = a.trav_split;
+        const int part = (int)(ts.u % split);
+        ts.sc = super_chunk(a, ts.u / split, ts.ntile_dev);
+        const int n = ts.sc.pre[2];
+        ts.next = n * part / split;
+        ts.cn = n * (part + 1) / split;
+    }
+}
 
 __device__ __forceinline__ void stream_open(const WalkArgs &a, TravStream &ts, int64_t gw)
 {
     ts.ntile_dev = 2 * (npad_of(a) / kTile);
-    ts.nsuper = (ts.ntile_dev / 2 + 1) / 2;  // pairs of tiles of the first stream
-    ts.s = gw;
-    ts.cn = ts.next = 0;
-    ts.more = ts.s < ts.nsuper;
-    if (ts.more) {
-        ts.sc = super_chunk(a, ts.s, ts.ntile_dev);
-        ts.cn = ts.sc.pre[2];
-    }
+    ts.nunit = (ts.ntile_dev / 2 + 1) / 2 * a.trav_split;  // super-chunks: pairs of tiles of the first stream
+    ts.u = gw;
+    stream_load(a, ts);
 }
 
-// the next super-chunk while the current one is handed out (the same in every lane)
+// the next unit while the current one is handed out (the same in every lane)
 __device__ __forceinline__ void stream_advance(const WalkArgs &a, TravStream &ts)
 {
     while (ts.more && ts.next >= ts.cn) {
-        ts.s += a.trav_waves;
-        ts.more = ts.s < ts.nsuper;
-        ts.cn = ts.next = 0;
-        if (ts.more) {
-            ts.sc = super_chunk(a, ts.s, ts.ntile_dev);
-            ts.cn = ts.sc.pre[2];
-        }
+        ts.u += a.trav_waves;
+        stream_load(a, ts);
     }
 }
 
@@ -1145,9 +1154,14 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         a.tlist = static_cast<TravRec *>(ctx_buf(c, c.walk_tlist, sizeof(TravRec) * kTile * (size_t)ntile, "walk pairs"));
         a.tcount = static_cast<int32_t *>(ctx_buf(c, c.walk_tcount, sizeof(int32_t) * (size_t)ntile, "walk pairs"));
         a.qoff = static_cast<uint32_t *>(ctx_buf(c, c.walk_qoff, 3 * sizeof(uint32_t) * kTile * (size_t)ntile, "walk pairs"));
-        // the traversal passes: about one super-chunk per wavefront, at most kTravWavesMax
+        // the traversal passes: about one unit per wavefront, at most kTravWavesMax; at a
+        // depth of fewer than kTravWavesFill super-chunks, each is cut into slices (up to
+        // kTravSplitMax) so that its traversals spread over more wavefronts
         const int64_t nsuper_cap = (ntile + kSuper - 1) / kSuper;
-        const int tw = (int)std::min<int64_t>(kTravWavesMax, (nsuper_cap + kTileWaves - 1) / kTileWaves * kTileWaves);
+        const int split = (int)std::min<int64_t>(kTravSplitMax, std::max<int64_t>(1, kTravWavesFill / nsuper_cap));
+        const int64_t nunit_cap = nsuper_cap * split;
+        const int tw = (int)std::min<int64_t>(kTravWavesMax, (nunit_cap + kTileWaves - 1) / kTileWaves * kTileWaves);
+        a.trav_split = split;
         const int nslot = tw * kContPerWave;
         a.trav_waves = tw;
         a.cont = static_cast<Cont *>(ctx_buf(c, c.big_pairs, sizeof(Cont) * (size_t)nslot, "walk continuations"));
