@@ -160,6 +160,7 @@ struct Ctx {
     DevBuf walk_rec, walk_trec, dl_a, dl_b, pair_cnt, big_pairs, walk_tlist, walk_tcount, walk_qoff, walk_ccount,
         walk_cidx;
     DevBuf p2p_order, p2p_order_tmp;          // P2P launch order (heaviest target leaves first)
+    DevBuf p2p_part;                          // P2P forces per slice of the lists (few particles)
     DevBuf m2l_item_off, m2l_item_tgt, m2l_part;  // balanced M2L work items
     DevBuf scan_tmp;
     // build temporaries (level order)
@@ -226,6 +227,19 @@ T *dptr(DevBuf &b)
 {
     return reinterpret_cast<T *>(b.p);
 }
+// box owning particle slot i (tree order keeps each box's particles in its input range):
+// the last b with box_off[b] <= i
+__device__ __forceinline__ int box_of_slot(const int64_t *__restrict__ box_off, int nbox, int64_t i)
+{
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (box_off[mid] <= i) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 inline BoxView box_view(Ctx &c) { return BoxView{dptr<double>(c.d_box_len), dptr<int>(c.cell_box)}; }
 // sets the boxes of the next evaluation (counts and lengths; uploads them)
 void set_boxes(Ctx &c, int nbox, const int64_t *counts, const double *lengths);

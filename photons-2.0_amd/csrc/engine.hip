@@ -133,7 +133,7 @@ static std::vector<DevBuf *> eval_bufs(Ctx &c)
             &c.l_count, &c.l_lo, &c.l_hi, &c.l_parent, &c.seg_chunk, &c.chunk_part, &c.chunk_rcnt, &c.seg_nleft,
             &c.seg_flag, &c.seg_nn, &c.seg_non, &c.t_size, &c.t_lsize, &c.t_dfs, &c.t_lfirst, &c.l_final,
             &c.walk_rec, &c.walk_trec, &c.dl_a, &c.dl_b, &c.pair_cnt, &c.big_pairs, &c.walk_tlist, &c.walk_tcount, &c.walk_qoff, &c.walk_ccount, &c.walk_cidx, &c.m2l_item_off, &c.m2l_item_tgt, &c.m2l_part, &c.d_box_off, &c.d_box_len,
-            &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp};
+            &c.d_box_root, &c.cell_box, &c.p2p_order, &c.p2p_order_tmp, &c.p2p_part};
 }
 
 static void release(DevBuf &b)

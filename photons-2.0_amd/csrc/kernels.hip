@@ -264,6 +264,11 @@ struct P2PArgs {
     const int32_t *order; // target leaf of each launch slot (heaviest first; past nleaf: none)
     const DevSizes *sz;
     int nchunk;           // workgroups per target leaf: chunks of 64 targets (maxleaf / 64 up)
+    int nslice;           // launch bound of the slices of a target leaf's list (p2p_slices of its box)
+    int nbox;
+    const int64_t *box_off;  // [nbox + 1]: particle slots of each box
+    int64_t n;            // particles (the slices' forces: part[slice][n][3])
+    double *part;
 };
 
 __device__ __forceinline__ void shift_vec(int sid, double L, double &sx, double &sy, double &sz)
@@ -302,6 +307,59 @@ __device__ __forceinline__ int p2p_target(int b, int nb)
     if (b >= full) return b;
     const int x = b % kXcd, k = b / kXcd;
     return ((k / kTile) * kXcd + x) * kTile + k % kTile;
+}
+
+// Slices of the P2P lists of the target leaves of a box of nb particles: up to
+// kP2PSliceMax, while slices x particles stay within kP2PSliceParticles (a 16^3 box: 8,
+// 32^3: 4, 64^3 and more: 1)
+constexpr int kP2PSliceMax = 8;
+constexpr int64_t kP2PSliceParticles = (int64_t)1 << 17;
+
+__host__ __device__ inline int p2p_slices(int64_t nb)
+{
+    int ns = 1;
+    while (ns < kP2PSliceMax && 2 * ns * nb <= kP2PSliceParticles) ns *= 2;
+    return ns;
+}
+
+// A P2P workgroup's target leaf, its chunk of 64 of the leaf's particles and its slice of
+// the leaf's list: a.nslice slices (the slowest index) of a.nchunk consecutive workgroups
+// per launch slot
+__device__ __forceinline__ int p2p_work(const P2PArgs &a, int &chunk, int &slice)
+{
+    const int nws = gridDim.x / a.nslice;  // workgroups per slice
+    slice = blockIdx.x / nws;
+    const int w = blockIdx.x - slice * nws;
+    chunk = w % a.nchunk;
+    return a.order[p2p_target(w / a.nchunk, nws / a.nchunk)];
+}
+
+// the slices of the lists of a box's target leaves
+__device__ __forceinline__ int p2p_box_slices(const P2PArgs &a, int b)
+{
+    return p2p_slices(a.box_off[b + 1] - a.box_off[b]);
+}
+
+// the entries [b, e) of target leaf T's P2P list that slice `slice` of ns takes
+__device__ __forceinline__ void p2p_slice(const P2PArgs &a, int T, int slice, int ns, int64_t &b, int64_t &e)
+{
+    const int64_t b0 = span_beg(a.off, T), len = span_end(a.off, T) - b0;
+    b = b0 + len * slice / ns;
+    e = b0 + len * (slice + 1) / ns;
+}
+
+// a target's force: stored, or its slice's part (of ns), summed by k_p2p_merge
+__device__ __forceinline__ void p2p_out(const P2PArgs &a, int slice, int ns, int it, double ax, double ay,
+                                        double az)
+{
+    if (ns == 1) {
+        store_acc(a, it, ax, ay, az);
+        return;
+    }
+    double *q = a.part + 3 * ((size_t)slice * a.n + it);
+    q[0] = ax;
+    q[1] = ay;
+    q[2] = az;
 }
 
 // Launch order of the P2P workgroups: target leaves by decreasing work (list length times
@@ -509,15 +567,17 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     __shared__ float stage[3 * STAGE];
     float *const sx = stage, *const sy = stage + STAGE, *const sz = stage + 2 * STAGE;
     __shared__ SplitRow32 tab[kSplitN32 + 1];
-    // a.nchunk consecutive workgroups per target leaf, one per chunk of 64 of its
-    // particles (one chunk for leaves of up to 64)
-    const int chunk = blockIdx.x % a.nchunk;
-    const int T = a.order[p2p_target(blockIdx.x / a.nchunk, gridDim.x / a.nchunk)];
+    // a workgroup per chunk of 64 of the target leaf's particles (one chunk for leaves of
+    // up to 64) and slice of its list
+    int chunk, slice;
+    const int T = p2p_work(a, chunk, slice);
     const int lane = threadIdx.x;
     if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
     const int c0 = 64 * chunk;
     if (c0 >= IT.npart) return;
+    const int ns = p2p_box_slices(a, a.bx.cell_box[T]);
+    if (slice >= ns) return;
     const int n = min(64, IT.npart - c0), i0 = IT.ipart + c0;  // this chunk's targets
     // the split-factor table into LDS (read after the first barrier below)
     for (int j = lane; j <= kSplitN32; j += 64) tab[j] = reinterpret_cast<const SplitRow32 *>(kSplitTab32)[j];
@@ -537,8 +597,9 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
     pc.ieps = (float)fmin(1.0 / (a.soft * lam_d), 1e12);
     pc.vmax = (float)kSplitN32;
     float axA = 0.f, ayA = 0.f, azA = 0.f, axB = 0.f, ayB = 0.f, azB = 0.f;
-    const int64_t e1 = span_end(a.off, T);
-    for (int64_t base = span_beg(a.off, T); base < e1;) {
+    int64_t b1, e1;
+    p2p_slice(a, T, slice, ns, b1, e1);
+    for (int64_t base = b1; base < e1;) {
         const int nb = (int)min((int64_t)64, e1 - base);
         // entry `lane` of the batch: its centre offset, its particle count and the slot
         // offset of its first particle (in registers; other lanes read them by shuffle)
@@ -656,7 +717,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
         by *= m;
         bz *= m;
         const int it = i0 + t;
-        if (a.with_l2p) {
+        if (a.with_l2p && slice == 0) {
             constexpr int NT = nterms(P);
             const float *Lp = reinterpret_cast<const float *>(a.L) + (size_t)T * NT;
             float Lr[NT];
@@ -670,7 +731,7 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(6))) k_
             by += ly * is2;
             bz += lz * is2;
         }
-        store_acc(a, it, (double)bx, (double)by, (double)bz);
+        p2p_out(a, slice, ns, it, (double)bx, (double)by, (double)bz);
     }
 }
 
@@ -738,15 +799,16 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     __shared__ double ecx[64], ecy[64], ecz[64];
     __shared__ int eincl[64], ecnt[64], eip[64];
     __shared__ SplitRow64 tab[kSplitW64 * (kSplitN64 + 1)];
-    // a.nchunk consecutive workgroups per target leaf, one per chunk of 64 of its
-    // particles (one chunk for leaves of up to 64)
-    const int chunk = blockIdx.x % a.nchunk;
-    const int T = a.order[p2p_target(blockIdx.x / a.nchunk, gridDim.x / a.nchunk)];
+    // a workgroup per chunk of 64 of the target leaf's particles and slice of its list
+    int chunk, slice;
+    const int T = p2p_work(a, chunk, slice);
     const int lane = threadIdx.x;
     if (a.sz->overflow || T >= a.sz->nleaf) return;
     const CellInfo IT = a.info[T];
     const int c0 = 64 * chunk;
     if (c0 >= IT.npart) return;
+    const int ns = p2p_box_slices(a, a.bx.cell_box[T]);
+    if (slice >= ns) return;
     const int n = min(64, IT.npart - c0), i0 = IT.ipart + c0;  // this chunk's targets
     // the split-factor table into LDS (read after the first barrier below)
     for (int j = lane; j < kSplitW64 * (kSplitN64 + 1); j += 64)
@@ -758,8 +820,9 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
     const double inv2rs = a.inv2rs, vscale = inv2rs * (kSplitTab64N / kSplitTab64U);
     const double boxlen = a.bx.len_of(T);
     double axA = 0.0, ayA = 0.0, azA = 0.0, axB = 0.0, ayB = 0.0, azB = 0.0;
-    const int64_t e1 = span_end(a.off, T);
-    for (int64_t base = span_beg(a.off, T); base < e1;) {
+    int64_t b1, e1;
+    p2p_slice(a, T, slice, ns, b1, e1);
+    for (int64_t base = b1; base < e1;) {
         const int nb = (int)min((int64_t)64, e1 - base);
         int cnt = 0;
         if (lane < nb) {
@@ -833,7 +896,7 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
             }
         }
         const int it = i0 + t;
-        if (a.with_l2p) {
+        if (a.with_l2p && slice == 0) {
             constexpr int NT = nterms(P);
             const double *Lp = reinterpret_cast<const double *>(a.L) + (size_t)T * NT;
             double Lr[NT];
@@ -846,8 +909,26 @@ __global__ void __launch_bounds__(64) k_p2p_f64(P2PArgs a)
             by += ly;
             bz += lz;
         }
-        store_acc(a, it, bx, by, bz);
+        p2p_out(a, slice, ns, it, bx, by, bz);
     }
+}
+
+// the slices' parts of the forces of the particles of sliced boxes, summed in slice order
+// and stored
+__global__ void k_p2p_merge(P2PArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (a.sz->overflow || i >= a.n) return;
+    const int ns = p2p_box_slices(a, box_of_slot(a.box_off, a.nbox, i));
+    if (ns == 1) return;  // stored by the P2P kernel
+    double ax = 0.0, ay = 0.0, az = 0.0;
+    for (int k = 0; k < ns; k++) {
+        const double *q = a.part + 3 * ((size_t)k * a.n + i);
+        ax += q[0];
+        ay += q[1];
+        az += q[2];
+    }
+    store_acc(a, (int)i, ax, ay, az);
 }
 
 // ---------------------------------------------------------------------------- host side
@@ -1045,14 +1126,27 @@ void p2p_l2p_launch(Ctx &c, bool with_l2p, bool accumulate, bool caller_order, d
     a.order = dptr<int32_t>(c.p2p_order);
     const bool f32 = c.prm.precision == PHOTONS_FP32_MIXED;
     hipStream_t st = c.stream;
-    if ((int64_t)nlb * a.nchunk >= ((int64_t)1 << 31)) throw std::runtime_error("too many P2P workgroups");
-    const int nwg = nlb * a.nchunk;
+    // Few particles (few target leaves, a long list each) leave most CUs idle: the list of
+    // each target leaf of a small box is cut into slices, one workgroup each, whose parts
+    // k_p2p_merge sums.  The slices depend on the box's particle count alone, so that a box
+    // gives the same forces bit for bit alone or with other boxes, planned or not.
+    a.nbox = c.nbox;
+    a.box_off = dptr<int64_t>(c.d_box_off);
+    a.n = c.n;
+    a.nslice = 1;
+    for (int b = 0; b < c.nbox; b++) a.nslice = std::max(a.nslice, p2p_slices(c.box_off[b + 1] - c.box_off[b]));
+    a.part = a.nslice > 1 ? static_cast<double *>(ctx_buf(c, c.p2p_part, sizeof(double) * 3 * (size_t)a.nslice * c.n,
+                                                           "P2P slices"))
+                          : nullptr;
+    if ((int64_t)nlb * a.nchunk * a.nslice >= ((int64_t)1 << 31)) throw std::runtime_error("too many P2P workgroups");
+    const int nwg = nlb * a.nchunk * a.nslice;
 #define P2P_L(PP)                                              \
     if (f32 && c.p2p_stage == 256) k_p2p_f32<PP, 256><<<nwg, 64, 0, st>>>(a); \
     else if (f32) k_p2p_f32<PP, 512><<<nwg, 64, 0, st>>>(a);          \
     else k_p2p_f64<PP><<<nwg, 64, 0, st>>>(a);
     if (nlb > 0) PH_DISPATCH(c.prm.order, P2P_L)
 #undef P2P_L
+    if (nlb > 0 && a.nslice > 1) k_p2p_merge<<<nblk(c.n, 256), 256, 0, st>>>(a);
     PH_CHECK(hipGetLastError());
 }
 

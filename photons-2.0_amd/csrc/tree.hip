@@ -462,17 +462,6 @@ __global__ void k_root_starts(int nbox, const DevSizes *__restrict__ sz, const i
     }
 }
 
-// box owning particle slot i: the last b with box_off[b] <= i
-__device__ __forceinline__ int box_of_slot(const int64_t *__restrict__ box_off, int nbox, int64_t i)
-{
-    int lo = 0, hi = nbox - 1;
-    while (lo < hi) {
-        int mid = (lo + hi + 1) >> 1;
-        if (box_off[mid] <= i) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
 
 __global__ void k_emit_nodes(const DevSizes *__restrict__ sz, const int *__restrict__ dfs,
                              const int *__restrict__ lfirst, const int *__restrict__ size, TreeTmp T,
