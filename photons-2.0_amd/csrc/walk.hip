@@ -221,20 +221,15 @@ __device__ __forceinline__ int visit(const WalkArgs &a, const TgtRec &t, const W
                                      int64_t qm, int64_t qd, uint32_t lp, uint32_t lm, uint32_t ld)
 {
     const int act = decide(a, t, R, r, sid, sx, sy, sz);
-    if (act == kActP2P) {
-        if (WRITE && np < lp) a.p2p_out[qp + np] = pack_entry(R.cell, sid);
-        np++;
-    } else if (act == kActM2L) {
-        if (WRITE && nm < lm) a.m2l_out[qm + nm] = pack_entry(R.cell, sid);
-        nm++;
-    } else if (act == kActOpenA) {
-        if (WRITE && nd < ld) {
+    const bool isp = act == kActP2P, ism = act == kActM2L, isd = act == kActOpenA, iss = act == kActSelfNode;
+    if (WRITE) {
+        if (isp && np < lp) a.p2p_out[qp + np] = pack_entry(R.cell, sid);
+        if (ism && nm < lm) a.m2l_out[qm + nm] = pack_entry(R.cell, sid);
+        if (isd && nd < ld) {
             const bool lf = R.dsize == 1;
             write_down(a, qd + nd, lf ? R.cell : r, lf, sid, t);
         }
-        nd++;
-    } else if (act == kActSelfNode) {
-        if (WRITE && nd + 1 < ld) {
+        if (iss && nd + 1 < ld) {
             const int nleaf = a.sz->nleaf;
 #pragma unroll
             for (int k = 0; k < 2; k++) {
@@ -242,8 +237,13 @@ __device__ __forceinline__ int visit(const WalkArgs &a, const TgtRec &t, const W
                 write_down(a, qd + nd + k, lf ? t.son[k] : t.sonr[k], lf, 0, t);
             }
         }
-        nd += 2;
     }
+    // the counts by arithmetic: increments under a branch each were merged by the
+    // compiler into one increment through a selected address, which put the three
+    // counters in scratch memory
+    np += isp ? 1u : 0u;
+    nm += ism ? 1u : 0u;
+    nd += isd ? 1u : (iss ? 2u : 0u);
     if (act != kActOpenB) return r + R.dsize;
     if (!t.leaf) return r + 1;
     // a target leaf against a leaf takes P2P whatever the geometry: the opened node's leaf
@@ -432,23 +432,23 @@ __device__ __forceinline__ void leaf_source_pairs(const WalkArgs &a, int nleaf, 
     }
 #pragma unroll
     for (int i = 0; i < 2; i++) {
-        np[i] = nm[i] = nd[i] = 0;
-        if (!use[i]) continue;
-        const int cell = entry_cell(D[i].entry), sid = entry_shift(D[i].entry);
-        if (!node[i]) {
-            np[i] = 1;
-            continue;
+        int act = kActNone;
+        if (node[i]) {
+            const int sid = entry_shift(D[i].entry);
+            WalkRec R;
+            R.g = g[i];
+            R.cell = entry_cell(D[i].entry);
+            R.dsize = 1;
+            R.kid[0] = R.kid[1] = -1;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            if (sid != 0) shift_of(sid, t[i].L, sx, sy, sz);
+            act = decide(a, t[i], R, -1, sid, sx, sy, sz);  // rank -1: not the target itself
         }
-        WalkRec R;
-        R.g = g[i];
-        R.cell = cell;
-        R.dsize = 1;
-        R.kid[0] = R.kid[1] = -1;
-        double sx = 0.0, sy = 0.0, sz = 0.0;
-        if (sid != 0) shift_of(sid, t[i].L, sx, sy, sz);
-        const int act = decide(a, t[i], R, -1, sid, sx, sy, sz);  // rank -1: not the target itself
-        if (act == kActM2L) nm[i] = 1;
-        else if (act == kActOpenA) nd[i] = 1;
+        // (selects: stores under branches were merged into one store through a selected
+        // address, which put the counters in scratch memory)
+        np[i] = use[i] && !node[i] ? 1u : 0u;
+        nm[i] = act == kActM2L ? 1u : 0u;
+        nd[i] = act == kActOpenA ? 1u : 0u;
     }
 }
 
@@ -566,7 +566,9 @@ __device__ __forceinline__ SuperChunk super_chunk(const WalkArgs &a, int64_t s, 
     sc.pre[0] = 0;
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        const int c0 = __shfl(cnt, j, 64), c1 = __shfl(cnt, 2 + j, 64);
+        // the same in every lane: kept in scalar registers
+        const int c0 = __builtin_amdgcn_readfirstlane(__shfl(cnt, j, 64));
+        const int c1 = __builtin_amdgcn_readfirstlane(__shfl(cnt, 2 + j, 64));
         sc.n0[j] = c0;
         sc.m[j] = min(c0, c1);
         sc.pre[j + 1] = sc.pre[j] + c0 + c1;
@@ -578,14 +580,15 @@ __device__ __forceinline__ SuperChunk super_chunk(const WalkArgs &a, int64_t s, 
 // first and second stream alternate while both have pairs, then the longer one's rest
 __device__ __forceinline__ TravRec super_pair(const WalkArgs &a, const SuperChunk &sc, int idx)
 {
+    // (selects, not indexing: an indexed array would live in scratch memory)
     const int j = idx >= sc.pre[1] ? 1 : 0;
-    const int r = idx - sc.pre[j], m = sc.m[j];
+    const int r = idx - (j ? sc.pre[1] : sc.pre[0]), m = j ? sc.m[1] : sc.m[0];
     int st, i;
     if (r < 2 * m) {
         st = r & 1;
         i = r >> 1;
     } else {
-        st = sc.n0[j] > m ? 0 : 1;
+        st = (j ? sc.n0[1] : sc.n0[0]) > m ? 0 : 1;
         i = m + (r - 2 * m);
     }
     return a.tlist[(st * sc.H + sc.k0 + j) * kTile + i];
@@ -638,11 +641,11 @@ __device__ __forceinline__ void stream_advance(const WalkArgs &a, TravStream &ts
 // one lane only; each step of the loop is one rule (one record load) per busy lane.  A
 // traversal that reaches kThreadVisits steps is handed to k_walk_deep_count through one
 // of the wavefront's continuation slots; when they are used up, its lane goes on alone.
-__global__ void __launch_bounds__(kWalkThreads) k_walk_count_trav(WalkArgs a)
+__global__ void __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(6))) k_walk_count_trav(WalkArgs a)
 {
     const int lane = threadIdx.x % 64;
     const uint64_t below = (1ull << lane) - 1ull;
-    const int64_t gw = (int64_t)blockIdx.x * kTileWaves + threadIdx.x / 64;
+    const int64_t gw = (int64_t)blockIdx.x * kTileWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     Cont *slots = a.cont + gw * kContPerWave;
     int ncont = 0;  // slots in use (the same in every lane)
     TravStream ts;
@@ -882,12 +885,12 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_write(WalkArgs a, const C
 // Write pass, the listed pairs, as in k_walk_count_trav, storing at the offsets the tile
 // pass left; a continued traversal (flagged in its pair's word) stops where the count
 // pass handed it on and leaves where its rest goes.
-__global__ void __launch_bounds__(kWalkThreads) k_walk_write_trav(WalkArgs a, const Cnt3 *__restrict__ tile_off)
+__global__ void __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(5))) k_walk_write_trav(WalkArgs a, const Cnt3 *__restrict__ tile_off)
 {
     if (a.sz->overflow) return;
     const int lane = threadIdx.x % 64;
     const uint64_t below = (1ull << lane) - 1ull;
-    const int64_t gw = (int64_t)blockIdx.x * kTileWaves + threadIdx.x / 64;
+    const int64_t gw = (int64_t)blockIdx.x * kTileWaves + __builtin_amdgcn_readfirstlane(threadIdx.x / 64);
     const int64_t p2p_base = a.sz->p2p_base, m2l_base = a.sz->m2l_base;
     TravStream ts;
     stream_open(a, ts, gw);
@@ -939,7 +942,7 @@ __global__ void __launch_bounds__(kWalkThreads) k_walk_write_trav(WalkArgs a, co
 }
 
 // the continued traversals' outputs, where the write pass left them
-__global__ void __launch_bounds__(256) k_walk_deep_write(WalkArgs a)
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) k_walk_deep_write(WalkArgs a)
 {
     constexpr int G = kGroupLanes;
     if (a.sz->overflow) return;
