@@ -185,8 +185,8 @@ constexpr int kWalkThreads = 64 * kTileWaves;
 // on (a target leaf beside a dense cluster opens one source node after another) is
 // continued from the rank it reached by a group of kGroupLanes lanes, that many ranks at
 // a time.
-constexpr int kThreadVisits = 64;
-constexpr int kGroupLanes = 16;
+constexpr int kThreadVisits = 32;
+constexpr int kGroupLanes = 64;
 
 // A pair's outputs as one 64-bit word: bit 0 clear, then down (19 bits), M2L (20), P2P
 // (24); bit 0 set: the pair was continued, bits 1.. hold its continuation's slot.  A
@@ -279,12 +279,12 @@ __device__ __forceinline__ void group_traverse(const WalkArgs &a, const TgtRec &
                                                int64_t qm = 0, int64_t qd = 0, uint32_t lp = 0, uint32_t lm = 0,
                                                uint32_t ld = 0)
 {
-    static_assert(G <= 32 && (G & (G - 1)) == 0, "a power-of-two group of at most 32 lanes");
+    static_assert(G <= 64 && (G & (G - 1)) == 0, "a power-of-two group of at most 64 lanes");
     double sx = 0.0, sy = 0.0, sz = 0.0;
     if (sid != 0) shift_of(sid, t.L, sx, sy, sz);
     const int gshift = (int)(threadIdx.x % 64) / G * G;  // the group's first lane in the wavefront
-    const uint32_t gmask = G == 32 ? 0xffffffffu : ((1u << G) - 1u);
-    const uint32_t below = (1u << gl) - 1u;
+    const uint64_t gmask = G == 64 ? ~0ull : ((1ull << G) - 1ull);
+    const uint64_t below = (1ull << gl) - 1ull;
     int carry = r;
     while (carry < rend) {
         const int s = carry + gl;
@@ -308,19 +308,19 @@ __device__ __forceinline__ void group_traverse(const WalkArgs &a, const TgtRec &
         const bool visited = s < rend && s >= excl;
         const bool isp = visited && act == kActP2P, ism = visited && act == kActM2L;
         const bool isd = visited && act == kActOpenA;
-        const uint32_t bp = (uint32_t)(__ballot(isp) >> gshift) & gmask;
-        const uint32_t bm = (uint32_t)(__ballot(ism) >> gshift) & gmask;
-        const uint32_t bd = (uint32_t)(__ballot(isd) >> gshift) & gmask;
+        const uint64_t bp = (__ballot(isp) >> gshift) & gmask;
+        const uint64_t bm = (__ballot(ism) >> gshift) & gmask;
+        const uint64_t bd = (__ballot(isd) >> gshift) & gmask;
         if (WRITE) {
-            const uint32_t ip = np + __popc(bp & below), im = nm + __popc(bm & below);
-            const uint32_t id = nd + __popc(bd & below);
+            const uint32_t ip = np + __popcll(bp & below), im = nm + __popcll(bm & below);
+            const uint32_t id = nd + __popcll(bd & below);
             if (isp && ip < lp) a.p2p_out[qp + ip] = pack_entry(v, sid);
             if (ism && im < lm) a.m2l_out[qm + im] = pack_entry(v, sid);
             if (isd && id < ld) write_down(a, qd + id, lf ? v : s, lf, sid, t);
         }
-        np += __popc(bp);
-        nm += __popc(bm);
-        nd += __popc(bd);
+        np += __popcll(bp);
+        nm += __popcll(bm);
+        nd += __popcll(bd);
         carry = max(carry + G, __shfl(incl, G - 1, G));
     }
 }
