@@ -42,7 +42,7 @@ P2P_FLOPS = 27  # per particle-particle interaction
 # HBM bytes per FP32 P2P launch (k_p2p_f32<4, 256>) at 256^3 clustered, p = 4, from
 # rocprofv3 --pmc FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from a 1 GiB copy in the
 # same run); the counts and commands are kept with the P2P counter summary under profiles/
-P2P_FP32_TRAFFIC_256 = (2.0 * 13181521 + 793589) * 1024
+P2P_FP32_TRAFFIC_256 = (2.0 * 13064333 + 793660) * 1024
 
 
 def log(msg):
