@@ -318,8 +318,8 @@ constexpr int64_t kP2PSliceParticles = (int64_t)1 << 17;
 __host__ __device__ inline int p2p_slices(int64_t nb)
 {
     int ns = 1;
-    while (ns < kP2PSliceMax && 2 * ns * nb <= kP2PSliceParticles) ns *= 2;
-    return ns;
+    while (nb > 0 && ns < kP2PSliceMax && 2 * ns * nb <= kP2PSliceParticles) ns *= 2;
+    return ns;  // an empty box has no leaves to slice
 }
 
 // A P2P workgroup's target leaf, its chunk of 64 of the leaf's particles and its slice of
