@@ -184,7 +184,10 @@ constexpr int kWalkThreads = 64 * kTileWaves;
 // A lane applies the rule to at most kThreadVisits ranks of a pair; a traversal that goes
 // on (a target leaf beside a dense cluster opens one source node after another) is
 // continued from the rank it reached by a group of kGroupLanes lanes, that many ranks at
-// a time.
+// a time.  A whole wavefront per group loads 64 consecutive 64-byte records at once, where
+// single lanes each chase their own subtree: at 256^3 groups of 8, 16, 32 and 64 lanes
+// walked in 54, 48, 44 and 42 ms, and handing on after 32 steps rather than 64 (or 16)
+// gained another half millisecond.
 constexpr int kThreadVisits = 32;
 constexpr int kGroupLanes = 64;
 
