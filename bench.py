@@ -7,7 +7,7 @@ Per-phase GPU times come from HIP events the library records on its own stream.
 
     python bench.py [--gpus 1] [--steps 5] [--warmup 2] [--nside 256] [--order 4]
 
-Three more workloads print their own JSON line (not the default):
+Four more workloads print their own JSON line (not the default):
     --workload ensemble   64 independent clustered boxes of 32^3..128^3 particles (log-uniform,
                           seed 5) in one batched evaluation, leaf size and P2P staging autotuned
     --workload serving    256 resident boxes of 16^3..192^3 particles (seed 6), single-step
@@ -17,6 +17,8 @@ Three more workloads print their own JSON line (not the default):
                           FP32-mixed against FP64 error of a served step
     --workload treepm     whole TreePM steps (kick, drift, short-range and PM forces, kick) of the
                           --nside^3 clustered box, particles resident in HBM
+    --workload small      latency of one small clustered box (--small-sides, default 16^3 and
+                          32^3): ordinary evaluations against the replayed hipGraph
 """
 from __future__ import annotations
 
@@ -293,10 +295,11 @@ def run_ensemble(args, torch):
 
 
 def small_box_latency(torch, args):
-    """one clustered box of 16^3 / 32^3 particles: ordinary evaluation (host round trips
-    per tree level and walk depth) against the replayed graph, wall clock per evaluation"""
+    """one clustered box of 16^3 / 32^3 particles (--small-sides): ordinary evaluations
+    (planned from the previous one, a wait at the end of each) against the replayed
+    graph, wall clock per evaluation"""
     out = {}
-    for ns in (16, 32):
+    for ns in args.small_sides:
         n = ns ** 3
         box = 500000.0 * (n / 256.0 ** 3) ** (1.0 / 3.0)
         pos, _ = ic.clustered(n, box, seed=1)
@@ -325,6 +328,21 @@ def small_box_latency(torch, args):
         out[f"{ns}^3"] = {"ordinary": t_ord, "graph": t_graph}
         log(f"{ns}^3 box: ordinary {t_ord:.3f} ms, replayed graph {t_graph:.3f} ms per evaluation")
     return out
+
+
+def run_small(args, torch):
+    """the small-box latencies alone (FP32-mixed)"""
+    small = small_box_latency(torch, args)
+    ns = args.small_sides[0]
+    ms = small[f"{ns}^3"]["graph"]
+    return {
+        "metric": METRIC, "value": ns ** 3 / (ms * 1e-3), "unit": "particles/s", "n_gpus": 1,
+        "steps": 50, "warmup": 5, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-mixed", "data": "synthetic",
+        "config": {"workload": f"one clustered box of {ns}^3 particles at the density of the 256^3 workload, "
+                               "planned and replayed as a hipGraph", "particles": ns ** 3, "order": args.order},
+        "small_box_latency_ms": small,
+    }
 
 
 def run_treepm(args, torch):
@@ -513,7 +531,10 @@ def main():
     ap.add_argument("--order", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["fp32", "fp64", "both"], default="both")
-    ap.add_argument("--workload", choices=["clustered", "ensemble", "serving", "treepm"], default="clustered")
+    ap.add_argument("--workload", choices=["clustered", "ensemble", "serving", "treepm", "small"],
+                    default="clustered")
+    ap.add_argument("--small-sides", type=int, nargs="+", default=[16, 32],
+                    help="particles per side of the small boxes (latency)")
     ap.add_argument("--serve-boxes", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=4.0, help="arrival span per rate")
     ap.add_argument("--rates32", type=float, nargs="+", default=[4, 8, 16, 32, 64, 128, 256, 512])
@@ -543,7 +564,8 @@ def main():
         log(f"copy calibration: {nbytes} bytes read and written once")
         del src, dst
     if args.workload != "clustered":
-        fn = {"ensemble": run_ensemble, "serving": run_serving, "treepm": run_treepm}[args.workload]
+        fn = {"ensemble": run_ensemble, "serving": run_serving, "treepm": run_treepm,
+              "small": run_small}[args.workload]
         print(json.dumps(fn(args, torch)), flush=True)
         return
     log("generating particles")
