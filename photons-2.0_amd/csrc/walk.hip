@@ -137,6 +137,7 @@ struct WalkArgs {
     uint32_t *qoff;           // [3 per pair]: a listed pair's output offsets from its tile's base
     int trav_waves;           // wavefronts of the traversal passes (a multiple of kTileWaves)
     int trav_split;           // slices per super-chunk handed to the traversal wavefronts
+    int visits;               // steps of a traversal on its lane before it is continued
     Cont *cont;               // continued traversals: kContPerWave slots per traversal wavefront
     int32_t *ccount;          // per traversal wavefront, its slots in use
     const int32_t *coff;      // their exclusive scan
@@ -189,6 +190,11 @@ constexpr int kWalkThreads = 64 * kTileWaves;
 // walked in 54, 48, 44 and 42 ms, and handing on after 32 steps rather than 64 (or 16)
 // gained another half millisecond.
 constexpr int kThreadVisits = 32;
+// at a depth of few records (a small box, the first depths of any) the traversal passes
+// run few wavefronts whose lanes each walk their records one dependent load at a time:
+// there a traversal is handed on after kThreadVisitsFew steps (a 16^3 box's replay
+// 1.45 -> 1.31 ms, 32^3 2.52 -> 2.13 ms; 8 steps: 1.35, 2.17 ms)
+constexpr int kThreadVisitsFew = 4;
 constexpr int kGroupLanes = 64;
 
 // A pair's outputs as one 64-bit word: bit 0 clear, then down (19 bits), M2L (20), P2P
@@ -642,7 +648,7 @@ __device__ __forceinline__ void stream_advance(const WalkArgs &a, TravStream &ts
 // Count pass, the listed pairs (a node source): super-chunks per wavefront; lanes take the
 // pairs in turn and a lane that finishes one takes the next, so a long traversal holds up
 // one lane only; each step of the loop is one rule (one record load) per busy lane.  A
-// traversal that reaches kThreadVisits steps is handed to k_walk_deep_count through one
+// traversal that reaches a.visits steps is handed to k_walk_deep_count through one
 // of the wavefront's continuation slots; when they are used up, its lane goes on alone.
 __global__ void __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_eu(6))) k_walk_count_trav(WalkArgs a)
 {
@@ -673,7 +679,7 @@ __global__ void __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_
             L.r = visit<false>(a, L.t, L.R, L.r, L.sid, L.sx, L.sy, L.sz, L.np, L.nm, L.nd, 0, 0, 0, 0, 0, 0);
             L.visits++;
             if (L.r >= L.rend) fin = true;
-            else if (L.visits == kThreadVisits) cap = true;
+            else if (L.visits == a.visits) cap = true;
         }
         const uint64_t bh = __ballot(cap);
         const bool room = ncont + __popcll(bh) <= kContPerWave;  // the same in every lane
@@ -929,7 +935,7 @@ __global__ void __launch_bounds__(kWalkThreads) __attribute__((amdgpu_waves_per_
             L.visits++;
             if (L.r >= L.rend) {
                 L.q = -1;
-            } else if (L.visits == kThreadVisits && (wd & 1u)) {
+            } else if (L.visits == a.visits && (wd & 1u)) {
                 // the rest is k_walk_deep_write's, from here (flagged by the count pass at
                 // the same step)
                 Cont &ct = a.cont[wd >> 1];
@@ -1168,6 +1174,7 @@ void run_walk_from(Ctx &c, int shift_lo, int shift_hi, int start_target, int sta
         const int64_t nunit_cap = nsuper_cap * split;
         const int tw = (int)std::min<int64_t>(kTravWavesMax, (nunit_cap + kTileWaves - 1) / kTileWaves * kTileWaves);
         a.trav_split = split;
+        a.visits = split > 1 ? kThreadVisitsFew : kThreadVisits;
         const int nslot = tw * kContPerWave;
         a.trav_waves = tw;
         a.cont = static_cast<Cont *>(ctx_buf(c, c.big_pairs, sizeof(Cont) * (size_t)nslot, "walk continuations"));
